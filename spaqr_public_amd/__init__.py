@@ -1,0 +1,11 @@
+"""spaqr_public_amd — MI355X-native spaQR (sparsified hierarchical QR).
+
+The hot path of the reference (Tree::factorize per-level phases and
+Tree::solve inside GMRES) runs as batched gfx950 HIP kernels behind the C ABI
+of include/spaqr.h; this package is the thin Python binding of that library.
+"""
+from .core import (FP32, FP64, Matrix, SpaqrError, Tree, colscale, default_levels, gmres, gmres_device,
+                   load_library, random_vec, run_flow)
+
+__all__ = ["FP32", "FP64", "Matrix", "SpaqrError", "Tree", "colscale", "default_levels", "gmres",
+           "gmres_device", "load_library", "random_vec", "run_flow"]

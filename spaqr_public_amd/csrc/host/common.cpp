@@ -1,0 +1,75 @@
+#include "common.hpp"
+
+#include <cstring>
+
+namespace spq {
+
+void* Arena::alloc(size_t bytes, size_t align) {
+  if (bytes == 0) bytes = align;
+  while (cur_ < chunks_.size()) {
+    Chunk& c = chunks_[cur_];
+    size_t off = (c.off + align - 1) / align * align;
+    if (off + bytes <= c.size) {
+      c.off = off + bytes;
+      used_ += bytes;
+      return c.p + off;
+    }
+    ++cur_;
+  }
+  size_t sz = bytes > chunk_ ? (bytes + align) : chunk_;
+  void* p = nullptr;
+  hipError_t e = hipMalloc(&p, sz);
+  if (e != hipSuccess)
+    throw Error(kErrOom, "device allocation of " + std::to_string(sz) + " bytes failed: " + hipGetErrorString(e));
+  chunks_.push_back({(char*)p, sz, 0});
+  reserved_ += sz;
+  cur_ = chunks_.size() - 1;
+  Chunk& c = chunks_.back();
+  c.off = bytes;
+  used_ += bytes;
+  return c.p;
+}
+
+void Arena::rewind() {
+  for (auto& c : chunks_) c.off = 0;
+  cur_ = 0;
+  used_ = 0;
+}
+
+void Arena::release() {
+  for (auto& c : chunks_) hipFree(c.p);
+  chunks_.clear();
+  cur_ = 0;
+  used_ = 0;
+  reserved_ = 0;
+}
+
+Stager::~Stager() {
+  if (host_) hipHostFree(host_);
+  if (dev_) hipFree(dev_);
+}
+
+void Stager::init(hipStream_t s, size_t bytes) {
+  stream_ = s;
+  cap_ = bytes;
+  SPQ_HIP(hipHostMalloc((void**)&host_, bytes, hipHostMallocDefault));
+  SPQ_HIP(hipMalloc((void**)&dev_, bytes));
+  off_ = 0;
+}
+
+const void* Stager::push_raw(const void* src, size_t bytes) {
+  if (bytes == 0) return nullptr;
+  size_t need = (bytes + 255) / 256 * 256;
+  if (need > cap_) throw Error(kErrOom, "descriptor table larger than the staging ring");
+  if (off_ + need > cap_) {
+    SPQ_HIP(hipStreamSynchronize(stream_));
+    off_ = 0;
+  }
+  std::memcpy(host_ + off_, src, bytes);
+  SPQ_HIP(hipMemcpyAsync(dev_ + off_, host_ + off_, bytes, hipMemcpyHostToDevice, stream_));
+  const void* d = dev_ + off_;
+  off_ += need;
+  return d;
+}
+
+}  // namespace spq

@@ -1,0 +1,131 @@
+// Host-side infrastructure of the spaQR engine: separator / cluster IDs,
+// device memory arenas, the pinned descriptor stager and error handling.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <chrono>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace spq {
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+enum : int { kOk = 0, kErrHip = -1, kErrArg = -2, kErrNumeric = -3, kErrOom = -4, kErrState = -5 };
+
+#define SPQ_HIP(call)                                                                              \
+  do {                                                                                             \
+    hipError_t e_ = (call);                                                                        \
+    if (e_ != hipSuccess)                                                                          \
+      throw ::spq::Error(::spq::kErrHip, std::string("HIP error ") + hipGetErrorString(e_) + " at " + \
+                                             __FILE__ + ":" + std::to_string(__LINE__));            \
+  } while (0)
+
+inline double wall() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// Separator ID (level, index) and cluster ID (self, left, right) with the
+// reference's lexicographic order (cluster.h:22-111).
+struct SepID {
+  int lvl = -1, sep = 0;
+  SepID() {}
+  SepID(int l, int s) : lvl(l), sep(s) {}
+  bool operator==(const SepID& o) const { return lvl == o.lvl && sep == o.sep; }
+  bool operator!=(const SepID& o) const { return !(*this == o); }
+  bool operator<(const SepID& o) const { return lvl < o.lvl || (lvl == o.lvl && sep < o.sep); }
+};
+struct CID {
+  SepID self, l, r;
+  int part = 2, section = 0;
+  bool operator==(const CID& o) const {
+    return self == o.self && l == o.l && r == o.r && part == o.part && section == o.section;
+  }
+  bool operator!=(const CID& o) const { return !(*this == o); }
+  bool operator<(const CID& o) const {
+    if (self != o.self) return self < o.self;
+    if (l != o.l) return l < o.l;
+    if (r != o.r) return r < o.r;
+    if (section != o.section) return section < o.section;
+    return part < o.part;
+  }
+};
+// cluster.cpp:19-32
+inline SepID sep_merge(const SepID& s) { return SepID(s.lvl + 1, s.sep / 2); }
+inline CID cid_merge_if(const CID& c, int lvl) {
+  CID o;
+  o.self = c.self;
+  o.l = c.l.lvl < lvl ? sep_merge(c.l) : c.l;
+  o.r = c.r.lvl < lvl ? sep_merge(c.r) : c.r;
+  o.section = lvl > 0 ? c.section / 2 : c.section;
+  o.part = c.self.lvl == lvl ? 2 : c.part;
+  return o;
+}
+
+// Bump allocator over large hipMalloc'd chunks.  Permanent storage (factors,
+// blocks) lives until the arena is destroyed; a scratch arena is rewound
+// between phases (stream order guarantees earlier kernels are done with it).
+class Arena {
+ public:
+  explicit Arena(size_t chunk = size_t(256) << 20) : chunk_(chunk) {}
+  ~Arena() { release(); }
+  Arena(const Arena&) = delete;
+  Arena& operator=(const Arena&) = delete;
+  void* alloc(size_t bytes, size_t align = 256);
+  void rewind();  // keep chunks, reset offsets
+  void release();
+  size_t used() const { return used_; }
+  size_t reserved() const { return reserved_; }
+
+ private:
+  struct Chunk { char* p; size_t size, off; };
+  size_t chunk_;
+  std::vector<Chunk> chunks_;
+  size_t cur_ = 0, used_ = 0, reserved_ = 0;
+};
+
+// Pinned host staging + device ring for descriptor tables.  push() copies a
+// table into the pinned ring, enqueues the H2D copy on the stream and returns
+// the device address.  The ring is reset at host synchronisation points;
+// overflow forces a synchronisation.
+class Stager {
+ public:
+  Stager() {}
+  ~Stager();
+  void init(hipStream_t s, size_t bytes);
+  template <typename X>
+  const X* push(const std::vector<X>& v) {
+    return (const X*)push_raw(v.data(), sizeof(X) * v.size());
+  }
+  const void* push_raw(const void* src, size_t bytes);
+  void reset() { off_ = 0; }
+  // Persistent copy (for solve programs): allocated from `perm` and kept.
+  template <typename X>
+  X* upload(Arena& perm, const std::vector<X>& v) {
+    if (v.empty()) return nullptr;
+    X* d = (X*)perm.alloc(sizeof(X) * v.size());
+    SPQ_HIP(hipMemcpyAsync(d, v.data(), sizeof(X) * v.size(), hipMemcpyHostToDevice, stream_));
+    SPQ_HIP(hipStreamSynchronize(stream_));
+    return d;
+  }
+
+ private:
+  hipStream_t stream_ = nullptr;
+  char* host_ = nullptr;
+  char* dev_ = nullptr;
+  size_t cap_ = 0, off_ = 0;
+};
+
+// Prefix sums of per-descriptor tile counts (tstart table of tiled kernels).
+inline int prefix_tiles(const std::vector<int>& counts, std::vector<int>& start) {
+  start.resize(counts.size());
+  int s = 0;
+  for (size_t i = 0; i < counts.size(); ++i) { start[i] = s; s += counts[i]; }
+  return s;
+}
+
+}  // namespace spq

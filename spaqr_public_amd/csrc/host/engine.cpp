@@ -1,0 +1,1205 @@
+// spaQR engine implementation.  Reference behaviour cited per function
+// (tree.cpp line numbers).  Cluster sets that the reference keeps as
+// std::set<Cluster*> are kept sorted by creation sequence, which is the address
+// order of the reference's consecutive `new Cluster` allocations.
+#include "engine.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <numeric>
+
+#include "../kernels/launch.hpp"
+
+namespace spq {
+
+namespace {
+inline int iceil(int a, int b) { return (a + b - 1) / b; }
+}  // namespace
+
+template <typename T>
+class Engine final : public EngineBase {
+ public:
+  Engine(int nlevels, int skip, int device) : nlevels_(nlevels), skip_(skip), perm_(size_t(512) << 20),
+                                               scratch_(size_t(256) << 20), ws_(size_t(64) << 20) {
+    SPQ_HIP(hipSetDevice(device));
+    SPQ_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    stager_.init(stream_, size_t(64) << 20);
+    bottoms_.resize(nlevels);
+    info_.resize(nlevels);
+    nb_ = panel_nb();
+    SPQ_HIP(hipHostMalloc((void**)&rank_host_, sizeof(int) * 65536, hipHostMallocDefault));
+  }
+  ~Engine() override {
+    reset_kernel_stats();
+    if (graph_exec_fwd_) hipGraphExecDestroy(graph_exec_fwd_);
+    if (rank_host_) hipHostFree(rank_host_);
+    if (xbuf_) hipFree(xbuf_);
+    if (bin_) hipFree(bin_);
+    if (xout_) hipFree(xout_);
+    hipStreamDestroy(stream_);
+  }
+
+  void set_tol(double t) override { tol_ = t; }
+  void set_skip(int s) override { skip_ = s; }
+  void set_scale(int s) override { scale_ = s; }
+  void set_square(int s) override { square_ = s; }
+  void set_order(float o) override { order_ = o; }
+  void set_coords(const std::vector<double>& X, int dim) override { X_ = X; xdim_ = dim; geo_ = true; }
+  void set_verbose(int v) override { verbose_ = v; }
+  void set_timing(int t) override { timing_ = t; }
+  int rows() const override { return nrows_; }
+  int cols() const override { return ncols_; }
+  int levels() const override { return nlevels_; }
+  hipStream_t stream() const override { return stream_; }
+  const std::vector<int>& cperm() const override { return cperm_; }
+  const std::vector<int>& rperm() const override { return rperm_; }
+  const std::vector<LevelInfo>& level_info() const override { return info_; }
+  const std::vector<SparsifyInfo>& sparsified() const override { return sprs_; }
+  int precision_bytes() const override { return (int)sizeof(T); }
+
+  void set_kernel_timing(int on) override { kprof_ = on != 0; }
+  void reset_kernel_stats() override {
+    for (auto& kv : kev_)
+      for (auto& e : kv.second.ev) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
+    kev_.clear();
+  }
+  std::vector<KernelStat> kernel_stats() override {
+    SPQ_HIP(hipStreamSynchronize(stream_));
+    std::vector<KernelStat> out;
+    for (auto& kv : kev_) {
+      KernelStat k;
+      k.name = kv.first;
+      k.launches = (int)kv.second.ev.size();
+      k.flops = kv.second.flops;
+      k.bytes = kv.second.bytes;
+      for (auto& e : kv.second.ev) {
+        float ms = 0;
+        SPQ_HIP(hipEventElapsedTime(&ms, e.first, e.second));
+        k.ms += ms;
+      }
+      out.push_back(k);
+    }
+    return out;
+  }
+
+  void partition(const CscMat& A) override;
+  void assemble(const CscMat& A) override;
+  int factorize() override;
+  void solve_dev(const double* b, double* x) override;
+  void solve_nrml_dev(const double*, double*) override {
+    throw Error(kErrState, "solve_nrml: the least-squares apply is not available in this build");
+  }
+
+  std::vector<std::array<int, 12>> cluster_table() const override {
+    std::vector<std::array<int, 12>> t;
+    for (int l = 0; l < nlevels_; ++l)
+      for (int c : bottoms_[l]) {
+        const Clu& s = cl_[c];
+        t.push_back({l, s.id.self.lvl, s.id.self.sep, s.id.l.lvl, s.id.l.sep, s.id.r.lvl, s.id.r.sep, s.cstart,
+                     s.csize_org, s.rstart, s.rsize_org, s.order});
+      }
+    return t;
+  }
+  void counters(unsigned long long* nnz4, double* d8) const override {
+    nnz4[0] = nnzA_; nnz4[1] = nnzR_; nnz4[2] = nnzH_; nnz4[3] = nnzQ_;
+    d8[0] = flops_; d8[1] = t_partition_; d8[2] = t_assemble_; d8[3] = t_factorize_;
+    d8[4] = top_size_; d8[5] = (double)perm_.reserved(); d8[6] = factor_bytes_; d8[7] = solve_bytes_;
+  }
+
+ private:
+  struct Block {
+    T* p = nullptr;
+    int rows = 0, cols = 0, ld = 0;
+  };
+  struct Edge {
+    int n1, n2;
+    Block a;
+    bool alive = true;
+  };
+  struct Clu {
+    int cstart = 0, csize = 0, rstart = 0, rsize = 0, csize_org = 0, rsize_org = 0;
+    CID id;
+    int order = 0;
+    bool elim = false;
+    int parent = -1;
+    CID parentid;
+    std::vector<int> children;
+    int rpos = 0, cpos = 0;
+    std::vector<int> out, in;  // edge ids in reference list order
+    std::vector<int> rsp;      // rsparsity, creation order
+    std::vector<int> cnbrs;    // sorted, creation order
+    int64_t xoff = -1;
+    int xcap = 0;
+    int level() const { return id.self.lvl; }
+  };
+  // Solve op records (sizes captured at creation, operations.h / toperations.h)
+  enum OpKind { OpQR, OpScaleD, OpOrthoD, OpSplitD, OpMerge };
+  struct Op {
+    OpKind kind;
+    int level, c = -1, f = -1;
+    // reflector data
+    const T* V = nullptr;
+    const T* Tb = nullptr;
+    int ldv = 0, m = 0, k = 0;
+    std::vector<std::pair<int, int>> segs;  // (cluster, length)
+    // backward data
+    const T* R = nullptr;
+    int ldr = 0, kr = 0;
+    std::vector<std::pair<Block, std::pair<int, int>>> r12;  // block, (cluster, s)
+    // SplitD
+    int crows = 0, ccols = 0, rank = 0;
+  };
+
+  // ------------------------------------------------------------ helpers
+  int new_cluster(int cs, int cn, int rs, int rn, const CID& id) {
+    Clu c;
+    c.cstart = cs; c.csize = cn; c.rstart = rs; c.rsize = rn; c.csize_org = cn; c.rsize_org = rn;
+    c.id = id;
+    c.order = max_order_++;
+    cl_.push_back(c);
+    return (int)cl_.size() - 1;
+  }
+  void alloc_x(int c, int cap) {
+    cl_[c].xoff = xsize_;
+    cl_[c].xcap = cap;
+    xsize_ += cap;
+  }
+  T* alloc_perm(size_t n) { return (T*)perm_.alloc(sizeof(T) * std::max<size_t>(n, 1)); }
+  T* alloc_scr(size_t n) { return (T*)scratch_.alloc(sizeof(T) * std::max<size_t>(n, 1)); }
+  Block new_block(int rows, int cols) {
+    Block b;
+    b.rows = rows; b.cols = cols; b.ld = std::max(rows, 1);
+    b.p = alloc_perm((size_t)b.ld * std::max(cols, 1));
+    return b;
+  }
+  int add_edge(int c1, int c2, std::vector<Block>* zero_list) {  // tree.cpp:251-261
+    Edge e;
+    e.n1 = c1; e.n2 = c2;
+    e.a = new_block(cl_[c2].rsize, cl_[c1].csize);
+    if (zero_list) zero_list->push_back(e.a);
+    ed_.push_back(e);
+    const int id = (int)ed_.size() - 1;
+    auto& cn = cl_[c1].cnbrs;
+    auto it = std::lower_bound(cn.begin(), cn.end(), c2);
+    if (it == cn.end() || *it != c2) cn.insert(it, c2);
+    cl_[c1].out.push_back(id);
+    if (c1 != c2) cl_[c2].in.push_back(id);
+    return id;
+  }
+  int find_out(int c, int n2) const {
+    for (int e : cl_[c].out)
+      if (ed_[e].n2 == n2) return e;
+    return -1;
+  }
+  void sort_out(int c) {
+    auto& o = cl_[c].out;
+    std::stable_sort(o.begin(), o.end(), [this](int a, int b) { return cl_[ed_[a].n2].order < cl_[ed_[b].n2].order; });
+  }
+  std::pair<int, int> topsize() const {
+    int r = 0, c = 0;
+    for (int s : bottoms_[cur_])
+      if (cl_[s].level() == nlevels_ - 1) { r += cl_[s].rsize; c += cl_[s].csize; }
+    return {r, c};
+  }
+  bool want_sparsify(int c) const {  // tree.cpp:583-591
+    const Clu& s = cl_[c];
+    return s.level() > ilvl_ && s.id.l.lvl <= ilvl_ && s.id.r.lvl <= ilvl_;
+  }
+  void sync() { SPQ_HIP(hipStreamSynchronize(stream_)); stager_.reset(); }
+
+  // Records a HIP event pair around one launch of a kernel family when kernel
+  // timing is on (events are recorded on the engine's stream, where the
+  // kernels run).
+  template <typename F>
+  void timed(const char* name, double flops, double bytes, F&& launch) {
+    if (!kprof_) { launch(); return; }
+    hipEvent_t a, b;
+    SPQ_HIP(hipEventCreate(&a));
+    SPQ_HIP(hipEventCreate(&b));
+    SPQ_HIP(hipEventRecord(a, stream_));
+    launch();
+    SPQ_HIP(hipEventRecord(b, stream_));
+    auto& k = kev_[name];
+    k.ev.push_back({a, b});
+    k.flops += flops;
+    k.bytes += bytes;
+  }
+
+  // --------------------------------------------------- batched launch helpers
+  void copy_blocks(const std::vector<CopyDesc>& ds) {
+    if (ds.empty()) return;
+    std::vector<int> cnt(ds.size()), st;
+    for (size_t i = 0; i < ds.size(); ++i) cnt[i] = iceil(std::max(ds[i].rows, 1), 32) * iceil(std::max(ds[i].cols, 1), 32);
+    const int nt = prefix_tiles(cnt, st);
+    double bytes = 0;
+    for (auto& d : ds) bytes += 2.0 * sizeof(T) * d.rows * d.cols;
+    const CopyDesc* dd = stager_.push(ds);
+    const int* dst = stager_.push(st);
+    timed("copy", 0.0, bytes, [&] { launch_copy(stream_, sizeof(T), sizeof(T), dd, dst, (int)ds.size(), nt); });
+  }
+  void init_blocks(const std::vector<Block>& bs, int identity) {
+    if (bs.empty()) return;
+    std::vector<CopyDesc> ds;
+    for (auto& b : bs) {
+      if (b.rows == 0 || b.cols == 0) continue;
+      CopyDesc d{};
+      d.dst = b.p; d.ldd = b.ld; d.rows = b.rows; d.cols = b.cols;
+      ds.push_back(d);
+    }
+    if (ds.empty()) return;
+    for (auto& d : ds) d.trans = identity;
+    std::vector<int> cnt(ds.size()), st;
+    for (size_t i = 0; i < ds.size(); ++i) cnt[i] = iceil(ds[i].rows, 32) * iceil(ds[i].cols, 32);
+    const int nt = prefix_tiles(cnt, st);
+    double bytes = 0;
+    for (auto& d : ds) bytes += (double)sizeof(T) * d.rows * d.cols;
+    const CopyDesc* dd = stager_.push(ds);
+    const int* dst = stager_.push(st);
+    timed("init", 0.0, bytes, [&] { launch_init_blocks(stream_, sizeof(T), dd, dst, (int)ds.size(), nt); });
+  }
+  void gemm_batch(const std::vector<GemmDesc<T>>& ds) {
+    if (ds.empty()) return;
+    std::vector<int> cnt(ds.size()), st;
+    for (size_t i = 0; i < ds.size(); ++i) cnt[i] = gemm_tiles(ds[i].m, ds[i].n);
+    const int nt = prefix_tiles(cnt, st);
+    double fl = 0, by = 0;
+    for (auto& d : ds) {
+      fl += 2.0 * d.m * d.n * d.k;
+      by += sizeof(T) * ((double)d.m * d.k + (double)d.k * d.n + (d.beta == T(0) ? 1.0 : 2.0) * d.m * d.n);
+    }
+    const GemmDesc<T>* dd = stager_.push(ds);
+    const int* dst = stager_.push(st);
+    timed("gemm", fl, by, [&] { launch_gemm<T>(stream_, dd, dst, (int)ds.size(), nt); });
+  }
+  static GemmDesc<T> gd(const T* A, int lda, const T* B, int ldb, T* C, int ldc, int m, int n, int k, int flags,
+                        T alpha, T beta) {
+    GemmDesc<T> d;
+    d.A = A; d.B = B; d.C = C; d.lda = lda; d.ldb = ldb; d.ldc = ldc; d.m = m; d.n = n; d.k = k;
+    d.flags = flags; d.alpha = alpha; d.beta = beta;
+    return d;
+  }
+
+  // Blocked Householder QR of a batch of panels.  For every block step the
+  // panel kernel factors NB columns, then the block reflector is applied to the
+  // panel's trailing columns and to every extra matrix `extra[p]` (same row
+  // space as the panel, ld given) with three batched GEMMs:
+  //   Y = V_b^T C ; Z = T_b^T Y ; C -= V_b Z.
+  struct QrJob {
+    T* P; int ld, m, k;
+    T* tau; T* Tb;
+    std::vector<Block> extra;  // matrices receiving H^T (rows = m)
+  };
+  void blocked_qr(std::vector<QrJob>& jobs) {
+    int kmax = 0;
+    for (auto& j : jobs) kmax = std::max(kmax, std::min(j.m, j.k));
+    for (int jb = 0; jb < kmax; jb += nb_) {
+      std::vector<PanelDesc<T>> pd;
+      std::vector<int> live;
+      for (int q = 0; q < (int)jobs.size(); ++q) {
+        auto& j = jobs[q];
+        if (jb >= std::min(j.m, j.k)) continue;
+        PanelDesc<T> d{};
+        d.P = j.P; d.tau = j.tau; d.Tb = j.Tb; d.ld = j.ld; d.m = j.m; d.k = j.k;
+        pd.push_back(d);
+        live.push_back(q);
+      }
+      if (pd.empty()) break;
+      double pfl = 0, pby = 0;
+      for (auto& d : pd) {
+        const int w = std::min(nb_, std::min(d.m, d.k) - jb), r = d.m - jb, c = std::min(nb_, d.k - jb);
+        pfl += 4.0 * r * w * c + 2.0 * r * w * w;  // column steps: dots + updates, then T
+        pby += 2.0 * sizeof(T) * (double)r * c;
+      }
+      const PanelDesc<T>* dpd = stager_.push(pd);
+      timed("panel", pfl, pby, [&] { launch_panel<T>(stream_, dpd, (int)pd.size(), jb); });
+      // trailing matrices (Y, Z temporaries live in ws_, rewound per step)
+      ws_.rewind();
+      std::vector<GemmDesc<T>> g1, g2, g3;
+      for (int q : live) {
+        auto& j = jobs[q];
+        const int w = std::min(nb_, std::min(j.m, j.k) - jb);
+        const int rows = j.m - jb;
+        const T* Vb = j.P + jb + (int64_t)jb * j.ld;
+        const T* Tb = j.Tb + (int64_t)jb * nb_;
+        struct Tgt { T* C; int ld, cols; };
+        std::vector<Tgt> tg;
+        const int c0 = jb + std::min(nb_, j.k - jb);
+        if (j.k > c0) tg.push_back({j.P + jb + (int64_t)c0 * j.ld, j.ld, j.k - c0});
+        for (auto& b : j.extra)
+          if (b.cols > 0) tg.push_back({b.p + jb, b.ld, b.cols});
+        for (auto& t : tg) {
+          T* Y = (T*)ws_.alloc(sizeof(T) * (size_t)nb_ * t.cols);
+          T* Z = (T*)ws_.alloc(sizeof(T) * (size_t)nb_ * t.cols);
+          g1.push_back(gd(Vb, j.ld, t.C, t.ld, Y, nb_, w, t.cols, rows, gemm_flags(true, false, kStructUnitLower), T(1), T(0)));
+          g2.push_back(gd(Tb, nb_, Y, nb_, Z, nb_, w, t.cols, w, gemm_flags(true, false, kStructUpper), T(1), T(0)));
+          g3.push_back(gd(Vb, j.ld, Z, nb_, t.C, t.ld, rows, t.cols, w, gemm_flags(false, false, kStructUnitLower), T(-1), T(1)));
+        }
+      }
+      gemm_batch(g1);
+      gemm_batch(g2);
+      gemm_batch(g3);
+    }
+  }
+
+  // ------------------------------------------------------------ phases
+  void get_sparsity(int c);
+  void eliminate_level(const std::vector<int>& cs);
+  void scale_level(const std::vector<int>& cs);
+  void sparsify_level(const std::vector<int>& cs);
+  void merge_all();
+  void build_solve_program();
+
+  // ------------------------------------------------------------ state
+  int nlevels_, skip_, scale_ = 0, square_ = 0, verbose_ = 0, timing_ = 1;
+  double tol_ = 0;
+  float order_ = 1;
+  bool geo_ = false;
+  std::vector<double> X_;
+  int xdim_ = 0;
+  int nrows_ = 0, ncols_ = 0, ilvl_ = -1, cur_ = 0, max_order_ = 0, nb_ = 32;
+  std::vector<int> cperm_, rperm_;
+  std::vector<Clu> cl_;
+  std::vector<Edge> ed_;
+  std::vector<std::vector<int>> bottoms_;
+  std::vector<Op> ops_;
+  std::vector<LevelInfo> info_;
+  std::vector<SparsifyInfo> sprs_;
+  unsigned long long nnzA_ = 0, nnzR_ = 0, nnzH_ = 0, nnzQ_ = 0;
+  double flops_ = 0, t_partition_ = 0, t_assemble_ = 0, t_factorize_ = 0, top_size_ = 0;
+  double factor_bytes_ = 0, solve_bytes_ = 0;
+  int64_t xsize_ = 0;
+  T* xbuf_ = nullptr;
+  hipStream_t stream_ = nullptr;
+  Arena perm_, scratch_, ws_;
+  Stager stager_;
+  int* rank_host_ = nullptr;
+  bool factored_ = false;
+  bool kprof_ = false;
+  struct KEv { std::vector<std::pair<hipEvent_t, hipEvent_t>> ev; double flops = 0, bytes = 0; };
+  std::map<std::string, KEv> kev_;
+
+  // solve program (device descriptor tables, built once after factorize)
+  struct Stage {
+    int type;  // 0 refl, 1 bwd, 2 move
+    int nd = 0, mmax = 0;
+    const ReflDesc<T>* refl = nullptr;
+    const BwdDesc<T>* bwd = nullptr;
+    const MoveDesc* mv = nullptr;
+    T* work = nullptr;
+  };
+  std::vector<Stage> fwd_, bwd_;
+  const Seg* segs_dev_ = nullptr;
+  const R12Desc<T>* r12_dev_ = nullptr;
+  int64_t* in_dst_ = nullptr;
+  int* in_src_ = nullptr;
+  int64_t* out_src_ = nullptr;
+  int* out_dst_ = nullptr;
+  int n_in_ = 0, n_out_ = 0;
+  hipGraphExec_t graph_exec_fwd_ = nullptr;
+  double* bin_ = nullptr;
+  double* xout_ = nullptr;
+  void run_program(const double* b, double* x);
+};
+
+// ======================================================================
+template <typename T>
+void Engine<T>::partition(const CscMat& A) {  // tree.cpp:595-723
+  const double t0 = wall();
+  nrows_ = A.m;
+  ncols_ = A.n;
+  if (!geo_) throw Error(kErrArg, "partition: coordinates are required (geometric partition)");
+  if ((int64_t)X_.size() != (int64_t)xdim_ * A.n) throw Error(kErrArg, "partition: coordinate array has wrong size");
+  std::vector<CID> lab = geometric_labels(A, nlevels_, X_, xdim_);
+  rperm_.resize(A.m);
+  cperm_.resize(A.n);
+  std::iota(rperm_.begin(), rperm_.end(), 0);
+  std::iota(cperm_.begin(), cperm_.end(), 0);
+  order_columns(nlevels_, lab, cperm_);
+  std::vector<CID> cpermed(A.n);
+  for (int k = 0; k < A.n; ++k) cpermed[k] = lab[cperm_[k]];
+  std::vector<int> rows_per_col;
+  if (!square_) {
+    CscMat Ap = csc_permute(A, nullptr, &cperm_);
+    order_rows_ls(Ap, cpermed, rperm_, rows_per_col);
+  } else {
+    rows_per_col.assign(A.n, 1);
+    rperm_ = cperm_;
+  }
+  int r0 = 0;
+  for (int k = 0; k < A.n;) {
+    int kn = k + 1, nr = rows_per_col[k];
+    while (kn < A.n && cpermed[kn] == cpermed[k]) nr += rows_per_col[kn++];
+    if (nr == 0) throw Error(kErrArg, "partition: a cluster received no rows");
+    bottoms_[0].push_back(new_cluster(k, kn - k, r0, nr, cpermed[k]));
+    r0 += nr;
+    k = kn;
+  }
+  for (int lvl = 1; lvl < nlevels_; ++lvl) {
+    auto& prev = bottoms_[lvl - 1];
+    size_t b = 0;
+    while (b < prev.size() && cl_[prev[b]].level() < lvl) ++b;
+    for (size_t q = b; q < prev.size(); ++q) cl_[prev[q]].parentid = cid_merge_if(cl_[prev[q]].id, lvl);
+    for (size_t q = b; q < prev.size();) {
+      const CID pid = cl_[prev[q]].parentid;
+      std::vector<int> ch{prev[q]};
+      int cs = cl_[prev[q]].csize, rs = cl_[prev[q]].rsize;
+      const int c0 = cl_[prev[q]].cstart, rr0 = cl_[prev[q]].rstart;
+      ++q;
+      while (q < prev.size() && cl_[prev[q]].parentid == pid) {
+        ch.push_back(prev[q]);
+        cs += cl_[prev[q]].csize;
+        rs += cl_[prev[q]].rsize;
+        ++q;
+      }
+      const int p = new_cluster(c0, cs, rr0, rs, pid);
+      for (int c : ch) { cl_[c].parent = p; cl_[p].children.push_back(c); }
+      bottoms_[lvl].push_back(p);
+    }
+  }
+  t_partition_ = wall() - t0;
+  if (verbose_) printf("Time to partition: %g\n", t_partition_);
+}
+
+template <typename T>
+void Engine<T>::assemble(const CscMat& A) {  // tree.cpp:725-778
+  const double t0 = wall();
+  nnzA_ = A.nnz();
+  const CscMat App = csc_permute(A, &rperm_, &cperm_);
+  std::vector<int> rmap(nrows_);
+  for (int c : bottoms_[0])
+    for (int i = cl_[c].rstart; i < cl_[c].rstart + cl_[c].rsize; ++i) rmap[i] = c;
+  for (int c : bottoms_[0]) alloc_x(c, cl_[c].rsize);
+  // host-side dense blocks, uploaded in one batch
+  std::vector<T> host;
+  struct Pending { int edge; size_t off; };
+  std::vector<Pending> pend;
+  for (int self : bottoms_[0]) {
+    std::vector<int> nb;
+    const Clu& s = cl_[self];
+    for (int j = s.cstart; j < s.cstart + s.csize; ++j)
+      for (int q = App.p[j]; q < App.p[j + 1]; ++q) nb.push_back(rmap[App.i[q]]);
+    nb.push_back(self);
+    std::sort(nb.begin(), nb.end());
+    nb.erase(std::unique(nb.begin(), nb.end()), nb.end());
+    for (int n : nb) {
+      Edge e;
+      e.n1 = self; e.n2 = n;
+      e.a.rows = cl_[n].rsize; e.a.cols = s.csize; e.a.ld = std::max(1, e.a.rows);
+      const size_t off = host.size();
+      host.resize(off + (size_t)e.a.ld * e.a.cols, T(0));
+      const int r0 = cl_[n].rstart, r1 = r0 + cl_[n].rsize;
+      for (int col = 0; col < s.csize; ++col) {
+        const int j = s.cstart + col;
+        const int* beg = App.i.data() + App.p[j];
+        const int* end = App.i.data() + App.p[j + 1];
+        for (const int* it = std::lower_bound(beg, end, r0); it != end && *it < r1; ++it)
+          host[off + (size_t)(*it - r0) + (size_t)col * e.a.ld] = (T)App.x[it - App.i.data()];
+      }
+      ed_.push_back(e);
+      const int id = (int)ed_.size() - 1;
+      pend.push_back({id, off});
+      cl_[self].out.push_back(id);
+      if (self != n) cl_[n].in.push_back(id);
+    }
+    sort_out(self);
+    cl_[self].cnbrs = nb;
+  }
+  T* dev = alloc_perm(host.size());
+  SPQ_HIP(hipMemcpyAsync(dev, host.data(), sizeof(T) * host.size(), hipMemcpyHostToDevice, stream_));
+  for (auto& p : pend) ed_[p.edge].a.p = dev + p.off;
+  sync();
+  t_assemble_ = wall() - t0;
+  if (verbose_) {
+    auto ts = topsize();
+    printf("Time to assemble: %g\n", t_assemble_);
+    printf("Aspect ratio of top separator: %g\n", (double)ts.first / (double)ts.second);
+  }
+}
+
+template <typename T>
+void Engine<T>::get_sparsity(int c) {  // tree.cpp:834-870 (+ check_if_valid 168-224, geometric)
+  std::vector<SepID> colsp;
+  for (int e : cl_[c].out) colsp.push_back(cl_[ed_[e].n2].id.self);
+  std::sort(colsp.begin(), colsp.end());
+  auto has = [&colsp](const SepID& s) { return std::binary_search(colsp.begin(), colsp.end(), s); };
+  const SepID cs = cl_[c].id.self;
+  auto valid = [&](const CID& nid) {
+    const SepID ns = nid.self;
+    if (cs.lvl > ns.lvl) return false;
+    const int d = ns.lvl - cs.lvl;
+    if (cs.sep / (int)std::pow(2, d) != ns.sep) return false;
+    auto side_ok = [&](const SepID& s) {
+      const int dd = s.lvl - cs.lvl;
+      if (s.lvl >= cs.lvl) return cs.sep / (int)std::pow(2, dd) == s.sep;
+      return s.sep / (int)std::pow(2, -dd) == cs.sep;
+    };
+    if (!side_ok(nid.l) && !side_ok(nid.r)) return false;
+    if (!has(ns)) return false;
+    if (!has(nid.l) && !has(nid.r)) return false;
+    return true;
+  };
+  std::vector<int> rs;
+  for (int e : cl_[c].out) {
+    if (!ed_[e].alive) continue;
+    const int n = ed_[e].n2;
+    rs.push_back(n);
+    for (int ein : cl_[n].in) {
+      const int nin = ed_[ein].n1;
+      if (ed_[ein].alive && nin != c && !cl_[nin].elim && valid(cl_[nin].id)) rs.push_back(nin);
+    }
+  }
+  std::sort(rs.begin(), rs.end());
+  rs.erase(std::unique(rs.begin(), rs.end()), rs.end());
+  cl_[c].rsp = rs;
+}
+
+// tree.cpp:780-832 (eliminate_cluster, householder, update_cluster) for every
+// cluster of the level at once.
+template <typename T>
+void Engine<T>::eliminate_level(const std::vector<int>& cs) {
+  struct Plan {
+    int c, m, k, Nw;
+    std::vector<int> pe;                  // panel edges
+    std::vector<int> tg;                  // target clusters
+    std::vector<std::vector<int>> te;     // target edges aligned with pe
+    std::vector<int> touched;             // sorted target edge ids
+    T *P, *W, *tau, *Tb;
+  };
+  std::vector<Plan> plans;
+  std::vector<Block> zero;
+  for (int c : cs) {
+    if (cl_[c].csize == 0) { cl_[c].elim = true; continue; }
+    Plan p;
+    p.c = c; p.m = 0; p.k = cl_[c].csize; p.Nw = 0;
+    for (int e : cl_[c].out)
+      if (ed_[e].alive) { p.pe.push_back(e); p.m += ed_[e].a.rows; }
+    if (ed_[p.pe[0]].n2 != c) throw Error(kErrState, "eliminate: self block is not first");
+    for (int n : cl_[c].rsp) {
+      if (cl_[n].elim || n == c) continue;
+      std::vector<int> row;
+      for (int e : p.pe) {
+        const int kc = ed_[e].n2;
+        int f = find_out(n, kc);
+        if (f < 0) f = add_edge(n, kc, &zero);
+        row.push_back(f);
+        p.touched.push_back(f);
+      }
+      p.tg.push_back(n);
+      p.te.push_back(row);
+      p.Nw += cl_[n].csize;
+      nnzR_ += (unsigned long long)(cl_[c].csize * cl_[n].csize);
+    }
+    std::sort(p.touched.begin(), p.touched.end());
+    nnzH_ += (unsigned long long)(p.m * p.k);
+    const int kk = std::min(p.m, p.k);
+    flops_ += 4.0 * p.m * p.k * kk - 2.0 * (p.m + p.k) * kk * kk + 4.0 / 3.0 * kk * kk * kk;  // geqrf
+    flops_ += (double)p.m * kk * kk - (double)kk * kk * kk / 3.0;                           // larft
+    flops_ += 4.0 * p.m * p.Nw * kk - (double)p.Nw * kk * kk;                               // larfb
+    plans.push_back(std::move(p));
+  }
+  init_blocks(zero, 0);
+  // waves: a cluster waits for every earlier cluster sharing a target block
+  std::vector<int> wave(plans.size(), 0);
+  int nw = plans.empty() ? 0 : 1;
+  for (size_t i = 0; i < plans.size(); ++i) {
+    for (size_t j = 0; j < i; ++j) {
+      const auto& a = plans[i].touched;
+      const auto& b = plans[j].touched;
+      size_t x = 0, y = 0;
+      bool hit = false;
+      while (x < a.size() && y < b.size()) {
+        if (a[x] == b[y]) { hit = true; break; }
+        if (a[x] < b[y]) ++x; else ++y;
+      }
+      if (hit) wave[i] = std::max(wave[i], wave[j] + 1);
+    }
+    nw = std::max(nw, wave[i] + 1);
+  }
+  info_[ilvl_].elim_waves = nw;
+  info_[ilvl_].n_elim = (int)plans.size();
+  for (auto& p : plans) {
+    p.P = alloc_perm((size_t)p.m * p.k);
+    p.tau = alloc_perm(p.k);
+    p.Tb = alloc_perm((size_t)nb_ * iceil(p.k, nb_) * nb_);
+    factor_bytes_ += sizeof(T) * ((double)p.m * p.k + (double)nb_ * iceil(p.k, nb_) * nb_);
+  }
+  for (int w = 0; w < nw; ++w) {
+    scratch_.rewind();
+    std::vector<CopyDesc> gather, scatter;
+    std::vector<QrJob> jobs;
+    for (size_t i = 0; i < plans.size(); ++i) {
+      if (wave[i] != w) continue;
+      Plan& p = plans[i];
+      p.W = alloc_scr((size_t)p.m * std::max(p.Nw, 1));
+      int r = 0;
+      for (size_t s = 0; s < p.pe.size(); ++s) {
+        const Block& b = ed_[p.pe[s]].a;
+        gather.push_back(CopyDesc{b.p, p.P + r, b.ld, p.m, b.rows, b.cols, 0, 0});
+        int col = 0;
+        for (size_t t = 0; t < p.tg.size(); ++t) {
+          const Block& tb = ed_[p.te[t][s]].a;
+          CopyDesc d{tb.p, p.W + r + (int64_t)col * p.m, tb.ld, p.m, tb.rows, tb.cols, 0, 0};
+          gather.push_back(d);
+          scatter.push_back(CopyDesc{d.dst, (void*)tb.p, p.m, tb.ld, tb.rows, tb.cols, 0, 0});
+          col += cl_[p.tg[t]].csize;
+        }
+        r += b.rows;
+      }
+      QrJob j;
+      j.P = p.P; j.ld = p.m; j.m = p.m; j.k = p.k; j.tau = p.tau; j.Tb = p.Tb;
+      if (p.Nw > 0) j.extra.push_back(Block{p.W, p.m, p.Nw, p.m});
+      jobs.push_back(std::move(j));
+    }
+    copy_blocks(gather);
+    blocked_qr(jobs);
+    copy_blocks(scatter);
+  }
+  // ops and bookkeeping (tree.cpp:826-829)
+  for (auto& p : plans) {
+    Op o;
+    o.kind = OpQR; o.level = ilvl_; o.c = p.c;
+    o.V = p.P; o.ldv = p.m; o.m = p.m; o.k = p.k; o.Tb = p.Tb;
+    for (int e : p.pe) o.segs.push_back({ed_[e].n2, cl_[ed_[e].n2].rsize});
+    o.R = p.P; o.ldr = p.m; o.kr = p.k;
+    for (int e : cl_[p.c].in) o.r12.push_back({ed_[e].a, {ed_[e].n1, ed_[e].a.cols}});
+    ops_.push_back(std::move(o));
+    cl_[p.c].elim = true;
+  }
+}
+
+// tree.cpp:533-580 (diagScale) for every interface of the level: (i) QR of
+// the diagonal blocks with the in-edge blocks updated by Q^T in the same
+// blocked sweep, (ii) out-edge blocks times R^{-1}.
+template <typename T>
+void Engine<T>::scale_level(const std::vector<int>& cs) {
+  std::vector<QrJob> jobs;
+  std::vector<Block> ident;
+  struct TrsmJob { Block R; std::vector<Block> B; };
+  std::vector<TrsmJob> tj;
+  for (int c : cs) {
+    const int es = find_out(c, c);
+    const Block q = ed_[es].a;  // becomes the ScaleD factor storage
+    const int rr = q.rows, cc = cl_[c].csize, kk = std::min(rr, cc);
+    Block nb = new_block(rr, cc);
+    ident.push_back(nb);
+    ed_[es].a = nb;
+    QrJob j;
+    j.P = q.p; j.ld = q.ld; j.m = rr; j.k = cc;
+    j.tau = alloc_perm(cc);
+    j.Tb = alloc_perm((size_t)nb_ * iceil(cc, nb_) * nb_);
+    factor_bytes_ += sizeof(T) * ((double)rr * cc + (double)nb_ * iceil(cc, nb_) * nb_);
+    for (int e : cl_[c].in)
+      if (!cl_[ed_[e].n1].elim) j.extra.push_back(ed_[e].a);
+    TrsmJob t;
+    t.R = q;
+    for (int e : cl_[c].out)
+      if (!cl_[ed_[e].n2].elim && ed_[e].n2 != c) t.B.push_back(ed_[e].a);
+    nnzR_ = (unsigned long long)((double)nnzR_ + 0.5 * cc * cc);
+    nnzH_ += (unsigned long long)(cl_[c].rsize * cl_[c].csize);
+    flops_ += 4.0 * rr * cc * kk - 2.0 * (rr + cc) * kk * kk + 4.0 / 3.0 * kk * kk * kk + (double)rr * kk * kk -
+              (double)kk * kk * kk / 3.0;
+    for (auto& b : j.extra) flops_ += 4.0 * rr * b.cols * kk - (double)b.cols * kk * kk;
+    for (auto& b : t.B) flops_ += (double)b.rows * cc * cc;
+    Op o;
+    o.kind = OpScaleD; o.level = ilvl_; o.c = c;
+    o.V = q.p; o.ldv = q.ld; o.m = cl_[c].rsize; o.k = cc; o.Tb = j.Tb;
+    o.segs.push_back({c, cl_[c].rsize});
+    o.R = q.p; o.ldr = q.ld; o.kr = cc;
+    ops_.push_back(std::move(o));
+    jobs.push_back(std::move(j));
+    tj.push_back(std::move(t));
+  }
+  info_[ilvl_].n_scale = (int)cs.size();
+  init_blocks(ident, 1);
+  scratch_.rewind();
+  blocked_qr(jobs);
+  // (ii) B <- B R^{-1}, column block by column block
+  int kmax = 0;
+  for (auto& t : tj) kmax = std::max(kmax, t.R.cols);
+  for (int jb = 0; jb < kmax; jb += nb_) {
+    std::vector<GemmDesc<T>> g;
+    std::vector<TrsmDesc<T>> td;
+    std::vector<int> cnt;
+    for (auto& t : tj) {
+      if (jb >= t.R.cols) continue;
+      const int w = std::min(nb_, t.R.cols - jb);
+      for (auto& b : t.B) {
+        if (b.rows == 0) continue;
+        if (jb > 0)
+          g.push_back(gd(b.p, b.ld, t.R.p + (int64_t)jb * t.R.ld, t.R.ld, b.p + (int64_t)jb * b.ld, b.ld, b.rows, w, jb,
+                         gemm_flags(false, false, kStructNone, kStructNone), T(-1), T(1)));
+        td.push_back(TrsmDesc<T>{b.p, t.R.p, b.ld, t.R.ld, b.rows, t.R.cols});
+        cnt.push_back(iceil(b.rows, trsm_rows_per_tile()));
+      }
+    }
+    gemm_batch(g);
+    if (!td.empty()) {
+      std::vector<int> st;
+      const int nt = prefix_tiles(cnt, st);
+      double tfl = 0;
+      for (auto& d : td) tfl += (double)d.rows * std::min(nb_, d.k - jb) * std::min(nb_, d.k - jb);
+      const TrsmDesc<T>* dtd = stager_.push(td);
+      const int* dst = stager_.push(st);
+      timed("trsm", tfl, 0.0, [&] { launch_trsm_block<T>(stream_, dtd, dst, (int)td.size(), nt, jb); });
+    }
+  }
+}
+
+// tree.cpp:992-1114 (sparsifyD, scale = 1, order = 1) in dependency waves:
+// an interface waits for every earlier interface it shares a block with.
+template <typename T>
+void Engine<T>::sparsify_level(const std::vector<int>& cs) {
+  std::vector<int> S;
+  for (int c : cs)
+    if (cl_[c].csize > 0) S.push_back(c);
+  std::vector<int> wave(S.size(), 0);
+  int nw = S.empty() ? 0 : 1;
+  for (size_t i = 0; i < S.size(); ++i) {
+    for (size_t j = 0; j < i; ++j) {
+      const bool adj = find_out(S[i], S[j]) >= 0 || find_out(S[j], S[i]) >= 0;
+      if (adj) wave[i] = std::max(wave[i], wave[j] + 1);
+    }
+    nw = std::max(nw, wave[i] + 1);
+  }
+  info_[ilvl_].sprs_waves = nw;
+  info_[ilvl_].n_sprs = (int)S.size();
+  std::vector<SparsifyInfo> recs(S.size());
+  std::vector<int> slot(cl_.size(), -1);
+  for (size_t i = 0; i < S.size(); ++i) slot[S[i]] = (int)i;
+  for (int w = 0; w < nw; ++w) {
+    scratch_.rewind();
+    struct Job {
+      int c, k, N, smp;
+      std::vector<int> amp, apm;
+      T *Ct, *V, *tau;
+      double* vn;
+      int* pos;
+    };
+    std::vector<Job> jobs;
+    std::vector<CopyDesc> gather;
+    std::vector<Qp3Desc<T>> qd;
+    int kmax = 0;
+    int* ranks_dev = (int*)scratch_.alloc(sizeof(int) * S.size());
+    for (size_t i = 0; i < S.size(); ++i) {
+      if (wave[i] != w) continue;
+      const int c = S[i];
+      Job jb;
+      jb.c = c; jb.k = cl_[c].csize; jb.smp = 0;
+      int spm = 0;
+      for (int e : cl_[c].in)
+        if (!cl_[ed_[e].n1].elim) { jb.apm.push_back(e); spm += ed_[e].a.cols; }
+      for (int e : cl_[c].out)
+        if (!cl_[ed_[e].n2].elim && ed_[e].n2 != c) { jb.amp.push_back(e); jb.smp += ed_[e].a.rows; }
+      jb.N = jb.smp + spm;
+      const int N = std::max(jb.N, 1);
+      jb.Ct = alloc_scr((size_t)N * jb.k);
+      jb.V = alloc_scr((size_t)jb.k * jb.k);
+      jb.tau = alloc_scr(jb.k);
+      jb.vn = (double*)scratch_.alloc(sizeof(double) * 2 * N);
+      jb.pos = (int*)scratch_.alloc(sizeof(int) * 2 * N);
+      int off = 0;
+      for (int e : jb.amp) {
+        const Block& b = ed_[e].a;
+        gather.push_back(CopyDesc{b.p, jb.Ct + off, b.ld, N, b.rows, jb.k, 0, 0});
+        off += b.rows;
+      }
+      for (int e : jb.apm) {
+        const Block& b = ed_[e].a;
+        gather.push_back(CopyDesc{b.p, jb.Ct + off, b.ld, N, jb.k, b.cols, 1, 0});
+        off += b.cols;
+      }
+      Qp3Desc<T> d{};
+      d.Ct = jb.Ct; d.V = jb.V; d.tau = jb.tau; d.vn = jb.vn; d.pos = jb.pos;
+      d.rank_out = ranks_dev + jobs.size();
+      d.ldc = N; d.k = jb.k; d.N = jb.N; d.ldv = jb.k; d.tol = tol_;
+      qd.push_back(d);
+      kmax = std::max(kmax, jb.k);
+      const int kk = std::min(jb.k, jb.N);
+      flops_ += 4.0 * jb.k * jb.N * kk - 2.0 * (jb.k + jb.N) * kk * kk + 4.0 / 3.0 * kk * kk * kk;
+      jobs.push_back(std::move(jb));
+    }
+    copy_blocks(gather);
+    std::vector<int> ranks(jobs.size(), 0);
+    if (tol_ >= 1.0) {
+      std::fill(ranks.begin(), ranks.end(), 0);  // util.cpp:798-799
+    } else {
+      double qfl = 0, qby = 0;
+      for (auto& d : qd) {
+        qfl += 4.0 * d.k * d.N * std::min(d.k, d.N);  // upper bound: full rank steps
+        qby += 2.0 * sizeof(T) * (double)d.k * d.N;
+      }
+      const Qp3Desc<T>* dqd = stager_.push(qd);
+      timed("qp3", qfl, qby, [&] { launch_qp3<T>(stream_, dqd, (int)qd.size(), kmax); });
+      SPQ_HIP(hipMemcpyAsync(rank_host_, ranks_dev, sizeof(int) * jobs.size(), hipMemcpyDeviceToHost, stream_));
+      sync();
+      for (size_t q = 0; q < jobs.size(); ++q) ranks[q] = rank_host_[q];
+    }
+    std::vector<CopyDesc> scatter;
+    std::vector<LarftDesc<T>> lf;
+    std::vector<int> lcnt;
+    for (size_t q = 0; q < jobs.size(); ++q) {
+      Job& jb = jobs[q];
+      const int c = jb.c, k = jb.k, rank = ranks[q];
+      const int before = cl_[c].csize;
+      if (rank >= k) {  // tree.cpp:1050
+        recs[slot[c]] = {ilvl_, cl_[c].id.self.lvl, cl_[c].id.self.sep, before, cl_[c].csize};
+        continue;
+      }
+      const int N = std::max(jb.N, 1);
+      // OrthogonalD op: V (k x rank) kept, block T factors from tau
+      T* V = alloc_perm((size_t)k * std::max(rank, 1));
+      T* tau = alloc_perm(std::max(rank, 1));
+      T* Tb = alloc_perm((size_t)nb_ * iceil(std::max(rank, 1), nb_) * nb_);
+      factor_bytes_ += sizeof(T) * ((double)k * rank + (double)nb_ * iceil(std::max(rank, 1), nb_) * nb_);
+      if (rank > 0) {
+        scatter.push_back(CopyDesc{jb.V, V, k, k, k, rank, 0, 0});
+        scatter.push_back(CopyDesc{jb.tau, tau, rank, rank, rank, 1, 0, 0});
+        lf.push_back(LarftDesc<T>{V, tau, Tb, k, k, rank, 0});
+        lcnt.push_back(iceil(rank, nb_));
+      }
+      nnzQ_ += (unsigned long long)(k * rank);
+      Op od;
+      od.kind = OpOrthoD; od.level = ilvl_; od.c = c;
+      od.V = V; od.ldv = k; od.m = k; od.k = rank; od.Tb = Tb;
+      od.segs.push_back({c, k});
+      ops_.push_back(std::move(od));
+      // fine cluster (tree.cpp:1068-1079)
+      const int cfn = k - rank;
+      const int cf = new_cluster(cl_[c].cstart + rank, cfn, cl_[c].rstart + k - cfn, cfn, cl_[c].id);
+      alloc_x(cf, cfn);
+      cl_[cf].elim = true;
+      Op sp;
+      sp.kind = OpSplitD; sp.level = ilvl_; sp.c = c; sp.f = cf;
+      sp.crows = cl_[c].rsize; sp.ccols = k; sp.rank = rank;
+      ops_.push_back(std::move(sp));
+      // new blocks (tree.cpp:1082-1110)
+      const int rest = cl_[c].rsize - k;
+      const int rank_rows = rank + rest;
+      cl_[c].rsize = rank_rows;
+      cl_[c].csize = rank;
+      int off = 0;
+      for (int e : jb.amp) {
+        Block& b = ed_[e].a;
+        if (rank > 0) scatter.push_back(CopyDesc{jb.Ct + off, b.p, N, b.ld, b.rows, rank, 0, 0});
+        b.cols = rank;
+        off += b.rows;
+      }
+      for (int e : jb.apm) {
+        Block& b = ed_[e].a;
+        if (rest > 0) throw Error(kErrState, "sparsify: rectangular interface blocks are not supported in this build");
+        if (rank > 0) scatter.push_back(CopyDesc{jb.Ct + off, b.p, N, b.ld, b.cols, rank, 1, 0});
+        b.rows = rank_rows;
+        off += b.cols;
+      }
+      const int es = find_out(c, c);
+      ed_[es].a.rows = rank_rows;
+      ed_[es].a.cols = rank;
+      recs[slot[c]] = {ilvl_, cl_[c].id.self.lvl, cl_[c].id.self.sep, before, rank};
+    }
+    copy_blocks(scatter);
+    if (!lf.empty()) {
+      std::vector<int> st;
+      const int nt = prefix_tiles(lcnt, st);
+      const LarftDesc<T>* dl = stager_.push(lf);
+      const int* dst = stager_.push(st);
+      timed("larft", 0.0, 0.0, [&] { launch_larft<T>(stream_, dl, dst, (int)lf.size(), nt); });
+    }
+  }
+  sprs_.insert(sprs_.end(), recs.begin(), recs.end());
+}
+
+template <typename T>
+void Engine<T>::merge_all() {  // tree.cpp:872-881, 414-475
+  cur_++;
+  for (int snew : bottoms_[cur_]) {
+    int rs = 0, cs = 0;
+    for (int s : cl_[snew].children) { cl_[s].rpos = rs; cl_[s].cpos = cs; rs += cl_[s].rsize; cs += cl_[s].csize; }
+    Clu& p = cl_[snew];
+    p.rsize = rs; p.csize = cs; p.rsize_org = rs; p.csize_org = cs;
+    alloc_x(snew, rs);
+    Op o;
+    o.kind = OpMerge; o.level = ilvl_; o.c = snew;
+    ops_.push_back(std::move(o));
+  }
+  std::vector<Block> zero;
+  std::vector<CopyDesc> copies;
+  for (int snew : bottoms_[cur_]) {
+    std::vector<int> nb;
+    for (int s : cl_[snew].children)
+      for (int e : cl_[s].out)
+        if (!cl_[ed_[e].n2].elim) nb.push_back(cl_[ed_[e].n2].parent);
+    std::sort(nb.begin(), nb.end());
+    nb.erase(std::unique(nb.begin(), nb.end()), nb.end());
+    cl_[snew].cnbrs = nb;
+    for (int n : nb) add_edge(snew, n, &zero);
+    sort_out(snew);
+    for (int s : cl_[snew].children)
+      for (int e : cl_[s].out) {
+        const int nold = ed_[e].n2;
+        if (cl_[nold].elim) continue;
+        const int f = find_out(snew, cl_[nold].parent);
+        Block& dst = ed_[f].a;
+        const Block& src = ed_[e].a;
+        copies.push_back(CopyDesc{src.p, dst.p + cl_[nold].rpos + (int64_t)cl_[s].cpos * dst.ld, src.ld, dst.ld,
+                                  src.rows, src.cols, 0, 0});
+        ed_[e].alive = false;
+      }
+  }
+  init_blocks(zero, 0);
+  copy_blocks(copies);
+}
+
+template <typename T>
+int Engine<T>::factorize() {  // tree.cpp:1309-1437
+  const double fstart = wall();
+  for (ilvl_ = 0; ilvl_ < nlevels_; ++ilvl_) {
+    LevelInfo& li = info_[ilvl_];
+    const double t0 = wall();
+    std::vector<int> lev;
+    for (int s : bottoms_[cur_])
+      if (cl_[s].level() == ilvl_) lev.push_back(s);
+    for (int s : lev) get_sparsity(s);
+    const double t1 = wall();
+    eliminate_level(lev);
+    if (timing_) sync();
+    const double t2 = wall();
+    if (!square_ && ilvl_ < nlevels_ - 1) throw Error(kErrState, "least-squares factorization is not available in this build");
+    const double t3 = wall();
+    std::vector<int> itf;
+    for (int s : bottoms_[cur_])
+      if (want_sparsify(s)) itf.push_back(s);
+    if (scale_ && ilvl_ < nlevels_ - 2 && ilvl_ >= skip_ && tol_ != 0) scale_level(itf);
+    if (timing_) sync();
+    const double t4 = wall();
+    if (ilvl_ >= skip_ && ilvl_ < nlevels_ - 2 && tol_ != 0) {
+      if (!scale_) throw Error(kErrState, "--scale 0 sparsification is not available in this build");
+      if (order_ == 1.5f) throw Error(kErrState, "--order 1.5 sparsification is not available in this build");
+      sparsify_level(itf);
+    }
+    if (timing_) sync();
+    const double t5 = wall();
+    if (ilvl_ < nlevels_ - 1) merge_all();
+    sync();
+    const double t6 = wall();
+    li.t_fill = t1 - t0; li.t_elmn = t2 - t1; li.t_shift = t3 - t2; li.t_scale = t4 - t3;
+    li.t_sprsfy = t5 - t4; li.t_merge = t6 - t5;
+    auto ts = topsize();
+    li.top_rows = ts.first; li.top_cols = ts.second;
+    if (verbose_)
+      printf("lvl: %d    update fill-in: %.3f  elmn: %.3f  shift: %.3f  scale: %.3f  sprsfy: %.3f  merge: %.3f  "
+             "size top_sep: %d, %d   a.r top_sep: %.3f\n",
+             ilvl_, li.t_fill, li.t_elmn, li.t_shift, li.t_scale, li.t_sprsfy, li.t_merge, ts.first, ts.second,
+             (double)ts.first / (double)ts.second);
+  }
+  build_solve_program();
+  sync();
+  t_factorize_ = wall() - fstart;
+  top_size_ = cl_[bottoms_[nlevels_ - 1][0]].csize;
+  factored_ = true;
+  if (verbose_) {
+    printf("Tolerance set: %.3e\n", tol_);
+    printf("Time to factorize:  %.3e\n", t_factorize_);
+    printf("Size of top separator: %d\n", (int)top_size_);
+    printf("nnzA: %llu nnzR: %llu\n", nnzA_, nnzR_);
+    printf("nnzH: %llu nnzQ: %llu\n", nnzH_, nnzQ_);
+  }
+  return 0;
+}
+
+// Solve program: per level, forward stages QR (waves) -> ScaleD -> OrthogonalD
+// -> SplitD -> Merge, backward stages in the reverse order (tree.cpp:1440-1466).
+template <typename T>
+void Engine<T>::build_solve_program() {
+  SPQ_HIP(hipMalloc((void**)&xbuf_, sizeof(T) * std::max<int64_t>(xsize_, 1)));
+  SPQ_HIP(hipMalloc((void**)&bin_, sizeof(double) * std::max(nrows_, 1)));
+  SPQ_HIP(hipMalloc((void**)&xout_, sizeof(double) * std::max(ncols_, 1)));
+  std::vector<Seg> segs;
+  std::vector<R12Desc<T>> r12;
+  struct HStage { int type; std::vector<ReflDesc<T>> refl; std::vector<BwdDesc<T>> bwd; std::vector<MoveDesc> mv; int mmax = 0; };
+  std::vector<HStage> hf, hb;
+  auto refl_of = [&](const Op& o, int trans) {
+    ReflDesc<T> d{};
+    d.V = o.V; d.Tb = o.Tb; d.ldv = o.ldv; d.m = o.m; d.k = o.k; d.nb = nb_;
+    d.seg0 = (int)segs.size(); d.nseg = (int)o.segs.size(); d.trans = trans;
+    for (auto& s : o.segs) segs.push_back(Seg{cl_[s.first].xoff, s.second, 0});
+    solve_bytes_ += sizeof(T) * ((double)o.m * o.k + (double)nb_ * o.k) + 2.0 * sizeof(T) * o.m;
+    return d;
+  };
+  auto bwd_of = [&](const Op& o) {
+    BwdDesc<T> d{};
+    d.R = o.R; d.ldr = o.ldr; d.k = o.kr; d.xoff = cl_[o.c].xoff;
+    d.blk0 = (int)r12.size(); d.nblk = (int)o.r12.size(); d.trans = 0;
+    for (auto& q : o.r12) {
+      r12.push_back(R12Desc<T>{q.first.p, q.first.ld, q.second.second, cl_[q.second.first].xoff});
+      solve_bytes_ += sizeof(T) * (double)o.kr * q.second.second;
+    }
+    solve_bytes_ += sizeof(T) * 0.5 * o.kr * o.kr + 2.0 * sizeof(T) * o.kr;
+    return d;
+  };
+  for (int lvl = 0; lvl < nlevels_; ++lvl) {
+    std::vector<const Op*> qr, sc, od, sd, mg;
+    for (auto& o : ops_) {
+      if (o.level != lvl) continue;
+      if (o.kind == OpQR) qr.push_back(&o);
+      else if (o.kind == OpScaleD) sc.push_back(&o);
+      else if (o.kind == OpOrthoD) od.push_back(&o);
+      else if (o.kind == OpSplitD) sd.push_back(&o);
+      else mg.push_back(&o);
+    }
+    // QR forward waves: ops sharing an x segment keep their order
+    std::vector<int> wv(qr.size(), 0);
+    int nw = 0;
+    for (size_t i = 0; i < qr.size(); ++i) {
+      for (size_t j = 0; j < i; ++j) {
+        bool hit = false;
+        for (auto& a : qr[i]->segs) {
+          for (auto& b : qr[j]->segs)
+            if (a.first == b.first) { hit = true; break; }
+          if (hit) break;
+        }
+        if (hit) wv[i] = std::max(wv[i], wv[j] + 1);
+      }
+      nw = std::max(nw, wv[i] + 1);
+    }
+    for (int w = 0; w < nw; ++w) {
+      HStage s; s.type = 0;
+      for (size_t i = 0; i < qr.size(); ++i)
+        if (wv[i] == w) { s.refl.push_back(refl_of(*qr[i], 1)); s.mmax = std::max(s.mmax, qr[i]->m); }
+      hf.push_back(std::move(s));
+    }
+    HStage ssc; ssc.type = 0;
+    for (auto o : sc) { ssc.refl.push_back(refl_of(*o, 1)); ssc.mmax = std::max(ssc.mmax, o->m); }
+    if (!ssc.refl.empty()) hf.push_back(std::move(ssc));
+    HStage sod; sod.type = 0;
+    for (auto o : od) if (o->k > 0) { sod.refl.push_back(refl_of(*o, 1)); sod.mmax = std::max(sod.mmax, o->m); }
+    if (!sod.refl.empty()) hf.push_back(std::move(sod));
+    HStage ssd; ssd.type = 2;
+    for (auto o : sd) {
+      const int n = o->ccols - o->rank;
+      if (n > 0) ssd.mv.push_back(MoveDesc{cl_[o->c].xoff + o->rank, cl_[o->f].xoff, n, 0});
+    }
+    if (!ssd.mv.empty()) hf.push_back(std::move(ssd));
+    HStage smg; smg.type = 2;
+    for (auto o : mg) {
+      int64_t k = 0;
+      for (int ch : cl_[o->c].children) {
+        if (cl_[ch].rsize > 0) smg.mv.push_back(MoveDesc{cl_[ch].xoff, cl_[o->c].xoff + k, cl_[ch].rsize, 0});
+        k += cl_[ch].rsize;
+      }
+    }
+    if (!smg.mv.empty()) hf.push_back(std::move(smg));
+  }
+  for (int lvl = nlevels_ - 1; lvl >= 0; --lvl) {
+    std::vector<const Op*> qr, sc, od, sd, mg;
+    for (auto& o : ops_) {
+      if (o.level != lvl) continue;
+      if (o.kind == OpQR) qr.push_back(&o);
+      else if (o.kind == OpScaleD) sc.push_back(&o);
+      else if (o.kind == OpOrthoD) od.push_back(&o);
+      else if (o.kind == OpSplitD) sd.push_back(&o);
+      else mg.push_back(&o);
+    }
+    HStage smg; smg.type = 2;
+    for (auto o : mg) {
+      int64_t k = 0;
+      for (int ch : cl_[o->c].children) {
+        if (cl_[ch].csize > 0) smg.mv.push_back(MoveDesc{cl_[o->c].xoff + k, cl_[ch].xoff, cl_[ch].csize, 0});
+        k += cl_[ch].csize;
+      }
+    }
+    if (!smg.mv.empty()) hb.push_back(std::move(smg));
+    HStage ssd; ssd.type = 2;
+    for (auto o : sd) {
+      const int n = o->ccols - o->rank;
+      if (n > 0) ssd.mv.push_back(MoveDesc{cl_[o->f].xoff, cl_[o->c].xoff + o->rank, n, 0});
+    }
+    if (!ssd.mv.empty()) hb.push_back(std::move(ssd));
+    HStage sod; sod.type = 0;
+    for (auto o : od) if (o->k > 0) { sod.refl.push_back(refl_of(*o, 0)); sod.mmax = std::max(sod.mmax, o->m); }
+    if (!sod.refl.empty()) hb.push_back(std::move(sod));
+    HStage ssc; ssc.type = 1;
+    for (auto o : sc) ssc.bwd.push_back(BwdDesc<T>{o->R, o->ldr, o->k, cl_[o->c].xoff, 0, 0, 0, 0});
+    if (!ssc.bwd.empty()) hb.push_back(std::move(ssc));
+    HStage sqr; sqr.type = 1;
+    for (auto o : qr) sqr.bwd.push_back(bwd_of(*o));
+    if (!sqr.bwd.empty()) hb.push_back(std::move(sqr));
+  }
+  segs_dev_ = stager_.upload(perm_, segs);
+  r12_dev_ = stager_.upload(perm_, r12);
+  auto finish = [&](std::vector<HStage>& hs, std::vector<Stage>& out) {
+    for (auto& h : hs) {
+      Stage s;
+      s.type = h.type;
+      s.mmax = h.mmax;
+      if (h.type == 0) {
+        s.nd = (int)h.refl.size();
+        s.refl = stager_.upload(perm_, h.refl);
+        if (h.mmax > refl_lds_cap(sizeof(T))) s.work = alloc_perm((size_t)s.nd * h.mmax);
+      } else if (h.type == 1) {
+        s.nd = (int)h.bwd.size();
+        s.bwd = stager_.upload(perm_, h.bwd);
+      } else {
+        s.nd = (int)h.mv.size();
+        s.mv = stager_.upload(perm_, h.mv);
+      }
+      out.push_back(s);
+    }
+  };
+  finish(hf, fwd_);
+  finish(hb, bwd_);
+  // entry / exit maps (set_vector / extract_vector, cluster.cpp:125-156)
+  std::vector<int64_t> idst, osrc;
+  std::vector<int> isrc, odst;
+  for (int c : bottoms_[0]) {
+    for (int i = 0; i < cl_[c].rsize_org; ++i) { idst.push_back(cl_[c].xoff + i); isrc.push_back(rperm_[cl_[c].rstart + i]); }
+    for (int i = 0; i < cl_[c].csize_org; ++i) { osrc.push_back(cl_[c].xoff + i); odst.push_back(cperm_[cl_[c].cstart + i]); }
+  }
+  n_in_ = (int)idst.size();
+  n_out_ = (int)osrc.size();
+  in_dst_ = stager_.upload(perm_, idst);
+  in_src_ = stager_.upload(perm_, isrc);
+  out_src_ = stager_.upload(perm_, osrc);
+  out_dst_ = stager_.upload(perm_, odst);
+}
+
+template <typename T>
+void Engine<T>::run_program(const double* b, double* x) {
+  launch_zero(stream_, sizeof(T), xbuf_, xsize_);
+  launch_gather_in<T>(stream_, in_dst_, in_src_, n_in_, b, xbuf_);
+  auto run = [&](const Stage& s) {
+    if (s.type == 0) launch_refl<T>(stream_, s.refl, s.nd, segs_dev_, xbuf_, s.work, s.mmax);
+    else if (s.type == 1) launch_bwd<T>(stream_, s.bwd, s.nd, r12_dev_, xbuf_);
+    else launch_move<T>(stream_, s.mv, s.nd, xbuf_, xbuf_);
+  };
+  for (auto& s : fwd_) run(s);
+  for (auto& s : bwd_) run(s);
+  launch_scatter_out<T>(stream_, out_src_, out_dst_, n_out_, xbuf_, x);
+}
+
+template <typename T>
+void Engine<T>::solve_dev(const double* b, double* x) {
+  if (!factored_) throw Error(kErrState, "solve before factorize");
+  // b is taken by value (tree.cpp:1440): copy into the program's fixed input,
+  // replay the captured apply, copy the fixed output out.
+  SPQ_HIP(hipMemcpyAsync(bin_, b, sizeof(double) * nrows_, hipMemcpyDeviceToDevice, stream_));
+  if (!graph_exec_fwd_) {
+    hipGraph_t g = nullptr;
+    SPQ_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+    run_program(bin_, xout_);
+    SPQ_HIP(hipStreamEndCapture(stream_, &g));
+    SPQ_HIP(hipGraphInstantiate(&graph_exec_fwd_, g, nullptr, nullptr, 0));
+    hipGraphDestroy(g);
+  }
+  timed("apply", 0.0, solve_bytes_, [&] { SPQ_HIP(hipGraphLaunch(graph_exec_fwd_, stream_)); });
+  SPQ_HIP(hipMemcpyAsync(x, xout_, sizeof(double) * ncols_, hipMemcpyDeviceToDevice, stream_));
+}
+
+EngineBase* make_engine(int nlevels, int skip, int precision_bytes, int device) {
+  if (precision_bytes == 4) return new Engine<float>(nlevels, skip, device);
+  return new Engine<double>(nlevels, skip, device);
+}
+
+}  // namespace spq

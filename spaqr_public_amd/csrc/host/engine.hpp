@@ -1,0 +1,74 @@
+// spaQR engine: the reference's Tree (tree.h:36-152) with the symbolic work
+// (partition, cluster tree, fill, sizes, op bookkeeping) on the host and every
+// dense numeric phase of Tree::factorize / Tree::solve as batched gfx950
+// kernels over all clusters of a level.
+#pragma once
+#include <array>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../kernels/descs.hpp"
+#include "common.hpp"
+#include "partition.hpp"
+
+namespace spq {
+
+struct LevelInfo {
+  double t_fill = 0, t_elmn = 0, t_shift = 0, t_scale = 0, t_sprsfy = 0, t_merge = 0;
+  int top_rows = 0, top_cols = 0;
+  int n_elim = 0, n_scale = 0, n_sprs = 0, elim_waves = 0, sprs_waves = 0;
+};
+
+struct SparsifyInfo {
+  int ilvl, lvl, sep, before, after;
+};
+
+// Per kernel family: launches, device time (HIP events on the engine stream),
+// algorithmic flops and bytes of those launches.
+struct KernelStat {
+  std::string name;
+  int launches = 0;
+  double ms = 0, flops = 0, bytes = 0;
+};
+
+// Interface the C ABI needs, independent of the factor precision.
+class EngineBase {
+ public:
+  virtual ~EngineBase() {}
+  // setters (tree.cpp:14-20)
+  virtual void set_tol(double t) = 0;
+  virtual void set_skip(int s) = 0;
+  virtual void set_scale(int s) = 0;
+  virtual void set_square(int s) = 0;
+  virtual void set_order(float o) = 0;
+  virtual void set_coords(const std::vector<double>& X, int dim) = 0;
+  virtual void set_verbose(int v) = 0;
+  virtual void set_timing(int t) = 0;
+  virtual int rows() const = 0;
+  virtual int cols() const = 0;
+  virtual int levels() const = 0;
+  virtual void partition(const CscMat& A) = 0;
+  virtual void assemble(const CscMat& A) = 0;
+  virtual int factorize() = 0;
+  // x = M^{-1} b with device FP64 vectors (tree.cpp:1440-1466)
+  virtual void solve_dev(const double* b, double* x) = 0;
+  virtual void solve_nrml_dev(const double* b, double* x) = 0;
+  virtual hipStream_t stream() const = 0;
+  // inspection
+  virtual const std::vector<int>& cperm() const = 0;
+  virtual const std::vector<int>& rperm() const = 0;
+  virtual const std::vector<LevelInfo>& level_info() const = 0;
+  virtual const std::vector<SparsifyInfo>& sparsified() const = 0;
+  virtual std::vector<std::array<int, 12>> cluster_table() const = 0;
+  virtual void counters(unsigned long long* nnz4, double* dbl8) const = 0;
+  virtual int precision_bytes() const = 0;
+  // kernel-family timing with HIP events (off by default)
+  virtual void set_kernel_timing(int on) = 0;
+  virtual std::vector<KernelStat> kernel_stats() = 0;
+  virtual void reset_kernel_stats() = 0;
+};
+
+EngineBase* make_engine(int nlevels, int skip, int precision_bytes, int device);
+
+}  // namespace spq

@@ -1,0 +1,141 @@
+#include "krylov.hpp"
+
+#include <cmath>
+#include <cstdio>
+#include <vector>
+
+#include "../kernels/launch.hpp"
+
+namespace spq {
+
+DevMatrix::~DevMatrix() {
+  hipFree(rp); hipFree(ci); hipFree(v);
+  hipFree(rpt); hipFree(cit); hipFree(vt);
+}
+
+namespace {
+void up(void** d, const void* h, size_t bytes) {
+  SPQ_HIP(hipMalloc(d, bytes > 0 ? bytes : 8));
+  if (bytes) SPQ_HIP(hipMemcpy(*d, h, bytes, hipMemcpyHostToDevice));
+}
+}  // namespace
+
+DevMatrix* upload_matrix(const CscMat& A, bool with_transpose) {
+  DevMatrix* M = new DevMatrix;
+  M->m = A.m; M->n = A.n; M->nnz = A.nnz();
+  // CSR of A == CSC of A^T
+  CscMat At = csc_transpose(A);
+  up((void**)&M->rp, At.p.data(), sizeof(int) * At.p.size());
+  up((void**)&M->ci, At.i.data(), sizeof(int) * At.i.size());
+  up((void**)&M->v, At.x.data(), sizeof(double) * At.x.size());
+  if (with_transpose) {
+    up((void**)&M->rpt, A.p.data(), sizeof(int) * A.p.size());
+    up((void**)&M->cit, A.i.data(), sizeof(int) * A.i.size());
+    up((void**)&M->vt, A.x.data(), sizeof(double) * A.x.size());
+  }
+  return M;
+}
+
+KrylovResult gmres_dev(EngineBase& M, const DevMatrix& A, const double* b, double* x, int iters, int restart,
+                       double tol, int verbose) {
+  KrylovResult res;
+  hipStream_t s = M.stream();
+  const int n = A.m;
+  restart = std::max(1, std::min(restart, n));
+  const double t0 = wall();
+  double *V = nullptr, *w = nullptr, *p0 = nullptr, *h = nullptr, *hh = nullptr;
+  SPQ_HIP(hipMalloc((void**)&V, sizeof(double) * (size_t)n * (restart + 1)));
+  SPQ_HIP(hipMalloc((void**)&w, sizeof(double) * n));
+  SPQ_HIP(hipMalloc((void**)&p0, sizeof(double) * n));
+  SPQ_HIP(hipMalloc((void**)&h, sizeof(double) * (restart + 2) * 2));
+  SPQ_HIP(hipHostMalloc((void**)&hh, sizeof(double) * (restart + 2) * 2, hipHostMallocDefault));
+  auto cleanup = [&]() { hipFree(V); hipFree(w); hipFree(p0); hipFree(h); hipHostFree(hh); };
+  auto norm_dev = [&](const double* v) {
+    launch_dots(s, n, 1, v, n, v, h);
+    SPQ_HIP(hipMemcpyAsync(hh, h, sizeof(double), hipMemcpyDeviceToHost, s));
+    SPQ_HIP(hipStreamSynchronize(s));
+    return std::sqrt(hh[0]);
+  };
+  auto precond = [&](const double* in, double* out) {
+    const double a = wall();
+    M.solve_dev(in, out);
+    if (verbose) { SPQ_HIP(hipStreamSynchronize(s)); res.t_precond += wall() - a; }
+    res.n_precond++;
+  };
+  if (norm_dev(b) <= std::numeric_limits<double>::min()) {
+    SPQ_HIP(hipMemsetAsync(x, 0, sizeof(double) * A.n, s));
+    SPQ_HIP(hipStreamSynchronize(s));
+    cleanup();
+    return res;
+  }
+  std::vector<double> H((size_t)(restart + 1) * restart), g(restart + 1), cs(restart), sn(restart), y(restart);
+  int it = 0;
+  for (;;) {
+    // r0 = M^{-1} (b - A x)
+    SPQ_HIP(hipMemcpyAsync(p0, b, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    launch_spmv(s, n, A.rp, A.ci, A.v, x, p0, -1.0, 1.0);
+    precond(p0, V);
+    const double beta = norm_dev(V);
+    if (beta == 0.0) break;
+    launch_axpby(s, n, 0.0, V, 1.0 / beta, V);
+    std::fill(g.begin(), g.end(), 0.0);
+    g[0] = beta;
+    int k = 0;
+    bool stop = false;
+    while (k < restart) {
+      ++it;
+      ++k;
+      double* vk = V + (size_t)n * (k - 1);
+      double* vn = V + (size_t)n * k;
+      launch_spmv(s, n, A.rp, A.ci, A.v, vk, w, 1.0, 0.0);
+      precond(w, vn);
+      // CGS2 against V(:, 0:k)
+      launch_dots(s, n, k, V, n, vn, h);
+      launch_gemv_sub(s, n, k, V, n, h, vn);
+      launch_dots(s, n, k, V, n, vn, h + k);
+      launch_gemv_sub(s, n, k, V, n, h + k, vn);
+      launch_dots(s, n, 1, vn, n, vn, h + 2 * k);
+      SPQ_HIP(hipMemcpyAsync(hh, h, sizeof(double) * (2 * k + 1), hipMemcpyDeviceToHost, s));
+      SPQ_HIP(hipStreamSynchronize(s));
+      double* col = &H[(size_t)(k - 1) * (restart + 1)];
+      for (int i = 0; i < k; ++i) col[i] = hh[i] + hh[k + i];
+      const double hk = std::sqrt(hh[2 * k]);
+      col[k] = hk;
+      if (hk != 0.0) launch_axpby(s, n, 0.0, vn, 1.0 / hk, vn);
+      for (int i = 0; i < k - 1; ++i) {
+        const double a = col[i], bb = col[i + 1];
+        col[i] = cs[i] * a + sn[i] * bb;
+        col[i + 1] = -sn[i] * a + cs[i] * bb;
+      }
+      const double a = col[k - 1], bb = col[k];
+      const double r = std::hypot(a, bb);
+      cs[k - 1] = (r == 0.0) ? 1.0 : a / r;
+      sn[k - 1] = (r == 0.0) ? 0.0 : bb / r;
+      col[k - 1] = r;
+      col[k] = 0.0;
+      g[k] = -sn[k - 1] * g[k - 1];
+      g[k - 1] = cs[k - 1] * g[k - 1];
+      res.est = std::fabs(g[k]) / beta;
+      if (verbose > 1) printf("%d: |Ax-b|/|b| = %3.2e <? %3.2e\n", it, res.est, tol);
+      stop = (k == n || res.est < tol || it >= iters || hk == 0.0);
+      if (stop) break;
+    }
+    // x += V(:, 0:k) y, H(0:k,0:k) y = g(0:k)
+    for (int i = k - 1; i >= 0; --i) {
+      double t = g[i];
+      for (int j = i + 1; j < k; ++j) t -= H[(size_t)j * (restart + 1) + i] * y[j];
+      y[i] = t / H[(size_t)i * (restart + 1) + i];
+    }
+    SPQ_HIP(hipMemcpyAsync(h, y.data(), sizeof(double) * k, hipMemcpyHostToDevice, s));
+    launch_gemv(s, n, k, V, n, h, w);
+    launch_axpby(s, n, 1.0, w, 1.0, x);
+    if (stop) break;
+  }
+  SPQ_HIP(hipStreamSynchronize(s));
+  res.iters = it;
+  res.t_total = wall() - t0;
+  cleanup();
+  return res;
+}
+
+}  // namespace spq

@@ -1,0 +1,32 @@
+// GPU Krylov drivers preconditioned by the spaQR factors (is.cpp:95-346).
+#pragma once
+#include "engine.hpp"
+
+namespace spq {
+
+// Device CSR copy of A (and of A^T for the normal equations), FP64.
+struct DevMatrix {
+  int m = 0, n = 0, nnz = 0;
+  int *rp = nullptr, *ci = nullptr;
+  double* v = nullptr;
+  int *rpt = nullptr, *cit = nullptr;  // CSR of A^T
+  double* vt = nullptr;
+  ~DevMatrix();
+};
+DevMatrix* upload_matrix(const CscMat& A, bool with_transpose);
+
+struct KrylovResult {
+  int iters = 0;
+  double est = 0;       // the solver's own stopping quantity
+  double t_total = 0;   // seconds, device synchronised
+  double t_precond = 0; // seconds spent in preconditioner applies (host clock)
+  int n_precond = 0;
+};
+
+// Left-preconditioned restarted GMRES: r0 = M^{-1}(b - A x), stop when
+// |g_k| / ||r0|| < tol (is.cpp:217-218), Arnoldi by classical Gram-Schmidt
+// with one reorthogonalisation, FP64 throughout.  b, x are device vectors.
+KrylovResult gmres_dev(EngineBase& M, const DevMatrix& A, const double* b, double* x, int iters, int restart,
+                       double tol, int verbose);
+
+}  // namespace spq

@@ -1,0 +1,128 @@
+// Descriptor records shared by the host planner and the batched kernels.
+// Every batched launch takes a device array of descriptors (one per problem)
+// plus, for tiled kernels, exclusive prefix sums of per-problem tile counts,
+// so one launch covers problems of uneven size.
+#pragma once
+#include <stdint.h>
+
+namespace spq {
+
+// 2-D block copy: dst(i,j) = src(i,j) (trans=0) or dst(j,i) = src(i,j)
+// (trans=1) for i < rows, j < cols of src.  Col-major, element offsets.
+// Source and destination may have different element types (double/float).
+struct CopyDesc {
+  const void* src;
+  void* dst;
+  int lds, ldd, rows, cols;
+  int trans;
+  int pad_;
+};
+
+// Batched GEMM: C = alpha * op(A) * op(B) + beta * C, C is m x n.
+// flags: bit0 transA, bit1 transB, bits 4-5 A structure, bits 6-7 B structure
+// (structure applies to the stored matrix: 1 = unit lower trapezoidal,
+// 2 = upper triangular incl. diagonal, 3 = lower triangular incl. diagonal).
+enum : int { kTransA = 1, kTransB = 2 };
+enum : int { kStructNone = 0, kStructUnitLower = 1, kStructUpper = 2, kStructLower = 3 };
+constexpr int gemm_flags(bool ta, bool tb, int sa = 0, int sb = 0) {
+  return (ta ? kTransA : 0) | (tb ? kTransB : 0) | (sa << 4) | (sb << 6);
+}
+template <typename T>
+struct GemmDesc {
+  const T* A;
+  const T* B;
+  T* C;
+  int lda, ldb, ldc;
+  int m, n, k;
+  int flags;
+  T alpha, beta;
+};
+
+// Householder panel block: factor columns [jb, jb+w) of P (m x k, ld), writing
+// reflectors below the diagonal, R on/above it, tau[j] and the nb x nb block
+// T factor of this block into Tb + jb*nb (ld nb).
+template <typename T>
+struct PanelDesc {
+  T* P;
+  T* tau;
+  T* Tb;
+  int ld, m, k;
+  int pad_;
+};
+
+// Block T factors of an existing reflector set (V m x r unit lower, tau).
+template <typename T>
+struct LarftDesc {
+  const T* V;
+  const T* tau;
+  T* Tb;
+  int ldv, m, r;
+  int pad_;
+};
+
+// Row-block triangular solve: B[:, jb:jb+w) <- B[:, jb:jb+w) * R[jb:jb+w, jb:jb+w]^{-1}
+template <typename T>
+struct TrsmDesc {
+  T* B;
+  const T* R;
+  int ldb, ldr, rows, k;
+};
+
+// Truncated column-pivoted QR on C (k x N) stored transposed: Ct is N x k
+// col-major (ld = ldc >= N), so row r of C is column r of Ct.
+template <typename T>
+struct Qp3Desc {
+  T* Ct;
+  T* V;          // k x k workspace: reflector i in column i (unit lower)
+  T* tau;        // k
+  double* vn;    // 2*N workspace (partial norms, FP64)
+  int* pos;      // 2*N workspace (position / pivot bookkeeping)
+  int* rank_out; // device int
+  int ldc, k, N, ldv;
+  double tol;
+};
+
+// Solve-side vector segment (cluster x buffer offset, length).
+struct Seg {
+  int64_t off;
+  int len;
+  int pad_;
+};
+
+// Apply a blocked reflector product to a gathered vector:
+// x = concat(segs) ; x <- H^T x (trans=1) or H x (trans=0) ; scatter back.
+template <typename T>
+struct ReflDesc {
+  const T* V;
+  const T* Tb;
+  int ldv, m, k, nb;
+  int seg0, nseg;  // segment range in the segment table
+  int trans;
+  int pad_;
+};
+
+// x_c[0:k] -= sum_n A_n[0:k, 0:s_n] x_n[0:s_n] ; x_c[0:k] <- R^{-1} x_c[0:k]
+template <typename T>
+struct BwdDesc {
+  const T* R;
+  int ldr, k;
+  int64_t xoff;
+  int blk0, nblk;  // range in the R12 table
+  int trans;       // 1: R^{-T} (transposed solve)
+  int pad_;
+};
+template <typename T>
+struct R12Desc {
+  const T* A;
+  int ld, s;
+  int64_t xoff;
+};
+
+// Vector segment move dst[0:len] = src[0:len] (x-buffer offsets).
+struct MoveDesc {
+  int64_t src, dst;
+  int len;
+  int pad_;
+};
+
+}  // namespace spq

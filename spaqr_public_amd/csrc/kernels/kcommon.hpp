@@ -1,0 +1,64 @@
+// Shared device helpers for the spaQR gfx950 kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace spq {
+
+constexpr int kWave = 64;
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+  for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+  return v;
+}
+
+// Block-wide sum of one value; `red` must hold >= blockDim.x/64 entries.
+template <typename T>
+__device__ __forceinline__ T block_sum(T v, T* red) {
+  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave, nw = blockDim.x / kWave;
+  v = wave_sum(v);
+  __syncthreads();
+  if (lane == 0) red[w] = v;
+  __syncthreads();
+  T s = 0;
+  for (int i = 0; i < nw; ++i) s += red[i];
+  return s;
+}
+
+// Block-wide sums of N values per thread (N small).  `red` holds N*nwaves.
+template <typename T, int N>
+__device__ __forceinline__ void block_sum_n(T (&v)[N], T* red, int n) {
+  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave, nw = blockDim.x / kWave;
+  __syncthreads();
+#pragma unroll
+  for (int c = 0; c < N; ++c) {
+    if (c < n) {
+      T s = wave_sum(v[c]);
+      if (lane == 0) red[c * nw + w] = s;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int c = 0; c < N; ++c) {
+    if (c < n) {
+      T s = 0;
+      for (int i = 0; i < nw; ++i) s += red[c * nw + i];
+      v[c] = s;
+    }
+  }
+}
+
+// Find the descriptor owning flattened tile `t` given exclusive prefix sums
+// `start[0..n)` (start[0] == 0, nondecreasing).
+__device__ __forceinline__ int find_owner(const int* start, int n, int t) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (start[mid] <= t) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+}  // namespace spq

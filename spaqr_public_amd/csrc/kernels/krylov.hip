@@ -1,0 +1,83 @@
+// FP64 kernels for the Krylov drivers: CSR SpMV, Gram-Schmidt GEMVs and
+// vector updates.  Reductions are deterministic (fixed workgroup order, no
+// atomics): one workgroup per output dot product.
+#include "kcommon.hpp"
+#include "descs.hpp"
+#include "launch.hpp"
+
+namespace spq {
+
+// y = alpha * A x + beta * y  (CSR, one wavefront per row)
+__global__ __launch_bounds__(256) void k_spmv(int n, const int* __restrict__ rp, const int* __restrict__ ci,
+                                              const double* __restrict__ v, const double* __restrict__ x,
+                                              double* __restrict__ y, double alpha, double beta) {
+  const int lane = threadIdx.x & 63;
+  const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (row >= n) return;
+  double s = 0;
+  for (int p = rp[row] + lane; p < rp[row + 1]; p += 64) s += v[p] * x[ci[p]];
+  s = wave_sum(s);
+  if (lane == 0) y[row] = alpha * s + (beta == 0.0 ? 0.0 : beta * y[row]);
+}
+
+// out[j] = V(:, j) . w  for j < ncol (V is n x ncol, ld)
+__global__ __launch_bounds__(256) void k_dots(int n, const double* __restrict__ V, int64_t ld,
+                                              const double* __restrict__ w, double* __restrict__ out) {
+  __shared__ double red[4];
+  const int j = blockIdx.x;
+  const double* __restrict__ vj = V + (int64_t)j * ld;
+  double s = 0;
+  for (int i = threadIdx.x; i < n; i += 256) s += vj[i] * w[i];
+  s = block_sum(s, red);
+  if (threadIdx.x == 0) out[j] = s;
+}
+
+// w -= V(:, 0:ncol) h
+__global__ __launch_bounds__(256) void k_gemv_sub(int n, int ncol, const double* __restrict__ V, int64_t ld,
+                                                  const double* __restrict__ h, double* __restrict__ w) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  double s = 0;
+  for (int j = 0; j < ncol; ++j) s += V[i + (int64_t)j * ld] * h[j];
+  w[i] -= s;
+}
+
+// y = a*x + b*y
+__global__ void k_axpby(int n, double a, const double* __restrict__ x, double b, double* __restrict__ y) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n) y[i] = a * x[i] + (b == 0.0 ? 0.0 : b * y[i]);
+}
+
+// y = V(:, 0:ncol) h   (overwrite)
+__global__ __launch_bounds__(256) void k_gemv(int n, int ncol, const double* __restrict__ V, int64_t ld,
+                                              const double* __restrict__ h, double* __restrict__ y) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  double s = 0;
+  for (int j = 0; j < ncol; ++j) s += V[i + (int64_t)j * ld] * h[j];
+  y[i] = s;
+}
+
+void launch_spmv(hipStream_t s, int n, const int* rp, const int* ci, const double* v, const double* x, double* y,
+                 double alpha, double beta) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_spmv, dim3((n + 3) / 4), dim3(256), 0, s, n, rp, ci, v, x, y, alpha, beta);
+}
+void launch_dots(hipStream_t s, int n, int ncol, const double* V, int64_t ld, const double* w, double* out) {
+  if (ncol <= 0) return;
+  hipLaunchKernelGGL(k_dots, dim3(ncol), dim3(256), 0, s, n, V, ld, w, out);
+}
+void launch_gemv_sub(hipStream_t s, int n, int ncol, const double* V, int64_t ld, const double* h, double* w) {
+  if (n <= 0 || ncol <= 0) return;
+  hipLaunchKernelGGL(k_gemv_sub, dim3((n + 255) / 256), dim3(256), 0, s, n, ncol, V, ld, h, w);
+}
+void launch_gemv(hipStream_t s, int n, int ncol, const double* V, int64_t ld, const double* h, double* y) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_gemv, dim3((n + 255) / 256), dim3(256), 0, s, n, ncol, V, ld, h, y);
+}
+void launch_axpby(hipStream_t s, int n, double a, const double* x, double b, double* y) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_axpby, dim3((n + 255) / 256), dim3(256), 0, s, n, a, x, b, y);
+}
+
+}  // namespace spq

@@ -1,0 +1,56 @@
+// Host-side launchers of the spaQR gfx950 kernels.  All take device pointers
+// and enqueue on the given stream; descriptor arrays live in device memory.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "descs.hpp"
+
+namespace spq {
+
+// copy.hip
+void launch_copy(hipStream_t s, int elem_src, int elem_dst, const CopyDesc* d, const int* tstart, int nd, int ntiles);
+void launch_init_blocks(hipStream_t s, int elem, const CopyDesc* d, const int* tstart, int nd, int ntiles);
+void launch_zero(hipStream_t s, int elem, void* p, int64_t n);
+
+// gemm.hip
+int gemm_tiles(int m, int n);
+template <typename T>
+void launch_gemm(hipStream_t s, const GemmDesc<T>* d, const int* tstart, int nd, int ntiles);
+
+// panel.hip
+int panel_nb();
+int trsm_rows_per_tile();
+template <typename T>
+void launch_panel(hipStream_t s, const PanelDesc<T>* d, int nd, int jb);
+template <typename T>
+void launch_larft(hipStream_t s, const LarftDesc<T>* d, const int* tstart, int nd, int ntiles);
+template <typename T>
+void launch_trsm_block(hipStream_t s, const TrsmDesc<T>* d, const int* tstart, int nd, int ntiles, int jb);
+
+// qp3.hip
+template <typename T>
+void launch_qp3(hipStream_t s, const Qp3Desc<T>* d, int nd, int kmax);
+
+// solve.hip
+int refl_lds_cap(int elem);
+template <typename T>
+void launch_refl(hipStream_t s, const ReflDesc<T>* d, int nd, const Seg* segs, T* xbuf, T* work, int mmax);
+template <typename T>
+void launch_bwd(hipStream_t s, const BwdDesc<T>* d, int nd, const R12Desc<T>* r12, T* xbuf);
+template <typename T>
+void launch_move(hipStream_t s, const MoveDesc* d, int nd, T* dst, const T* src);
+template <typename T>
+void launch_gather_in(hipStream_t s, const int64_t* dst, const int* src, int n, const double* b, T* xbuf);
+template <typename T>
+void launch_scatter_out(hipStream_t s, const int64_t* src, const int* dst, int n, const T* xbuf, double* x);
+
+// krylov.hip
+void launch_spmv(hipStream_t s, int n, const int* rp, const int* ci, const double* v, const double* x, double* y,
+                 double alpha, double beta);
+void launch_dots(hipStream_t s, int n, int ncol, const double* V, int64_t ld, const double* w, double* out);
+void launch_gemv_sub(hipStream_t s, int n, int ncol, const double* V, int64_t ld, const double* h, double* w);
+void launch_gemv(hipStream_t s, int n, int ncol, const double* V, int64_t ld, const double* h, double* y);
+void launch_axpby(hipStream_t s, int n, double a, const double* x, double b, double* y);
+
+}  // namespace spq

@@ -1,0 +1,223 @@
+// Kernel-level test entry points (libspaqr_ktest.so, loaded only by tests/).
+// Each function uploads host arrays, runs one batched kernel family on the
+// device and downloads the result, so single kernels can be checked against
+// the oracle's LAPACK restatement.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../kernels/descs.hpp"
+#include "../kernels/launch.hpp"
+
+using namespace spq;
+
+namespace {
+thread_local std::string g_err;
+#define KT(call)                                                     \
+  do {                                                               \
+    hipError_t e_ = (call);                                          \
+    if (e_ != hipSuccess) { g_err = hipGetErrorString(e_); return -1; } \
+  } while (0)
+
+template <typename X>
+X* dup(const X* h, size_t n) {
+  X* d = nullptr;
+  if (hipMalloc((void**)&d, sizeof(X) * (n ? n : 1)) != hipSuccess) return nullptr;
+  if (h && n) (void)hipMemcpy(d, h, sizeof(X) * n, hipMemcpyHostToDevice);
+  else (void)hipMemset(d, 0, sizeof(X) * (n ? n : 1));
+  return d;
+}
+template <typename X>
+void back(X* h, const X* d, size_t n) {
+  (void)hipMemcpy(h, d, sizeof(X) * n, hipMemcpyDeviceToHost);
+}
+}  // namespace
+
+extern "C" {
+
+const char* ktest_last_error() { return g_err.c_str(); }
+
+// C = alpha op(A) op(B) + beta C (single problem, double)
+int ktest_gemm(int m, int n, int k, const double* A, int lda, int acols, const double* B, int ldb, int bcols,
+               double* Cm, int ldc, int flags, double alpha, double beta) {
+  double *dA = dup(A, (size_t)lda * acols), *dB = dup(B, (size_t)ldb * bcols), *dC = dup(Cm, (size_t)ldc * n);
+  GemmDesc<double> g{dA, dB, dC, lda, ldb, ldc, m, n, k, flags, alpha, beta};
+  GemmDesc<double>* dg = dup(&g, 1);
+  int tiles = gemm_tiles(m, n), zero = 0;
+  int* ds = dup(&zero, 1);
+  launch_gemm<double>(0, dg, ds, 1, tiles);
+  KT(hipDeviceSynchronize());
+  back(Cm, dC, (size_t)ldc * n);
+  (void)hipFree(dA); (void)hipFree(dB); (void)hipFree(dC); (void)hipFree(dg); (void)hipFree(ds);
+  return 0;
+}
+
+// Blocked Householder QR of one m x k panel (col-major, ld m): A <- V\R,
+// tau (min(m,k)), Tb (nb x roundup(k,nb)), and C (m x nc) <- Q^T C.
+int ktest_qr(int m, int k, double* A, double* tau, double* Tb, int nc, double* Cm) {
+  const int nb = panel_nb(), kk = m < k ? m : k, nblk = (k + nb - 1) / nb;
+  double *dA = dup(A, (size_t)m * k), *dt = dup<double>(nullptr, k), *dT = dup<double>(nullptr, (size_t)nb * nb * nblk);
+  double* dC = dup(Cm, (size_t)m * (nc > 0 ? nc : 1));
+  double *dY = dup<double>(nullptr, (size_t)nb * (k + nc + 1)), *dZ = dup<double>(nullptr, (size_t)nb * (k + nc + 1));
+  PanelDesc<double> pd{dA, dt, dT, m, m, k, 0};
+  PanelDesc<double>* dpd = dup(&pd, 1);
+  int zero = 0;
+  int* ds = dup(&zero, 1);
+  for (int jb = 0; jb < kk; jb += nb) {
+    launch_panel<double>(0, dpd, 1, jb);
+    const int w = (nb < kk - jb) ? nb : kk - jb, rows = m - jb;
+    const double* Vb = dA + jb + (size_t)jb * m;
+    const double* Tbb = dT + (size_t)jb * nb;
+    struct Tg { double* C; int cols; };
+    std::vector<Tg> tg;
+    const int c0 = jb + ((nb < k - jb) ? nb : k - jb);
+    if (k > c0) tg.push_back({dA + jb + (size_t)c0 * m, k - c0});
+    if (nc > 0) tg.push_back({dC + jb, nc});
+    for (auto& t : tg) {
+      GemmDesc<double> g1{Vb, t.C, dY, m, m, nb, w, t.cols, rows, gemm_flags(true, false, kStructUnitLower), 1.0, 0.0};
+      GemmDesc<double> g2{Tbb, dY, dZ, nb, nb, nb, w, t.cols, w, gemm_flags(true, false, kStructUpper), 1.0, 0.0};
+      GemmDesc<double> g3{Vb, dZ, t.C, m, nb, m, rows, t.cols, w, gemm_flags(false, false, kStructUnitLower), -1.0, 1.0};
+      for (auto* g : {&g1, &g2, &g3}) {
+        GemmDesc<double>* dg = dup(g, 1);
+        launch_gemm<double>(0, dg, ds, 1, gemm_tiles(g->m, g->n));
+        KT(hipDeviceSynchronize());
+        (void)hipFree(dg);
+      }
+    }
+  }
+  KT(hipDeviceSynchronize());
+  back(A, dA, (size_t)m * k);
+  back(tau, dt, kk);
+  back(Tb, dT, (size_t)nb * nb * nblk);
+  if (nc > 0) back(Cm, dC, (size_t)m * nc);
+  for (void* p : {(void*)dA, (void*)dt, (void*)dT, (void*)dC, (void*)dY, (void*)dZ, (void*)dpd, (void*)ds}) (void)hipFree(p);
+  return 0;
+}
+
+// Block T factors of V (m x r) and tau.
+int ktest_larft(int m, int r, const double* V, const double* tau, double* Tb) {
+  const int nb = panel_nb(), nblk = (r + nb - 1) / nb;
+  double *dV = dup(V, (size_t)m * r), *dt = dup(tau, r), *dT = dup<double>(nullptr, (size_t)nb * nb * nblk);
+  LarftDesc<double> d{dV, dt, dT, m, m, r, 0};
+  LarftDesc<double>* dd = dup(&d, 1);
+  int zero = 0;
+  int* ds = dup(&zero, 1);
+  launch_larft<double>(0, dd, ds, 1, nblk);
+  KT(hipDeviceSynchronize());
+  back(Tb, dT, (size_t)nb * nb * nblk);
+  for (void* p : {(void*)dV, (void*)dt, (void*)dT, (void*)dd, (void*)ds}) (void)hipFree(p);
+  return 0;
+}
+
+// Truncated QP3 of C (k x N): Ct is N x k (ld N) in/out; V k x k, tau k; returns rank.
+int ktest_qp3(int k, int N, double* Ct, double* V, double* tau, double tol) {
+  double *dC = dup(Ct, (size_t)N * k), *dV = dup<double>(nullptr, (size_t)k * k), *dt = dup<double>(nullptr, k);
+  double* vn = dup<double>(nullptr, 2 * (size_t)N);
+  int* pos = dup<int>(nullptr, 2 * (size_t)N);
+  int* rk = dup<int>(nullptr, 1);
+  Qp3Desc<double> d{dC, dV, dt, vn, pos, rk, N, k, N, k, tol};
+  Qp3Desc<double>* dd = dup(&d, 1);
+  launch_qp3<double>(0, dd, 1, k);
+  KT(hipDeviceSynchronize());
+  int rank = 0;
+  back(&rank, rk, 1);
+  back(Ct, dC, (size_t)N * k);
+  back(V, dV, (size_t)k * k);
+  back(tau, dt, k);
+  for (void* p : {(void*)dC, (void*)dV, (void*)dt, (void*)vn, (void*)pos, (void*)rk, (void*)dd}) (void)hipFree(p);
+  return rank;
+}
+
+// x <- H^T x (trans=1) or H x for blocked reflectors (V m x k, Tb), with x
+// split into `nseg` equal-length segments stored at stride `gap` in a buffer.
+int ktest_refl(int m, int k, const double* V, const double* Tb, double* x, int trans, int nseg, int gap) {
+  const int nb = panel_nb(), nblk = (k + nb - 1) / nb;
+  const int seglen = m / nseg;
+  const size_t xlen = (size_t)nseg * (seglen + gap);
+  double *dV = dup(V, (size_t)m * k), *dT = dup(Tb, (size_t)nb * nb * nblk);
+  std::vector<double> xb(xlen, -7.0);
+  std::vector<Seg> segs;
+  for (int s = 0; s < nseg; ++s) {
+    segs.push_back(Seg{(int64_t)s * (seglen + gap), seglen, 0});
+    std::memcpy(&xb[(size_t)s * (seglen + gap)], x + (size_t)s * seglen, sizeof(double) * seglen);
+  }
+  double* dx = dup(xb.data(), xlen);
+  Seg* dsg = dup(segs.data(), segs.size());
+  ReflDesc<double> d{dV, dT, m, m, k, nb, 0, nseg, trans, 0};
+  ReflDesc<double>* dd = dup(&d, 1);
+  double* work = dup<double>(nullptr, m);
+  launch_refl<double>(0, dd, 1, dsg, dx, work, m);
+  KT(hipDeviceSynchronize());
+  back(xb.data(), dx, xlen);
+  for (int s = 0; s < nseg; ++s)
+    std::memcpy(x + (size_t)s * seglen, &xb[(size_t)s * (seglen + gap)], sizeof(double) * seglen);
+  for (void* p : {(void*)dV, (void*)dT, (void*)dx, (void*)dsg, (void*)dd, (void*)work}) (void)hipFree(p);
+  return 0;
+}
+
+// xs <- R^{-1}(xs - A1 x1) (trans=0) ; or xs <- R^{-T} xs, x1 -= A1^T xs (trans=1)
+int ktest_bwd(int k, const double* R, int s, const double* A1, double* xs, double* x1, int trans) {
+  double *dR = dup(R, (size_t)k * k), *dA = dup(A1, (size_t)k * (s > 0 ? s : 1));
+  std::vector<double> xb((size_t)k + s);
+  std::memcpy(xb.data(), xs, sizeof(double) * k);
+  if (s > 0) std::memcpy(xb.data() + k, x1, sizeof(double) * s);
+  double* dx = dup(xb.data(), xb.size());
+  R12Desc<double> q{dA, k, s, (int64_t)k};
+  R12Desc<double>* dq = dup(&q, 1);
+  BwdDesc<double> d{dR, k, k, 0, 0, s > 0 ? 1 : 0, trans, 0};
+  BwdDesc<double>* dd = dup(&d, 1);
+  launch_bwd<double>(0, dd, 1, dq, dx);
+  KT(hipDeviceSynchronize());
+  back(xb.data(), dx, xb.size());
+  std::memcpy(xs, xb.data(), sizeof(double) * k);
+  if (s > 0) std::memcpy(x1, xb.data() + k, sizeof(double) * s);
+  for (void* p : {(void*)dR, (void*)dA, (void*)dx, (void*)dq, (void*)dd}) (void)hipFree(p);
+  return 0;
+}
+
+// B (rows x k) <- B R^{-1} with the blocked GEMM + row-block solve.
+int ktest_trsm(int rows, int k, const double* R, double* B) {
+  const int nb = panel_nb();
+  double *dR = dup(R, (size_t)k * k), *dB = dup(B, (size_t)rows * k);
+  int zero = 0;
+  int* ds = dup(&zero, 1);
+  for (int jb = 0; jb < k; jb += nb) {
+    const int w = (nb < k - jb) ? nb : k - jb;
+    if (jb > 0) {
+      GemmDesc<double> g{dB, dR + (size_t)jb * k, dB + (size_t)jb * rows, rows, k, rows, rows, w, jb,
+                         gemm_flags(false, false), -1.0, 1.0};
+      GemmDesc<double>* dg = dup(&g, 1);
+      launch_gemm<double>(0, dg, ds, 1, gemm_tiles(rows, w));
+      KT(hipDeviceSynchronize());
+      (void)hipFree(dg);
+    }
+    TrsmDesc<double> t{dB, dR, rows, k, rows, k};
+    TrsmDesc<double>* dt = dup(&t, 1);
+    launch_trsm_block<double>(0, dt, ds, 1, (rows + trsm_rows_per_tile() - 1) / trsm_rows_per_tile(), jb);
+    KT(hipDeviceSynchronize());
+    (void)hipFree(dt);
+  }
+  back(B, dB, (size_t)rows * k);
+  for (void* p : {(void*)dR, (void*)dB, (void*)ds}) (void)hipFree(p);
+  return 0;
+}
+
+// Batched copy round trip: dst = src (trans 0/1) for one block.
+int ktest_copy(int rows, int cols, const double* src, int lds, double* dst, int ldd, int trans) {
+  const int drows = trans ? cols : rows, dcols = trans ? rows : cols;
+  double *ds_ = dup(src, (size_t)lds * cols), *dd = dup(dst, (size_t)ldd * dcols);
+  CopyDesc c{ds_, dd, lds, ldd, rows, cols, trans, 0};
+  CopyDesc* dc = dup(&c, 1);
+  int zero = 0;
+  int* st = dup(&zero, 1);
+  launch_copy(0, 8, 8, dc, st, 1, ((rows + 31) / 32) * ((cols + 31) / 32));
+  KT(hipDeviceSynchronize());
+  back(dst, dd, (size_t)ldd * dcols);
+  (void)drows;
+  for (void* p : {(void*)ds_, (void*)dd, (void*)dc, (void*)st}) (void)hipFree(p);
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,129 @@
+"""The oracle's spaQR restatement, pinned on the CPU:
+  * the reference's only published run (README.md:48-84) reproduced to every
+    printed integer (tests/golden/readme_advdiff2d_128.json);
+  * tol = 0 turns spaQR into an exact sparse QR (spaQR.cpp:150): one apply is
+    a direct solve (closed form against scipy's sparse LU);
+  * partition invariants of nested dissection on A^T A;
+  * both LAPACK backends (own restatement, OpenBLAS) give the same tree.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spl
+
+from spaqr_public_amd import gen
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden", "readme_advdiff2d_128.json")
+
+
+@pytest.fixture(scope="module")
+def readme():
+    with open(GOLD) as f:
+        return json.load(f)
+
+
+@pytest.mark.parametrize("backend", ["own", "openblas"])
+def test_readme_known_answer(oracle, readme, backend):
+    oracle.set_backend(backend)
+    A = gen.config_matrix("advdiff2d_128_central")
+    r = oracle.run_reference_flow(A, 128, 2, 1e-3)
+    oracle.set_backend("own")
+    assert r["nlevels"] == readme["levels"]
+    assert r["levels"][:, 7].astype(int).tolist() == readme["top_sep_after_level"]
+    assert r["top_size"] == readme["top_separator"]
+    for k in ("nnzA", "nnzR", "nnzH", "nnzQ"):
+        assert r[k] == readme[k], k
+    assert abs(r["oneshot_res"] - readme["oneshot_res"]) / readme["oneshot_res"] < 0.01
+    assert r["iters"] == readme["gmres_iterations"]
+    assert r["final_res"] < 1e-12
+
+
+def test_planted_matrix_change_breaks_known_answer(oracle, readme):
+    """The known-answer test discriminates: the upwind discretisation (a
+    plausible generator mistake) does not reproduce the published log."""
+    oracle.set_backend("own")
+    r = oracle.run_reference_flow(gen.config_matrix("advdiff2d_128"), 128, 2, 1e-3)
+    assert r["levels"][:, 7].astype(int).tolist() != readme["top_sep_after_level"]
+    assert r["nnzR"] != readme["nnzR"]
+
+
+@pytest.mark.parametrize("name,n,dim", [("advdiff2d_32", 32, 2), ("advdiff2d_48", 48, 2), ("advdiff3d_12", 12, 3)])
+def test_exact_solve_tol0(oracle, name, n, dim):
+    oracle.set_backend("own")
+    A = gen.config_matrix(name)
+    r = oracle.run_reference_flow(A, n, dim, 0.0)
+    assert r["oneshot_res"] < 1e-13
+    x = r["tree"].solve(r["b"])
+    xs = spl.spsolve(r["A"].tocsc(), r["b"])
+    assert np.linalg.norm(x - xs) / np.linalg.norm(xs) < 1e-12
+    assert r["iters"] <= 1
+
+
+def _partition(oracle, A, n, dim, nlevels):
+    t = oracle.Tree(nlevels, 0)
+    t.configure(1e-2, square=True)
+    t.set_tensor_coords(n, dim)
+    t.partition(A)
+    return t
+
+
+@pytest.mark.parametrize("n,dim", [(32, 2), (12, 3)])
+def test_partition_invariants(oracle, n, dim):
+    A = gen.config_matrix(f"advdiff{dim}d_{n}")
+    N = A.shape[1]
+    nlevels = oracle.default_levels(N)
+    t = _partition(oracle, A, n, dim, nlevels)
+    cperm, rperm = t.perms()
+    assert sorted(cperm.tolist()) == list(range(N))
+    assert np.array_equal(cperm, rperm)  # square: rperm = cperm (tree.cpp:656)
+    tab = t.cluster_table()
+    leaves = tab[tab[:, 0] == 0]
+    # leaf clusters tile the columns in order, each column exactly once
+    starts = leaves[:, 7]
+    sizes = leaves[:, 8]
+    assert starts[0] == 0 and np.all(starts[1:] == starts[:-1] + sizes[:-1]) and starts[-1] + sizes[-1] == N
+    # ClusterID order (cluster.h:104-110): self level never decreases along the order
+    assert np.all(np.diff(leaves[:, 1]) >= 0)
+    # separator property: no A^T A edge between the interiors of two different leaves
+    col_leaf = np.empty(N, dtype=int)
+    for k, (s, z, lvl) in enumerate(zip(starts, sizes, leaves[:, 1])):
+        col_leaf[cperm[s:s + z]] = k if lvl == 0 else -1
+    G = (abs(A).T @ abs(A)).tocoo()
+    a, b = col_leaf[G.row], col_leaf[G.col]
+    assert not np.any((a >= 0) & (b >= 0) & (a != b))
+    # parents: sizes are sums of children at every level
+    for lvl in range(1, nlevels):
+        assert tab[tab[:, 0] == lvl][:, 8].sum() == tab[(tab[:, 0] == lvl - 1) & (tab[:, 1] >= lvl)][:, 8].sum()
+
+
+def test_backends_agree_on_tree(oracle):
+    A = gen.config_matrix("advdiff3d_12")
+    out = {}
+    for be in ("own", "openblas"):
+        oracle.set_backend(be)
+        r = oracle.run_reference_flow(A, 12, 3, 1e-2)
+        out[be] = (r["levels"][:, 7].tolist(), r["tree"].sparsified().tolist(), r["iters"])
+    oracle.set_backend("own")
+    assert out["own"] == out["openblas"]
+
+
+def test_rhs_generator_is_mt19937_uniform(oracle):
+    """random(n, 2021) of util.cpp:943-952: libstdc++ mt19937 + U(-1,1)."""
+    x = oracle.random_vec(5, 2021)
+    assert np.all(np.abs(x) <= 1.0)
+    # mt19937 seeded with 2021: the first raw outputs drive generate_canonical
+    rng = np.random.RandomState(2021)  # same MT19937 state initialisation (init_genrand)
+    raw = rng.randint(0, 2**32, size=4, dtype=np.uint64).astype(np.float64)
+    u0 = (raw[0] + raw[1] * 2.0**32) / 2.0**64
+    assert abs(x[0] - (2.0 * u0 - 1.0)) < 1e-15
+
+
+def test_colscale(oracle):
+    A = gen.config_matrix("advdiff2d_32")
+    As, D = oracle.colscale(A)
+    norms = np.sqrt(np.asarray(As.multiply(As).sum(axis=0))).ravel()
+    assert np.allclose(norms, 10.0, rtol=1e-14)
+    assert np.allclose((A @ sp.diags(D) * 10).toarray(), As.toarray(), rtol=1e-15)
