@@ -135,6 +135,7 @@ int spaqr_solve_device(spaqr_tree* t, const double* b, double* x) {
     EngineBase& e = eng(t);
     e.solve_dev(b, x);
     SPQ_HIP(hipStreamSynchronize(e.stream()));
+    e.check_solve();
   });
 }
 int spaqr_solve(spaqr_tree* t, const double* b, double* x) {
@@ -147,6 +148,7 @@ int spaqr_solve(spaqr_tree* t, const double* b, double* x) {
     SPQ_HIP(hipMemcpy(db, b, sizeof(double) * e.rows(), hipMemcpyHostToDevice));
     e.solve_dev(db, dx);
     SPQ_HIP(hipStreamSynchronize(e.stream()));
+    e.check_solve();
     SPQ_HIP(hipMemcpy(x, dx, sizeof(double) * e.cols(), hipMemcpyDeviceToHost));
   });
 }
@@ -185,6 +187,7 @@ int spaqr_gmres_device(spaqr_tree* t, const spaqr_matrix* A, const double* b, do
     if (!A || !A->d) throw Error(kErrArg, "null matrix");
     if (A->d->m != A->d->n || A->d->m != e.rows()) throw Error(kErrArg, "gmres needs the square matrix of the tree");
     KrylovResult r = gmres_dev(e, *A->d, b, x, iters, restart, tol, verbose);
+    e.check_solve();
     if (iters_out) *iters_out = r.iters;
     if (est_out) *est_out = r.est;
   });
@@ -203,6 +206,7 @@ int spaqr_gmres(spaqr_tree* t, const spaqr_matrix* A, const double* b, double* x
     SPQ_HIP(hipMemcpy(dx, x, sizeof(double) * n, hipMemcpyHostToDevice));
     if (A->d->m != A->d->n || A->d->m != e.rows()) throw Error(kErrArg, "gmres needs the square matrix of the tree");
     KrylovResult r = gmres_dev(e, *A->d, db, dx, iters, restart, tol, verbose);
+    e.check_solve();
     SPQ_HIP(hipMemcpy(x, dx, sizeof(double) * n, hipMemcpyDeviceToHost));
     if (iters_out) *iters_out = r.iters;
     if (est_out) *est_out = r.est;

@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <numeric>
@@ -31,6 +32,11 @@ class Engine final : public EngineBase {
     info_.resize(nlevels);
     nb_ = panel_nb();
     SPQ_HIP(hipHostMalloc((void**)&rank_host_, sizeof(int) * 65536, hipHostMallocDefault));
+    qp3_err_ = (int*)perm_.alloc(256);
+    SPQ_HIP(hipMemset(qp3_err_, 0, 256));
+    if (const char* e = std::getenv("SPAQR_QP3_MAX_WG")) qp3_max_wg_ = std::max(1, std::atoi(e));
+    SPQ_HIP(hipDeviceGetAttribute(&num_cus_, hipDeviceAttributeMultiprocessorCount, device));
+    if (const char* e = std::getenv("SPAQR_SOLVE_STAGES")) stage_prof_ = std::atoi(e) != 0;
   }
   ~Engine() override {
     reset_kernel_stats();
@@ -61,6 +67,13 @@ class Engine final : public EngineBase {
   int precision_bytes() const override { return (int)sizeof(T); }
 
   void set_kernel_timing(int on) override { kprof_ = on != 0; }
+  void check_solve() override {
+    if (!solve_err_) return;
+    int e = 0;
+    SPQ_HIP(hipMemcpyAsync(&e, solve_err_, sizeof(int), hipMemcpyDeviceToHost, stream_));
+    SPQ_HIP(hipStreamSynchronize(stream_));
+    if (e) throw Error(kErrNumeric, "solve: an inter-workgroup barrier wait timed out");
+  }
   void reset_kernel_stats() override {
     for (auto& kv : kev_)
       for (auto& e : kv.second.ev) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
@@ -374,6 +387,8 @@ class Engine final : public EngineBase {
   T* xbuf_ = nullptr;
   hipStream_t stream_ = nullptr;
   Arena perm_, scratch_, ws_;
+  int* qp3_err_ = nullptr;
+  int qp3_max_wg_ = 64;
   Stager stager_;
   int* rank_host_ = nullptr;
   bool factored_ = false;
@@ -384,12 +399,17 @@ class Engine final : public EngineBase {
   // solve program (device descriptor tables, built once after factorize)
   struct Stage {
     int type;  // 0 refl, 1 bwd, 2 move
-    int nd = 0, mmax = 0;
+    int nd = 0, mmax = 0, nwg_total = 0;
     const ReflDesc<T>* refl = nullptr;
     const BwdDesc<T>* bwd = nullptr;
     const MoveDesc* mv = nullptr;
+    const int* wstart = nullptr;
     T* work = nullptr;
   };
+  unsigned int* solve_sync_ = nullptr;  // barrier counters of multi-workgroup solve ops
+  int n_solve_sync_ = 0;
+  int* solve_err_ = nullptr;
+  int num_cus_ = 256;
   std::vector<Stage> fwd_, bwd_;
   const Seg* segs_dev_ = nullptr;
   const R12Desc<T>* r12_dev_ = nullptr;
@@ -401,7 +421,8 @@ class Engine final : public EngineBase {
   hipGraphExec_t graph_exec_fwd_ = nullptr;
   double* bin_ = nullptr;
   double* xout_ = nullptr;
-  void run_program(const double* b, double* x);
+  void run_program(const double* b, double* x, bool prof = false);
+  bool stage_prof_ = false;  // SPAQR_SOLVE_STAGES=1: time every solve stage (uncaptured)
 };
 
 // ======================================================================
@@ -783,8 +804,13 @@ void Engine<T>::sparsify_level(const std::vector<int>& cs) {
     std::vector<Qp3Desc<T>> qd;
     int kmax = 0;
     int* ranks_dev = (int*)scratch_.alloc(sizeof(int) * S.size());
-    for (size_t i = 0; i < S.size(); ++i) {
-      if (wave[i] != w) continue;
+    std::vector<int> wave_idx;
+    for (size_t i = 0; i < S.size(); ++i)
+      if (wave[i] == w) { wave_idx.push_back((int)i); kmax = std::max(kmax, cl_[S[i]].csize); }
+    // workgroups per problem: >= 256 owned columns each, only when the
+    // reflector fits in LDS (multi-workgroup problems keep it there)
+    const bool lds_ok = sizeof(T) * (size_t)std::max(kmax, 1) <= (size_t)48 * 1024;
+    for (int i : wave_idx) {
       const int c = S[i];
       Job jb;
       jb.c = c; jb.k = cl_[c].csize; jb.smp = 0;
@@ -795,11 +821,12 @@ void Engine<T>::sparsify_level(const std::vector<int>& cs) {
         if (!cl_[ed_[e].n2].elim && ed_[e].n2 != c) { jb.amp.push_back(e); jb.smp += ed_[e].a.rows; }
       jb.N = jb.smp + spm;
       const int N = std::max(jb.N, 1);
+      const int nwg = lds_ok ? std::max(1, std::min(qp3_max_wg_, jb.N / 256)) : 1;
       jb.Ct = alloc_scr((size_t)N * jb.k);
       jb.V = alloc_scr((size_t)jb.k * jb.k);
       jb.tau = alloc_scr(jb.k);
       jb.vn = (double*)scratch_.alloc(sizeof(double) * 2 * N);
-      jb.pos = (int*)scratch_.alloc(sizeof(int) * 2 * N);
+      jb.pos = (int*)scratch_.alloc(sizeof(int) * 2 * (size_t)N * nwg);
       int off = 0;
       for (int e : jb.amp) {
         const Block& b = ed_[e].a;
@@ -813,10 +840,14 @@ void Engine<T>::sparsify_level(const std::vector<int>& cs) {
       }
       Qp3Desc<T> d{};
       d.Ct = jb.Ct; d.V = jb.V; d.tau = jb.tau; d.vn = jb.vn; d.pos = jb.pos;
+      d.part_v = (double*)scratch_.alloc(sizeof(double) * 2 * nwg);
+      d.part_p = (int*)scratch_.alloc(sizeof(int) * 4 * nwg);
+      d.beta = (double*)scratch_.alloc(sizeof(double) * std::max(jb.k, 1));
+      d.sync = (unsigned int*)scratch_.alloc(256);
+      d.err = qp3_err_;
       d.rank_out = ranks_dev + jobs.size();
-      d.ldc = N; d.k = jb.k; d.N = jb.N; d.ldv = jb.k; d.tol = tol_;
+      d.ldc = N; d.k = jb.k; d.N = jb.N; d.ldv = jb.k; d.tol = tol_; d.nwg = nwg;
       qd.push_back(d);
-      kmax = std::max(kmax, jb.k);
       const int kk = std::min(jb.k, jb.N);
       flops_ += 4.0 * jb.k * jb.N * kk - 2.0 * (jb.k + jb.N) * kk * kk + 4.0 / 3.0 * kk * kk * kk;
       jobs.push_back(std::move(jb));
@@ -826,15 +857,43 @@ void Engine<T>::sparsify_level(const std::vector<int>& cs) {
     if (tol_ >= 1.0) {
       std::fill(ranks.begin(), ranks.end(), 0);  // util.cpp:798-799
     } else {
-      double qfl = 0, qby = 0;
-      for (auto& d : qd) {
-        qfl += 4.0 * d.k * d.N * std::min(d.k, d.N);  // upper bound: full rank steps
-        qby += 2.0 * sizeof(T) * (double)d.k * d.N;
+      // barrier counters start at zero
+      for (auto& d : qd) launch_zero(stream_, 4, d.sync, 1);
+      // single-workgroup problems in one plain launch; multi-workgroup ones
+      // cooperatively, packed within the resident-workgroup capacity
+      const int cap = qp3_capacity<T>(kmax);
+      std::vector<std::vector<Qp3Desc<T>>> groups(1);
+      std::vector<Qp3Desc<T>> multi;
+      for (auto& d : qd) (d.nwg == 1 ? groups[0] : multi).push_back(d);
+      std::vector<int> coop_flag{0};
+      int used = 0;
+      for (auto& d : multi) {
+        if (d.nwg > cap) d.nwg = std::max(1, cap);
+        if (coop_flag.size() == 1 || used + d.nwg > cap) { groups.push_back({}); coop_flag.push_back(1); used = 0; }
+        groups.back().push_back(d);
+        used += d.nwg;
       }
-      const Qp3Desc<T>* dqd = stager_.push(qd);
-      timed("qp3", qfl, qby, [&] { launch_qp3<T>(stream_, dqd, (int)qd.size(), kmax); });
+      for (size_t q = 0; q < groups.size(); ++q) {
+        auto& gq = groups[q];
+        if (gq.empty()) continue;
+        std::vector<int> cnt, st;
+        double qfl = 0, qby = 0;
+        for (auto& d : gq) {
+          cnt.push_back(d.nwg);
+          qfl += 4.0 * d.k * d.N * std::min(d.k, d.N);  // upper bound: full rank steps
+          qby += 2.0 * sizeof(T) * (double)d.k * d.N;
+        }
+        const int nt = prefix_tiles(cnt, st);
+        const Qp3Desc<T>* dqd = stager_.push(gq);
+        const int* dst = stager_.push(st);
+        int rc = 0;
+        timed("qp3", qfl, qby, [&] { rc = launch_qp3<T>(stream_, dqd, dst, (int)gq.size(), nt, kmax, coop_flag[q]); });
+        if (rc != 0) throw Error(kErrHip, std::string("qp3 launch failed: ") + hipGetErrorString((hipError_t)rc));
+      }
       SPQ_HIP(hipMemcpyAsync(rank_host_, ranks_dev, sizeof(int) * jobs.size(), hipMemcpyDeviceToHost, stream_));
+      SPQ_HIP(hipMemcpyAsync(rank_host_ + 65535, qp3_err_, sizeof(int), hipMemcpyDeviceToHost, stream_));
       sync();
+      if (rank_host_[65535] != 0) throw Error(kErrNumeric, "qp3: an inter-workgroup barrier wait timed out");
       for (size_t q = 0; q < jobs.size(); ++q) ranks[q] = rank_host_[q];
     }
     std::vector<CopyDesc> scatter;
@@ -1129,27 +1188,123 @@ void Engine<T>::build_solve_program() {
   }
   segs_dev_ = stager_.upload(perm_, segs);
   r12_dev_ = stager_.upload(perm_, r12);
+  // Multi-workgroup split of large ops: row slices of >= 256 rows (reflector
+  // applies) or >= 4 blocks of 32 (triangular solves); the stage's
+  // multi-workgroup ops come first and use at most one workgroup per CU.
+  int nsync = 0;
+  auto plan_nwg = [&](std::vector<int>& want) {
+    int total = 0;
+    for (int v : want) if (v > 1) total += v;
+    if (total > num_cus_) {
+      const double f = (double)num_cus_ / total;
+      for (int& v : want) if (v > 1) v = std::max(2, (int)(v * f));
+      total = 0;
+      for (int v : want) if (v > 1) total += v;
+      for (int& v : want) if (v > 1 && total > num_cus_) { total -= v - 1; v = 1; }
+    }
+  };
   auto finish = [&](std::vector<HStage>& hs, std::vector<Stage>& out) {
     for (auto& h : hs) {
       Stage s;
       s.type = h.type;
       s.mmax = h.mmax;
+      std::vector<int> cnt, st;
       if (h.type == 0) {
-        s.nd = (int)h.refl.size();
-        s.refl = stager_.upload(perm_, h.refl);
-        if (h.mmax > refl_lds_cap(sizeof(T))) s.work = alloc_perm((size_t)s.nd * h.mmax);
+        std::vector<int> want;
+        for (auto& d : h.refl) want.push_back(d.m >= 2048 ? std::min(32, d.m / 256) : 1);
+        plan_nwg(want);
+        std::vector<int> order(h.refl.size());
+        std::iota(order.begin(), order.end(), 0);
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return (want[a] > 1) > (want[b] > 1); });
+        std::vector<ReflDesc<T>> ds;
+        int mmax1 = 0;
+        for (int i : order) {
+          ReflDesc<T> d = h.refl[i];
+          d.nwg = want[i];
+          if (d.nwg > 1) {
+            d.work = alloc_perm(d.m);
+            d.part = alloc_perm((size_t)2 * d.nwg * 32);
+            d.sync = (unsigned int*)(intptr_t)(nsync++);  // index, resolved below
+          } else {
+            mmax1 = std::max(mmax1, d.m);
+          }
+          ds.push_back(d);
+          cnt.push_back(d.nwg);
+        }
+        s.mmax = mmax1;
+        s.nwg_total = prefix_tiles(cnt, st);
+        s.nd = (int)ds.size();
+        if (mmax1 > refl_lds_cap(sizeof(T))) s.work = alloc_perm((size_t)s.nwg_total * mmax1);
+        h.refl = ds;
       } else if (h.type == 1) {
-        s.nd = (int)h.bwd.size();
-        s.bwd = stager_.upload(perm_, h.bwd);
+        std::vector<int> want;
+        for (auto& d : h.bwd) want.push_back(d.k >= 512 ? std::min(32, (d.k / 32) / 4) : 1);
+        plan_nwg(want);
+        std::vector<int> order(h.bwd.size());
+        std::iota(order.begin(), order.end(), 0);
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return (want[a] > 1) > (want[b] > 1); });
+        std::vector<BwdDesc<T>> ds;
+        for (int i : order) {
+          BwdDesc<T> d = h.bwd[i];
+          d.nwg = want[i];
+          if (d.nwg > 1) d.sync = (unsigned int*)(intptr_t)(nsync++);
+          ds.push_back(d);
+          cnt.push_back(d.nwg);
+        }
+        s.nwg_total = prefix_tiles(cnt, st);
+        s.nd = (int)ds.size();
+        h.bwd = ds;
       } else {
         s.nd = (int)h.mv.size();
-        s.mv = stager_.upload(perm_, h.mv);
       }
+      if (h.type != 2) s.wstart = stager_.upload(perm_, st);
       out.push_back(s);
     }
   };
   finish(hf, fwd_);
   finish(hb, bwd_);
+  n_solve_sync_ = nsync;
+  solve_sync_ = (unsigned int*)perm_.alloc(sizeof(unsigned int) * std::max(nsync, 1));
+  solve_err_ = (int*)perm_.alloc(256);
+  SPQ_HIP(hipMemsetAsync(solve_err_, 0, 256, stream_));
+  auto upload_stage = [&](std::vector<HStage>& hs, std::vector<Stage>& st) {
+    for (size_t i = 0; i < hs.size(); ++i) {
+      auto& h = hs[i];
+      if (h.type == 0) {
+        for (auto& d : h.refl) {
+          if (d.nwg > 1) d.sync = solve_sync_ + (intptr_t)d.sync;
+          d.err = solve_err_;
+        }
+        st[i].refl = stager_.upload(perm_, h.refl);
+      } else if (h.type == 1) {
+        for (auto& d : h.bwd) {
+          if (d.nwg > 1) d.sync = solve_sync_ + (intptr_t)d.sync;
+          d.err = solve_err_;
+        }
+        st[i].bwd = stager_.upload(perm_, h.bwd);
+      } else {
+        st[i].mv = stager_.upload(perm_, h.mv);
+      }
+    }
+  };
+  upload_stage(hf, fwd_);
+  upload_stage(hb, bwd_);
+  if (verbose_ > 1) {
+    auto dump = [&](const char* dir, std::vector<HStage>& hs) {
+      for (auto& h : hs) {
+        int mm = 0, kk = 0;
+        double bytes = 0;
+        if (h.type == 0)
+          for (auto& d : h.refl) { mm = std::max(mm, d.m); kk = std::max(kk, d.k); bytes += (double)d.m * d.k; }
+        if (h.type == 1)
+          for (auto& d : h.bwd) { kk = std::max(kk, d.k); bytes += 0.5 * d.k * d.k; }
+        fprintf(stderr, "solve %s stage type %d ops %zu max_m %d max_k %d factor_entries %.3g\n", dir, h.type,
+                h.type == 0 ? h.refl.size() : (h.type == 1 ? h.bwd.size() : h.mv.size()), mm, kk, bytes);
+      }
+    };
+    dump("fwd", hf);
+    dump("bwd", hb);
+  }
   // entry / exit maps (set_vector / extract_vector, cluster.cpp:125-156)
   std::vector<int64_t> idst, osrc;
   std::vector<int> isrc, odst;
@@ -1166,16 +1321,41 @@ void Engine<T>::build_solve_program() {
 }
 
 template <typename T>
-void Engine<T>::run_program(const double* b, double* x) {
+void Engine<T>::run_program(const double* b, double* x, bool prof) {
   launch_zero(stream_, sizeof(T), xbuf_, xsize_);
+  if (n_solve_sync_ > 0) launch_zero(stream_, 4, solve_sync_, n_solve_sync_);
   launch_gather_in<T>(stream_, in_dst_, in_src_, n_in_, b, xbuf_);
+  std::vector<hipEvent_t> ev;
   auto run = [&](const Stage& s) {
-    if (s.type == 0) launch_refl<T>(stream_, s.refl, s.nd, segs_dev_, xbuf_, s.work, s.mmax);
-    else if (s.type == 1) launch_bwd<T>(stream_, s.bwd, s.nd, r12_dev_, xbuf_);
+    if (prof) {
+      ev.emplace_back();
+      SPQ_HIP(hipEventCreate(&ev.back()));
+      SPQ_HIP(hipEventRecord(ev.back(), stream_));
+    }
+    if (s.type == 0) launch_refl<T>(stream_, s.refl, s.wstart, s.nd, s.nwg_total, segs_dev_, xbuf_, s.work, s.mmax);
+    else if (s.type == 1) launch_bwd<T>(stream_, s.bwd, s.wstart, s.nd, s.nwg_total, r12_dev_, xbuf_);
     else launch_move<T>(stream_, s.mv, s.nd, xbuf_, xbuf_);
   };
   for (auto& s : fwd_) run(s);
   for (auto& s : bwd_) run(s);
+  if (prof) {
+    ev.emplace_back();
+    SPQ_HIP(hipEventCreate(&ev.back()));
+    SPQ_HIP(hipEventRecord(ev.back(), stream_));
+    SPQ_HIP(hipStreamSynchronize(stream_));
+    double tot = 0;
+    for (size_t i = 0; i + 1 < ev.size(); ++i) {
+      float ms = 0;
+      SPQ_HIP(hipEventElapsedTime(&ms, ev[i], ev[i + 1]));
+      tot += ms;
+      const bool f = i < fwd_.size();
+      const Stage& s = f ? fwd_[i] : bwd_[i - fwd_.size()];
+      fprintf(stderr, "stage %s %3zu type %d ops %5d wgs %5d mmax %6d ms %.4f\n", f ? "fwd" : "bwd", i, s.type, s.nd,
+              s.type == 2 ? s.nd : s.nwg_total, s.mmax, ms);
+    }
+    fprintf(stderr, "stages total ms %.3f\n", tot);
+    for (auto e : ev) (void)hipEventDestroy(e);
+  }
   launch_scatter_out<T>(stream_, out_src_, out_dst_, n_out_, xbuf_, x);
 }
 
@@ -1185,6 +1365,11 @@ void Engine<T>::solve_dev(const double* b, double* x) {
   // b is taken by value (tree.cpp:1440): copy into the program's fixed input,
   // replay the captured apply, copy the fixed output out.
   SPQ_HIP(hipMemcpyAsync(bin_, b, sizeof(double) * nrows_, hipMemcpyDeviceToDevice, stream_));
+  if (stage_prof_) {
+    run_program(bin_, xout_, true);
+    SPQ_HIP(hipMemcpyAsync(x, xout_, sizeof(double) * ncols_, hipMemcpyDeviceToDevice, stream_));
+    return;
+  }
   if (!graph_exec_fwd_) {
     hipGraph_t g = nullptr;
     SPQ_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));

@@ -65,6 +65,8 @@ class EngineBase {
   virtual int precision_bytes() const = 0;
   // kernel-family timing with HIP events (off by default)
   virtual void set_kernel_timing(int on) = 0;
+  // throws if a multi-workgroup solve op reported a barrier timeout
+  virtual void check_solve() = 0;
   virtual std::vector<KernelStat> kernel_stats() = 0;
   virtual void reset_kernel_stats() = 0;
 };

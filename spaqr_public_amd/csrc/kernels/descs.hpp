@@ -69,17 +69,25 @@ struct TrsmDesc {
 };
 
 // Truncated column-pivoted QR on C (k x N) stored transposed: Ct is N x k
-// col-major (ld = ldc >= N), so row r of C is column r of Ct.
+// col-major (ld = ldc >= N), so row r of C is column r of Ct.  A problem is
+// processed by `nwg` workgroups that own contiguous column slices and meet at
+// one inter-workgroup barrier per pivot step (nwg == 1: no barrier).
 template <typename T>
 struct Qp3Desc {
   T* Ct;
-  T* V;          // k x k workspace: reflector i in column i (unit lower)
-  T* tau;        // k
-  double* vn;    // 2*N workspace (partial norms, FP64)
-  int* pos;      // 2*N workspace (position / pivot bookkeeping)
-  int* rank_out; // device int
+  T* V;               // k x k (ld ldv): reflector i in column i (unit lower)
+  T* tau;             // k
+  double* vn;         // 2*N partial norms (FP64)
+  int* pos;           // nwg * 2N: per-workgroup copy of perm / ipos
+  double* part_v;     // 2 * nwg partial argmax values (double-buffered by step parity)
+  int* part_p;        // 2 * nwg * 2: (position, column) of the partial argmax
+  double* beta;       // k: R(i,i) of each step
+  unsigned int* sync; // barrier counter (zero on entry)
+  int* err;           // set when a barrier wait gives up
+  int* rank_out;      // device int
   int ldc, k, N, ldv;
   double tol;
+  int nwg, pad_;
 };
 
 // Solve-side vector segment (cluster x buffer offset, length).
@@ -98,7 +106,11 @@ struct ReflDesc {
   int ldv, m, k, nb;
   int seg0, nseg;  // segment range in the segment table
   int trans;
-  int pad_;
+  int nwg;         // workgroups (row slices); > 1 uses work/part/sync
+  T* work;         // m: gathered vector (multi-workgroup form)
+  T* part;         // 2 * nwg * 32 partial V_b^T x sums
+  unsigned int* sync;
+  int* err;
 };
 
 // x_c[0:k] -= sum_n A_n[0:k, 0:s_n] x_n[0:s_n] ; x_c[0:k] <- R^{-1} x_c[0:k]
@@ -109,7 +121,9 @@ struct BwdDesc {
   int64_t xoff;
   int blk0, nblk;  // range in the R12 table
   int trans;       // 1: R^{-T} (transposed solve)
-  int pad_;
+  int nwg;         // workgroups (row slices of x_c); > 1 uses sync
+  unsigned int* sync;
+  int* err;
 };
 template <typename T>
 struct R12Desc {

@@ -50,6 +50,24 @@ __device__ __forceinline__ void block_sum_n(T (&v)[N], T* red, int n) {
   }
 }
 
+// Inter-workgroup barrier on a monotonically increasing counter (target =
+// arrivals expected so far).  All workgroups of the group must be resident;
+// the wait is bounded so the kernel always terminates (err is set instead).
+__device__ __forceinline__ void group_barrier(unsigned int* ctr, unsigned int target, int* err) {
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    atomicAdd(ctr, 1u);
+    unsigned int spins = 0;
+    while (__hip_atomic_load(ctr, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+      __builtin_amdgcn_s_sleep(2);
+      if (++spins > (1u << 24)) { atomicOr(err, 1); break; }
+    }
+  }
+  __syncthreads();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+}
+
 // Find the descriptor owning flattened tile `t` given exclusive prefix sums
 // `start[0..n)` (start[0] == 0, nondecreasing).
 __device__ __forceinline__ int find_owner(const int* start, int n, int t) {

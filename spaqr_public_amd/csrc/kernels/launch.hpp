@@ -29,15 +29,23 @@ template <typename T>
 void launch_trsm_block(hipStream_t s, const TrsmDesc<T>* d, const int* tstart, int nd, int ntiles, int jb);
 
 // qp3.hip
+int qp3_threads();
 template <typename T>
-void launch_qp3(hipStream_t s, const Qp3Desc<T>* d, int nd, int kmax);
+int qp3_capacity(int kmax);
+// wstart: exclusive prefix sums of the descriptors' nwg; coop != 0 launches
+// cooperatively (required when any descriptor has nwg > 1).  Returns a
+// hipError_t value.
+template <typename T>
+int launch_qp3(hipStream_t s, const Qp3Desc<T>* d, const int* wstart, int nd, int nwg_total, int kmax, int coop);
 
 // solve.hip
 int refl_lds_cap(int elem);
 template <typename T>
-void launch_refl(hipStream_t s, const ReflDesc<T>* d, int nd, const Seg* segs, T* xbuf, T* work, int mmax);
+void launch_refl(hipStream_t s, const ReflDesc<T>* d, const int* wstart, int nd, int nwg_total, const Seg* segs,
+                 T* xbuf, T* work, int mmax);
 template <typename T>
-void launch_bwd(hipStream_t s, const BwdDesc<T>* d, int nd, const R12Desc<T>* r12, T* xbuf);
+void launch_bwd(hipStream_t s, const BwdDesc<T>* d, const int* wstart, int nd, int nwg_total, const R12Desc<T>* r12,
+                T* xbuf);
 template <typename T>
 void launch_move(hipStream_t s, const MoveDesc* d, int nd, T* dst, const T* src);
 template <typename T>

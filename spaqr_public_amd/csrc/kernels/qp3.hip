@@ -2,8 +2,11 @@
 //
 // C (k x N, usually wide) is stored transposed: Ct is N x k col-major, so row
 // r of C is a contiguous column of Ct and a wavefront reading C(r, j..j+63)
-// is coalesced.  Threads own columns of C.  Column data never moves: the
-// LAPACK column swap is kept as a position permutation (perm/ipos), so the
+// is coalesced.  Threads own columns of C; a problem is split over `nwg`
+// workgroups owning contiguous column slices, which meet at one
+// inter-workgroup barrier per pivot step (a single workgroup needs none).
+// Column data never moves: the LAPACK column swap is kept as a position
+// permutation (each workgroup applies the same swaps to its own copy), so the
 // pivot rule is exactly idamax over positions (largest partial norm, first
 // position on ties) and the partial-norm downdate is the dlaqp2 rule with
 // tol3z = sqrt(eps) recomputation.  Norms, the pivot choice and the rank test
@@ -15,6 +18,11 @@
 // C(0:rank, j) holds triu(R(0:rank,:)) P^T (pivoted columns have R entries
 // above and zero below their pivot row), V(:, i) the unit-lower reflector i
 // and tau(i) its scalar.
+//
+// Every workgroup of a multi-workgroup problem must be resident at once: the
+// host launches those problems cooperatively within the device's capacity.
+// Barrier waits are bounded, so a kernel always terminates; a wait that gives
+// up sets `err` and the host reports it.
 #include <limits.h>
 
 #include "kcommon.hpp"
@@ -25,69 +33,86 @@ namespace spq {
 
 constexpr int kQThreads = 256;
 
-__device__ __forceinline__ void argmax_combine(double& v, int& p, double ov, int op) {
-  if (ov > v || (ov == v && op < p)) { v = ov; p = op; }
+__device__ __forceinline__ void argmax_combine(double& v, int& p, int& c, double ov, int op, int oc) {
+  if (ov > v || (ov == v && op < p)) { v = ov; p = op; c = oc; }
 }
 
 template <typename T>
-__global__ __launch_bounds__(kQThreads) void k_qp3(const Qp3Desc<T>* __restrict__ descs, int use_lds) {
+__global__ __launch_bounds__(kQThreads) void k_qp3(const Qp3Desc<T>* __restrict__ descs, const int* __restrict__ wstart,
+                                                   int nd, int use_lds) {
   extern __shared__ unsigned char smem_raw[];
   __shared__ double redv[kQThreads / kWave];
-  __shared__ int redp[kQThreads / kWave];
+  __shared__ int redp[kQThreads / kWave], redc[kQThreads / kWave];
   __shared__ double sred[kQThreads / kWave];
   __shared__ int s_piv, s_pos, s_stop;
 
-  const Qp3Desc<T> d = descs[blockIdx.x];
+  const int p = find_owner(wstart, nd, blockIdx.x);
+  const Qp3Desc<T> d = descs[p];
+  const int g = blockIdx.x - wstart[p], nwg = d.nwg;
   const int k = d.k, N = d.N, tid = threadIdx.x;
+  const int c0 = (int)((int64_t)g * N / nwg), c1 = (int)((int64_t)(g + 1) * N / nwg);
   const int64_t ldc = d.ldc;
   T* __restrict__ Ct = d.Ct;
   double* __restrict__ vn1 = d.vn;
   double* __restrict__ vn2 = d.vn + N;
-  int* __restrict__ ipos = d.pos;
-  int* __restrict__ perm = d.pos + N;
+  int* __restrict__ ipos = d.pos + (int64_t)g * 2 * N;
+  int* __restrict__ perm = ipos + N;
   const int kmin = min(k, N);
   const double tol3z = sqrt(ldexp(1.0, -53));
+  unsigned int target = 0;
 
-  for (int j = tid; j < N; j += kQThreads) {
+  for (int j = tid; j < N; j += kQThreads) { ipos[j] = j; perm[j] = j; }
+  // initial norms and the local argmax of step 0
+  double bv = -2.0;
+  int bp = INT_MAX, bc = -1;
+  for (int j = c0 + tid; j < c1; j += kQThreads) {
     double s = 0;
     for (int r = 0; r < k; ++r) { const double v = (double)Ct[j + r * ldc]; s += v * v; }
-    vn1[j] = sqrt(s);
-    vn2[j] = vn1[j];
-    ipos[j] = j;
-    perm[j] = j;
+    const double nv = sqrt(s);
+    vn1[j] = nv;
+    vn2[j] = nv;
+    argmax_combine(bv, bp, bc, (nv == nv) ? nv : -1.0, j, j);
   }
-  __syncthreads();
 
   double r00 = 0;
   int rank = kmin;
   for (int i = 0; i < kmin; ++i) {
-    // ---- pivot: max vn1 over positions >= i, first position on ties
-    // NaN norms rank below every number, so a position is always selected
-    double bv = -2.0;
-    int bp = INT_MAX;
-    for (int j = tid; j < N; j += kQThreads) {
-      const int pj = ipos[j];
-      const double v = vn1[j];
-      if (pj >= i) argmax_combine(bv, bp, (v == v) ? v : -1.0, pj);
-    }
+    const int buf = i & 1;
+    // ---- publish the local argmax, meet, reduce over the problem's workgroups
 #pragma unroll
     for (int o = kWave / 2; o > 0; o >>= 1) {
       const double ov = __shfl_xor(bv, o, kWave);
-      const int op = __shfl_xor(bp, o, kWave);
-      argmax_combine(bv, bp, ov, op);
+      const int op = __shfl_xor(bp, o, kWave), oc = __shfl_xor(bc, o, kWave);
+      argmax_combine(bv, bp, bc, ov, op, oc);
     }
-    if ((tid & (kWave - 1)) == 0) { redv[tid / kWave] = bv; redp[tid / kWave] = bp; }
+    if ((tid & (kWave - 1)) == 0) { redv[tid / kWave] = bv; redp[tid / kWave] = bp; redc[tid / kWave] = bc; }
     __syncthreads();
     if (tid == 0) {
       double v = redv[0];
-      int p = redp[0];
-      for (int w = 1; w < kQThreads / kWave; ++w) argmax_combine(v, p, redv[w], redp[w]);
-      s_pos = p;
-      s_piv = perm[p];
+      int pp = redp[0], cc = redc[0];
+      for (int w = 1; w < kQThreads / kWave; ++w) argmax_combine(v, pp, cc, redv[w], redp[w], redc[w]);
+      d.part_v[buf * nwg + g] = v;
+      d.part_p[2 * (buf * nwg + g)] = pp;
+      d.part_p[2 * (buf * nwg + g) + 1] = cc;
+    }
+    if (nwg > 1) {
+      target += nwg;
+      group_barrier(d.sync, target, d.err);
+    } else {
+      __syncthreads();
+    }
+    if (tid == 0) {
+      double v = -3.0;
+      int pp = INT_MAX, cc = -1;
+      for (int w = 0; w < nwg; ++w)
+        argmax_combine(v, pp, cc, d.part_v[buf * nwg + w], d.part_p[2 * (buf * nwg + w)],
+                       d.part_p[2 * (buf * nwg + w) + 1]);
+      s_pos = pp;
+      s_piv = cc;
     }
     __syncthreads();
     const int piv = s_piv, ppos = s_pos;
-    // ---- Householder vector of column piv, rows i..k-1
+    // ---- Householder vector of column piv, rows i..k-1 (identical in every workgroup)
     double s = 0;
     for (int r = i + 1 + tid; r < k; r += kQThreads) { const double v = (double)Ct[piv + r * ldc]; s += v * v; }
     s = block_sum(s, sred);
@@ -102,17 +127,16 @@ __global__ __launch_bounds__(kQThreads) void k_qp3(const Qp3Desc<T>* __restrict_
     if (tid == 0) s_stop = (i >= 1 && !(fabs(beta) / r00 >= d.tol)) ? 1 : 0;
     __syncthreads();
     if (s_stop) { rank = i; break; }
-    // reflector i: in LDS when the launch's largest k fits, else read back
-    // from its column of V (written here, visible after the barrier)
+    // reflector i: LDS, or (single-workgroup problems only) its column of V
     T* vsh = use_lds ? reinterpret_cast<T*>(smem_raw) : (d.V + (int64_t)i * d.ldv);
     for (int r = tid; r < k; r += kQThreads) {
-      T v = (r < i) ? T(0) : (r == i ? T(1) : (T)((double)Ct[piv + r * ldc] * scal));
+      const T v = (r < i) ? T(0) : (r == i ? T(1) : (T)((double)Ct[piv + r * ldc] * scal));
       if (use_lds) vsh[r] = v;
-      d.V[r + (int64_t)i * d.ldv] = v;
+      if (g == 0) d.V[r + (int64_t)i * d.ldv] = v;
     }
     if (tid == 0) {
-      d.tau[i] = (T)tau;
-      // LAPACK swap of positions i and ppos, expressed on the permutation
+      if (g == 0) { d.tau[i] = (T)tau; d.beta[i] = beta; }
+      // LAPACK swap of positions i and ppos, on this workgroup's permutation copy
       const int q = perm[i];
       perm[ppos] = q;
       ipos[q] = ppos;
@@ -120,11 +144,13 @@ __global__ __launch_bounds__(kQThreads) void k_qp3(const Qp3Desc<T>* __restrict_
       ipos[piv] = i;
     }
     __syncthreads();
-    // ---- pivot column becomes R(0:i, i) = (already final), R(i,i) = beta, zeros below
-    for (int r = i + tid; r < k; r += kQThreads) Ct[piv + r * ldc] = (r == i) ? (T)beta : T(0);
-    // ---- apply H_i to the remaining columns and downdate their norms
-    for (int j = tid; j < N; j += kQThreads) {
-      if (ipos[j] <= i) continue;
+    // ---- apply H_i to the owned remaining columns, downdate, next local argmax
+    bv = -2.0;
+    bp = INT_MAX;
+    bc = -1;
+    for (int j = c0 + tid; j < c1; j += kQThreads) {
+      const int pj = ipos[j];
+      if (pj <= i) continue;
       T* __restrict__ cj = Ct + j;
       double w = (double)cj[(int64_t)i * ldc];
       for (int r = i + 1; r < k; ++r) w += (double)vsh[r] * (double)cj[r * ldc];
@@ -144,30 +170,67 @@ __global__ __launch_bounds__(kQThreads) void k_qp3(const Qp3Desc<T>* __restrict_
           if (i < k - 1) {
             double ss = 0;
             for (int r = i + 1; r < k; ++r) { const double v = (double)cj[r * ldc]; ss += v * v; }
-            vn1[j] = sqrt(ss);
-            vn2[j] = vn1[j];
+            a = sqrt(ss);
+            vn2[j] = a;
           } else {
-            vn1[j] = 0.0;
+            a = 0.0;
             vn2[j] = 0.0;
           }
         } else {
-          vn1[j] = a * sqrt(tt);
+          a = a * sqrt(tt);
         }
+        vn1[j] = a;
       }
+      argmax_combine(bv, bp, bc, (a == a) ? a : -1.0, pj, j);
     }
     __syncthreads();
   }
-  if (tid == 0) *d.rank_out = rank;
+  // every workgroup is done reading pivot columns before any is overwritten
+  if (nwg > 1) {
+    target += nwg;
+    group_barrier(d.sync, target, d.err);
+  }
+  // pivoted columns: R(i,i) = beta_i on the pivot row, zeros below
+  for (int j = c0 + tid; j < c1; j += kQThreads) {
+    const int pj = ipos[j];
+    if (pj >= rank) continue;
+    for (int r = pj; r < k; ++r) Ct[j + r * ldc] = (r == pj) ? (T)d.beta[pj] : T(0);
+  }
+  if (g == 0 && tid == 0) *d.rank_out = rank;
+}
+
+int qp3_threads() { return kQThreads; }
+
+// Workgroups of one launch that can be resident at once (cooperative bound).
+template <typename T>
+int qp3_capacity(int kmax) {
+  const size_t need = sizeof(T) * (size_t)(kmax > 0 ? kmax : 1);
+  const int use_lds = need <= (size_t)48 * 1024;
+  int per_cu = 0, dev = 0, cus = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_qp3<T>, kQThreads, use_lds ? need : 0) !=
+      hipSuccess)
+    return 0;
+  return per_cu * cus;
 }
 
 template <typename T>
-void launch_qp3(hipStream_t s, const Qp3Desc<T>* d, int nd, int kmax) {
-  if (nd <= 0) return;
+int launch_qp3(hipStream_t s, const Qp3Desc<T>* d, const int* wstart, int nd, int nwg_total, int kmax, int coop) {
+  if (nd <= 0) return 0;
   const size_t need = sizeof(T) * (size_t)(kmax > 0 ? kmax : 1);
-  const int use_lds = need <= (size_t)48 * 1024;
-  hipLaunchKernelGGL((k_qp3<T>), dim3(nd), dim3(kQThreads), use_lds ? need : 0, s, d, use_lds);
+  int use_lds = need <= (size_t)48 * 1024;
+  const size_t shm = use_lds ? need : 0;
+  if (!coop) {
+    hipLaunchKernelGGL((k_qp3<T>), dim3(nwg_total), dim3(kQThreads), shm, s, d, wstart, nd, use_lds);
+    return (int)hipGetLastError();
+  }
+  void* args[] = {(void*)&d, (void*)&wstart, (void*)&nd, (void*)&use_lds};
+  return (int)hipLaunchCooperativeKernel((const void*)k_qp3<T>, dim3(nwg_total), dim3(kQThreads), args, shm, s);
 }
-template void launch_qp3<double>(hipStream_t, const Qp3Desc<double>*, int, int);
-template void launch_qp3<float>(hipStream_t, const Qp3Desc<float>*, int, int);
+template int qp3_capacity<double>(int);
+template int qp3_capacity<float>(int);
+template int launch_qp3<double>(hipStream_t, const Qp3Desc<double>*, const int*, int, int, int, int);
+template int launch_qp3<float>(hipStream_t, const Qp3Desc<float>*, const int*, int, int, int, int);
 
 }  // namespace spq

@@ -6,6 +6,13 @@
 //              or x_c <- R^{-T} x_c ; x_n -= R12_n^T x_c  (transposed apply)
 //   k_move   : segment copies (Merge, SplitD)
 //   k_gather_in / k_scatter_out : permuted entry/exit of the apply
+//
+// A launch mixes problem sizes: each descriptor gets `nwg` workgroups
+// (wstart = exclusive prefix sums).  Small problems use one workgroup with the
+// vector in LDS; large ones are split into row slices that meet at one
+// inter-workgroup barrier per 32-column block.  The host places the
+// multi-workgroup descriptors first and keeps their total within one
+// workgroup per CU, so they are resident together.
 #include "kcommon.hpp"
 #include "descs.hpp"
 #include "launch.hpp"
@@ -13,126 +20,311 @@
 namespace spq {
 
 constexpr int kSThreads = 256;
-constexpr int kRNB = 32;  // reflector block width (matches the panel kernel)
+constexpr int kSWaves = kSThreads / kWave;
+constexpr int kRNB = 32;      // reflector / triangular block width (matches the panel kernel)
+constexpr int kTLd = kRNB + 1;  // padded LDS leading dimension of a staged 32x32 block
 
-template <typename T>
-__global__ __launch_bounds__(kSThreads) void k_refl(const ReflDesc<T>* __restrict__ descs, const Seg* __restrict__ segs,
-                                                    T* __restrict__ xbuf, T* __restrict__ work, int lds_cap,
-                                                    int mmax) {
-  extern __shared__ unsigned char smem_raw[];
-  __shared__ T red[kRNB * (kSThreads / kWave)];
-  __shared__ T zsh[kRNB];
-  const ReflDesc<T> d = descs[blockIdx.x];
-  const int m = d.m, tid = threadIdx.x;
-  // LDS when every problem of the launch fits, else a per-workgroup slice of
-  // the global work buffer (nd * mmax elements).
-  T* x = (mmax <= lds_cap) ? reinterpret_cast<T*>(smem_raw) : (work + (int64_t)blockIdx.x * mmax);
-  // gather
-  {
-    int base = 0;
-    for (int s = 0; s < d.nseg; ++s) {
-      const Seg g = segs[d.seg0 + s];
-      for (int i = tid; i < g.len; i += kSThreads) x[base + i] = xbuf[g.off + i];
-      base += g.len;
-    }
-  }
-  __syncthreads();
-  const int nblk = (d.k + kRNB - 1) / kRNB;
-  for (int t = 0; t < nblk; ++t) {
-    const int bi = d.trans ? t : nblk - 1 - t;
-    const int jb = bi * kRNB, w = min(kRNB, d.k - jb);
-    T acc[kRNB];
+// Butterfly reduce-scatter of 32 per-lane partial sums over a wavefront: each
+// step exchanges half of the live values with lane ^ H, so 31 shuffles leave
+// lane l with the sum over its 32-lane half of a[l & 31]; one more folds the
+// two halves.  All indices are compile-time, so a[] stays in registers.
+template <typename T, int H>
+__device__ __forceinline__ void rs_step(T (&a)[kRNB], int lane) {
+  const bool up = (lane & H) != 0;
 #pragma unroll
-    for (int c = 0; c < kRNB; ++c) acc[c] = T(0);
-    for (int i = jb + tid; i < m; i += kSThreads) {
+  for (int j = 0; j < H; ++j) {
+    const T keep = up ? a[j + H] : a[j];
+    const T send = up ? a[j] : a[j + H];
+    a[j] = keep + __shfl_xor(send, H, kWave);
+  }
+}
+template <typename T>
+__device__ __forceinline__ T wave_reduce_scatter32(T (&a)[kRNB]) {
+  const int lane = threadIdx.x & (kWave - 1);
+  rs_step<T, 16>(a, lane);
+  rs_step<T, 8>(a, lane);
+  rs_step<T, 4>(a, lane);
+  rs_step<T, 2>(a, lane);
+  rs_step<T, 1>(a, lane);
+  return a[0] + __shfl_xor(a[0], 32, kWave);
+}
+
+// Entry (i, col) of a unit-lower reflector block; the load is in bounds for
+// every i < m, col < k (the strict upper part holds R or is unused).
+template <typename T>
+__device__ __forceinline__ T vval(const T* __restrict__ V, int64_t ldv, int i, int col) {
+  const T v = V[i + (int64_t)col * ldv];
+  return (i > col) ? v : (i == col ? T(1) : T(0));
+}
+
+// acc[c] += V(i, jb+c) * x[i] over rows i in [lo, hi)
+template <typename T>
+__device__ __forceinline__ void refl_partial(const T* __restrict__ V, int64_t ldv, const T* __restrict__ x, int jb, int w,
+                                             int lo, int hi, T (&acc)[kRNB]) {
+  if (w == kRNB) {
+    for (int i = lo + threadIdx.x; i < hi; i += kSThreads) {
       const T xi = x[i];
 #pragma unroll
-      for (int c = 0; c < kRNB; ++c) {
-        if (c < w) {
-          const int col = jb + c;
-          if (i > col) acc[c] += d.V[i + (int64_t)col * d.ldv] * xi;
-          else if (i == col) acc[c] += xi;
-        }
-      }
+      for (int c = 0; c < kRNB; ++c) acc[c] += vval(V, ldv, i, jb + c) * xi;
     }
-    block_sum_n<T, kRNB>(acc, red, w);
-    // z = T^T y (trans) or T y ; T block upper w x w, ld kRNB
-    const T* __restrict__ Tb = d.Tb + (int64_t)jb * kRNB;
-    if (tid < w) {
-      T z = 0;
-      if (d.trans) { for (int l = 0; l <= tid; ++l) z += Tb[l + tid * kRNB] * acc[l]; }
-      else { for (int l = tid; l < w; ++l) z += Tb[tid + l * kRNB] * acc[l]; }
-      zsh[tid] = z;
+  } else {
+    for (int i = lo + threadIdx.x; i < hi; i += kSThreads) {
+      const T xi = x[i];
+#pragma unroll
+      for (int c = 0; c < kRNB; ++c)
+        if (c < w) acc[c] += vval(V, ldv, i, jb + c) * xi;
     }
-    __syncthreads();
-    for (int i = jb + tid; i < m; i += kSThreads) {
+  }
+}
+// x[i] -= sum_c V(i, jb+c) z[c] over rows i in [lo, hi)
+template <typename T>
+__device__ __forceinline__ void refl_update(const T* __restrict__ V, int64_t ldv, T* __restrict__ x, int jb, int w, int lo,
+                                            int hi, const T* zsh) {
+  if (w == kRNB) {
+    for (int i = lo + threadIdx.x; i < hi; i += kSThreads) {
       T s = 0;
 #pragma unroll
-      for (int c = 0; c < kRNB; ++c) {
-        if (c < w) {
-          const int col = jb + c;
-          if (i > col) s += d.V[i + (int64_t)col * d.ldv] * zsh[c];
-          else if (i == col) s += zsh[c];
-        }
-      }
+      for (int c = 0; c < kRNB; ++c) s += vval(V, ldv, i, jb + c) * zsh[c];
       x[i] -= s;
     }
-    __syncthreads();
-  }
-  // scatter
-  {
-    int base = 0;
-    for (int s = 0; s < d.nseg; ++s) {
-      const Seg g = segs[d.seg0 + s];
-      for (int i = tid; i < g.len; i += kSThreads) xbuf[g.off + i] = x[base + i];
-      base += g.len;
+  } else {
+    for (int i = lo + threadIdx.x; i < hi; i += kSThreads) {
+      T s = 0;
+#pragma unroll
+      for (int c = 0; c < kRNB; ++c)
+        if (c < w) s += vval(V, ldv, i, jb + c) * zsh[c];
+      x[i] -= s;
     }
+  }
+}
+// Stage the block T factor of columns [jb, jb+32) (ld 32) into LDS (ld kTLd).
+template <typename T>
+__device__ __forceinline__ void stage_t(const T* __restrict__ Tb, int jb, T* tsh) {
+  for (int e = threadIdx.x; e < kRNB * kRNB; e += kSThreads) tsh[(e & (kRNB - 1)) + (e / kRNB) * kTLd] = Tb[(int64_t)jb * kRNB + e];
+}
+// z = T_b^T y (trans) or T_b y, threads c < w
+template <typename T>
+__device__ __forceinline__ void tmul(const T* tsh, int w, int trans, const T* y, T* zsh) {
+  const int tid = threadIdx.x;
+  if (tid < w) {
+    T z = 0;
+    if (trans) { for (int l = 0; l <= tid; ++l) z += tsh[l + tid * kTLd] * y[l]; }
+    else { for (int l = tid; l < w; ++l) z += tsh[tid + l * kTLd] * y[l]; }
+    zsh[tid] = z;
   }
 }
 
 template <typename T>
-__global__ __launch_bounds__(kSThreads) void k_bwd(const BwdDesc<T>* __restrict__ descs, const R12Desc<T>* __restrict__ r12,
-                                                   T* __restrict__ xbuf) {
-  __shared__ T xk;
-  const BwdDesc<T> d = descs[blockIdx.x];
-  const int k = d.k, tid = threadIdx.x;
-  T* __restrict__ xs = xbuf + d.xoff;
-  if (!d.trans) {
-    // x_c[0:k] -= sum_n A_n[0:k, 0:s] x_n[0:s]
-    for (int r = tid; r < k; r += kSThreads) {
-      T acc = 0;
-      for (int b = 0; b < d.nblk; ++b) {
-        const R12Desc<T> q = r12[d.blk0 + b];
-        const T* __restrict__ xn = xbuf + q.xoff;
-        for (int j = 0; j < q.s; ++j) acc += q.A[r + (int64_t)j * q.ld] * xn[j];
+__global__ __launch_bounds__(kSThreads) void k_refl(const ReflDesc<T>* __restrict__ descs, const int* __restrict__ wstart,
+                                                    int nd, const Seg* __restrict__ segs, T* __restrict__ xbuf,
+                                                    T* __restrict__ work, int lds_cap, int mmax) {
+  extern __shared__ unsigned char smem_raw[];
+  __shared__ T red[kRNB * kSWaves];
+  __shared__ T tsh[kRNB * kTLd];
+  __shared__ T ysh[kRNB];
+  __shared__ T zsh[kRNB];
+  const int p = find_owner(wstart, nd, blockIdx.x);
+  const ReflDesc<T> d = descs[p];
+  const int g = blockIdx.x - wstart[p];
+  const int m = d.m, tid = threadIdx.x, lane = tid & (kWave - 1), wv = tid / kWave;
+  const int nblk = (d.k + kRNB - 1) / kRNB;
+  const bool multi = d.nwg > 1;
+  // rows [r0, r1) of the gathered vector belong to this workgroup
+  const int r0 = multi ? (int)((int64_t)g * m / d.nwg) : 0;
+  const int r1 = multi ? (int)((int64_t)(g + 1) * m / d.nwg) : m;
+  T* __restrict__ x = multi ? d.work
+                            : ((mmax <= lds_cap) ? reinterpret_cast<T*>(smem_raw) : (work + (int64_t)blockIdx.x * mmax));
+  {
+    int base = 0;
+    for (int s = 0; s < d.nseg; ++s) {
+      const Seg sg = segs[d.seg0 + s];
+      const int lo = max(base, r0), hi = min(base + sg.len, r1);
+      for (int i = lo + tid; i < hi; i += kSThreads) x[i] = xbuf[sg.off + (i - base)];
+      base += sg.len;
+    }
+  }
+  __syncthreads();
+  unsigned int target = 0;
+  for (int t = 0; t < nblk; ++t) {
+    const int bi = d.trans ? t : nblk - 1 - t;
+    const int jb = bi * kRNB, w = min(kRNB, d.k - jb);
+    stage_t(d.Tb, jb, tsh);
+    T acc[kRNB];
+#pragma unroll
+    for (int c = 0; c < kRNB; ++c) acc[c] = T(0);
+    refl_partial(d.V, d.ldv, x, jb, w, max(r0, jb), r1, acc);
+    const T sw = wave_reduce_scatter32(acc);
+    if (lane < kRNB) red[wv * kRNB + lane] = sw;
+    __syncthreads();
+    if (tid < kRNB) {
+      T y = 0;
+#pragma unroll
+      for (int q = 0; q < kSWaves; ++q) y += red[q * kRNB + tid];
+      if (multi) d.part[((int64_t)(t & 1) * d.nwg + g) * kRNB + tid] = y;
+      else ysh[tid] = y;
+    }
+    if (multi) {
+      // partial sums are written by wave 0, the wave that releases the barrier
+      target += d.nwg;
+      group_barrier(d.sync, target, d.err);
+      if (tid < kRNB) {
+        T y = 0;
+        for (int q = 0; q < d.nwg; ++q) y += d.part[((int64_t)(t & 1) * d.nwg + q) * kRNB + tid];
+        ysh[tid] = y;
       }
-      xs[r] -= acc;
     }
     __syncthreads();
-    // back substitution with R (upper, ld)
-    for (int i = k - 1; i >= 0; --i) {
-      if (tid == 0) { xs[i] = xs[i] / d.R[i + (int64_t)i * d.ldr]; xk = xs[i]; }
+    tmul(tsh, w, d.trans, ysh, zsh);
+    __syncthreads();
+    refl_update(d.V, d.ldv, x, jb, w, max(r0, jb), r1, zsh);
+    __syncthreads();
+  }
+  {
+    int base = 0;
+    for (int s = 0; s < d.nseg; ++s) {
+      const Seg sg = segs[d.seg0 + s];
+      const int lo = max(base, r0), hi = min(base + sg.len, r1);
+      for (int i = lo + tid; i < hi; i += kSThreads) xbuf[sg.off + (i - base)] = x[i];
+      base += sg.len;
+    }
+  }
+}
+
+// Solve the w x w diagonal block (staged in LDS, ld kTLd) in place on
+// xs[jb:jb+w] with the first wavefront (lane l holds x_{jb+l});
+// trans: R^T (forward substitution).
+template <typename T>
+__device__ __forceinline__ void diag_solve(const T* rsh, T* __restrict__ xs, int jb, int w, int trans) {
+  if (threadIdx.x >= kWave) return;
+  const int l = threadIdx.x;
+  T xl = (l < w) ? xs[jb + l] : T(0);
+  if (!trans) {
+    for (int i = w - 1; i >= 0; --i) {
+      T xi = __shfl(xl, i, kWave);
+      xi = xi / rsh[i + i * kTLd];
+      if (l == i) xl = xi;
+      if (l < i) xl -= rsh[l + i * kTLd] * xi;
+    }
+  } else {
+    for (int i = 0; i < w; ++i) {
+      T xi = __shfl(xl, i, kWave);
+      xi = xi / rsh[i + i * kTLd];
+      if (l == i) xl = xi;
+      if (l > i && l < w) xl -= rsh[i + l * kTLd] * xi;
+    }
+  }
+  if (l < w) xs[jb + l] = xl;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kSThreads) void k_bwd(const BwdDesc<T>* __restrict__ descs, const int* __restrict__ wstart,
+                                                   int nd, const R12Desc<T>* __restrict__ r12, T* __restrict__ xbuf) {
+  __shared__ T rsh[kRNB * kTLd];
+  __shared__ T xsh[kRNB];
+  __shared__ T red[kSThreads];
+  const int p = find_owner(wstart, nd, blockIdx.x);
+  const BwdDesc<T> d = descs[p];
+  const int g = blockIdx.x - wstart[p], nwg = d.nwg;
+  const int k = d.k, tid = threadIdx.x, lane = tid & (kWave - 1), wv = tid / kWave;
+  const int nblk = (k + kRNB - 1) / kRNB;
+  const int64_t ldr = d.ldr;
+  T* __restrict__ xs = xbuf + d.xoff;
+  // row slices in whole 32-row blocks: blocks [b0, b1) belong to this workgroup
+  const int b0 = (int)((int64_t)g * nblk / nwg), b1 = (int)((int64_t)(g + 1) * nblk / nwg);
+  const int r0 = b0 * kRNB, r1 = min(k, b1 * kRNB);
+  unsigned int target = 0;
+  auto meet = [&]() {
+    if (nwg > 1) {
+      target += nwg;
+      group_barrier(d.sync, target, d.err);
+    } else {
       __syncthreads();
-      const T v = xk;
-      for (int r = tid; r < i; r += kSThreads) xs[r] -= d.R[r + (int64_t)i * d.ldr] * v;
+    }
+  };
+  auto stage_diag = [&](int jb, int w) {
+    for (int e = tid; e < kRNB * kRNB; e += kSThreads) {
+      const int r = e & (kRNB - 1), c = e / kRNB;
+      if (r < w && c < w) rsh[r + c * kTLd] = d.R[(jb + r) + (int64_t)(jb + c) * ldr];
+    }
+    __syncthreads();
+  };
+  if (!d.trans) {
+    // x_c[r] -= sum_n A_n[r, 0:s] x_n[0:s] for the owned rows: lanes take
+    // rows, waves take interleaved columns, LDS sums the waves
+    for (int rb = r0; rb < r1; rb += kWave) {
+      const int r = rb + lane;
+      T acc = 0;
+      if (r < r1) {
+        for (int b = 0; b < d.nblk; ++b) {
+          const R12Desc<T> q = r12[d.blk0 + b];
+          const T* __restrict__ xn = xbuf + q.xoff;
+          for (int j = wv; j < q.s; j += kSWaves) acc += q.A[r + (int64_t)j * q.ld] * xn[j];
+        }
+      }
+      red[tid] = acc;
+      __syncthreads();
+      if (wv == 0 && r < r1) {
+        T s = 0;
+#pragma unroll
+        for (int q = 0; q < kSWaves; ++q) s += red[q * kWave + lane];
+        xs[r] -= s;
+      }
+      __syncthreads();
+    }
+    // blocked back substitution, last block first
+    for (int bi = nblk - 1; bi >= 0; --bi) {
+      const int jb = bi * kRNB, w = min(kRNB, k - jb);
+      if (bi >= b0 && bi < b1) {
+        stage_diag(jb, w);
+        diag_solve(rsh, xs, jb, w, 0);
+      }
+      meet();
+      if (tid < w) xsh[tid] = xs[jb + tid];
+      __syncthreads();
+      for (int r = r0 + tid; r < min(r1, jb); r += kSThreads) {
+        T s = 0;
+        for (int c = 0; c < w; ++c) s += d.R[r + (int64_t)(jb + c) * ldr] * xsh[c];
+        xs[r] -= s;
+      }
       __syncthreads();
     }
   } else {
-    // forward substitution with R^T: x_i = (x_i - sum_{r<i} R(r,i) x_r) / R(i,i)
-    for (int i = 0; i < k; ++i) {
-      if (tid == 0) { xs[i] = xs[i] / d.R[i + (int64_t)i * d.ldr]; xk = xs[i]; }
+    // blocked forward substitution with R^T, first block first
+    const int hl = tid & (kRNB - 1), hw = tid / kRNB;  // half-wave lane / index
+    for (int bi = 0; bi < nblk; ++bi) {
+      const int jb = bi * kRNB, w = min(kRNB, k - jb);
+      if (bi >= b0 && bi < b1) {
+        stage_diag(jb, w);
+        diag_solve(rsh, xs, jb, w, 1);
+      }
+      meet();
+      if (tid < w) xsh[tid] = xs[jb + tid];
       __syncthreads();
-      const T v = xk;
-      for (int r = i + 1 + tid; r < k; r += kSThreads) xs[r] -= d.R[i + (int64_t)r * d.ldr] * v;
+      // x[r] -= sum_c R[jb+c, r] x[jb+c]: a half-wave reads the contiguous
+      // 32-entry column segment of R for one row r
+      for (int rb = max(r0, jb + w); rb < r1; rb += kSThreads / kRNB) {
+        const int r = rb + hw;
+        T v = (r < r1 && hl < w) ? d.R[(jb + hl) + (int64_t)r * ldr] * xsh[hl] : T(0);
+#pragma unroll
+        for (int o = kRNB / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+        if (hl == 0 && r < r1) xs[r] -= v;
+      }
       __syncthreads();
     }
-    // x_n[0:s] -= A_n[0:k, 0:s]^T x_c[0:k]   (atomic: several clusters share x_n)
+    if (nwg > 1) {
+      target += nwg;
+      group_barrier(d.sync, target, d.err);
+    }
+    // x_n[j] -= A_n[0:k, j]^T x_c[0:k]: one wavefront per column, columns
+    // interleaved over all waves of the problem (atomic: several clusters of
+    // one stage share x_n)
     for (int b = 0; b < d.nblk; ++b) {
       const R12Desc<T> q = r12[d.blk0 + b];
-      for (int j = tid; j < q.s; j += kSThreads) {
+      for (int j = g * kSWaves + wv; j < q.s; j += nwg * kSWaves) {
         T acc = 0;
-        for (int r = 0; r < k; ++r) acc += q.A[r + (int64_t)j * q.ld] * xs[r];
-        atomicAdd(xbuf + q.xoff + j, -acc);
+        for (int r = lane; r < k; r += kWave) acc += q.A[r + (int64_t)j * q.ld] * xs[r];
+        acc = wave_sum(acc);
+        if (lane == 0) atomicAdd(xbuf + q.xoff + j, -acc);
       }
     }
   }
@@ -163,16 +355,18 @@ __global__ void k_scatter_out(const int64_t* __restrict__ src, const int* __rest
 int refl_lds_cap(int elem) { return (48 * 1024) / elem; }
 
 template <typename T>
-void launch_refl(hipStream_t s, const ReflDesc<T>* d, int nd, const Seg* segs, T* xbuf, T* work, int mmax) {
+void launch_refl(hipStream_t s, const ReflDesc<T>* d, const int* wstart, int nd, int nwg_total, const Seg* segs,
+                 T* xbuf, T* work, int mmax) {
   if (nd <= 0) return;
   const int cap = refl_lds_cap(sizeof(T));
   size_t shm = (mmax <= cap) ? sizeof(T) * (size_t)mmax : 0;
-  hipLaunchKernelGGL((k_refl<T>), dim3(nd), dim3(kSThreads), shm, s, d, segs, xbuf, work, cap, mmax);
+  hipLaunchKernelGGL((k_refl<T>), dim3(nwg_total), dim3(kSThreads), shm, s, d, wstart, nd, segs, xbuf, work, cap, mmax);
 }
 template <typename T>
-void launch_bwd(hipStream_t s, const BwdDesc<T>* d, int nd, const R12Desc<T>* r12, T* xbuf) {
+void launch_bwd(hipStream_t s, const BwdDesc<T>* d, const int* wstart, int nd, int nwg_total, const R12Desc<T>* r12,
+                T* xbuf) {
   if (nd <= 0) return;
-  hipLaunchKernelGGL((k_bwd<T>), dim3(nd), dim3(kSThreads), 0, s, d, r12, xbuf);
+  hipLaunchKernelGGL((k_bwd<T>), dim3(nwg_total), dim3(kSThreads), 0, s, d, wstart, nd, r12, xbuf);
 }
 template <typename T>
 void launch_move(hipStream_t s, const MoveDesc* d, int nd, T* dst, const T* src) {
@@ -190,11 +384,11 @@ void launch_scatter_out(hipStream_t s, const int64_t* src, const int* dst, int n
   hipLaunchKernelGGL((k_scatter_out<T>), dim3((n + 255) / 256), dim3(256), 0, s, src, dst, n, xbuf, x);
 }
 
-#define SPQ_INST(T)                                                                                   \
-  template void launch_refl<T>(hipStream_t, const ReflDesc<T>*, int, const Seg*, T*, T*, int);        \
-  template void launch_bwd<T>(hipStream_t, const BwdDesc<T>*, int, const R12Desc<T>*, T*);            \
-  template void launch_move<T>(hipStream_t, const MoveDesc*, int, T*, const T*);                      \
-  template void launch_gather_in<T>(hipStream_t, const int64_t*, const int*, int, const double*, T*); \
+#define SPQ_INST(T)                                                                                            \
+  template void launch_refl<T>(hipStream_t, const ReflDesc<T>*, const int*, int, int, const Seg*, T*, T*, int); \
+  template void launch_bwd<T>(hipStream_t, const BwdDesc<T>*, const int*, int, int, const R12Desc<T>*, T*);     \
+  template void launch_move<T>(hipStream_t, const MoveDesc*, int, T*, const T*);                               \
+  template void launch_gather_in<T>(hipStream_t, const int64_t*, const int*, int, const double*, T*);          \
   template void launch_scatter_out<T>(hipStream_t, const int64_t*, const int*, int, const T*, double*);
 SPQ_INST(double)
 SPQ_INST(float)

@@ -111,22 +111,37 @@ int ktest_larft(int m, int r, const double* V, const double* tau, double* Tb) {
   return 0;
 }
 
-// Truncated QP3 of C (k x N): Ct is N x k (ld N) in/out; V k x k, tau k; returns rank.
-int ktest_qp3(int k, int N, double* Ct, double* V, double* tau, double tol) {
+// Truncated QP3 of C (k x N): Ct is N x k (ld N) in/out; V k x k, tau k; the
+// problem is split over nwg workgroups (cooperative launch when nwg > 1).
+// Returns the rank, or -1 on a launch/run error.
+int ktest_qp3(int k, int N, double* Ct, double* V, double* tau, double tol, int nwg) {
   double *dC = dup(Ct, (size_t)N * k), *dV = dup<double>(nullptr, (size_t)k * k), *dt = dup<double>(nullptr, k);
   double* vn = dup<double>(nullptr, 2 * (size_t)N);
-  int* pos = dup<int>(nullptr, 2 * (size_t)N);
+  int* pos = dup<int>(nullptr, 2 * (size_t)N * nwg);
+  double* pv = dup<double>(nullptr, 2 * (size_t)nwg);
+  int* pp = dup<int>(nullptr, 4 * (size_t)nwg);
+  double* beta = dup<double>(nullptr, k);
+  unsigned int* sync = dup<unsigned int>(nullptr, 1);
+  int* err = dup<int>(nullptr, 1);
   int* rk = dup<int>(nullptr, 1);
-  Qp3Desc<double> d{dC, dV, dt, vn, pos, rk, N, k, N, k, tol};
+  Qp3Desc<double> d{dC, dV, dt, vn, pos, pv, pp, beta, sync, err, rk, N, k, N, k, tol, nwg, 0};
   Qp3Desc<double>* dd = dup(&d, 1);
-  launch_qp3<double>(0, dd, 1, k);
+  int zero = 0;
+  int* ws = dup(&zero, 1);
+  if (nwg > qp3_capacity<double>(k)) { g_err = "nwg exceeds resident capacity"; return -1; }
+  int rc = launch_qp3<double>(0, dd, ws, 1, nwg, k, nwg > 1 ? 1 : 0);
+  if (rc != 0) { g_err = hipGetErrorString((hipError_t)rc); return -1; }
   KT(hipDeviceSynchronize());
-  int rank = 0;
+  int rank = 0, e = 0;
   back(&rank, rk, 1);
+  back(&e, err, 1);
   back(Ct, dC, (size_t)N * k);
   back(V, dV, (size_t)k * k);
   back(tau, dt, k);
-  for (void* p : {(void*)dC, (void*)dV, (void*)dt, (void*)vn, (void*)pos, (void*)rk, (void*)dd}) (void)hipFree(p);
+  for (void* q : {(void*)dC, (void*)dV, (void*)dt, (void*)vn, (void*)pos, (void*)pv, (void*)pp, (void*)beta,
+                  (void*)sync, (void*)err, (void*)rk, (void*)dd, (void*)ws})
+    (void)hipFree(q);
+  if (e) { g_err = "barrier wait timed out"; return -1; }
   return rank;
 }
 
@@ -145,15 +160,27 @@ int ktest_refl(int m, int k, const double* V, const double* Tb, double* x, int t
   }
   double* dx = dup(xb.data(), xlen);
   Seg* dsg = dup(segs.data(), segs.size());
-  ReflDesc<double> d{dV, dT, m, m, k, nb, 0, nseg, trans, 0};
+  const int nwg = m >= 2048 ? 4 : 1;
+  double* wk = dup<double>(nullptr, m);
+  double* part = dup<double>(nullptr, 2 * 32 * (size_t)nwg);
+  unsigned int* sync = dup<unsigned int>(nullptr, 1);
+  int* err = dup<int>(nullptr, 1);
+  ReflDesc<double> d{dV, dT, m, m, k, nb, 0, nseg, trans, nwg, wk, part, sync, err};
   ReflDesc<double>* dd = dup(&d, 1);
   double* work = dup<double>(nullptr, m);
-  launch_refl<double>(0, dd, 1, dsg, dx, work, m);
+  int zero = 0;
+  int* ws = dup(&zero, 1);
+  launch_refl<double>(0, dd, ws, 1, nwg, dsg, dx, work, nwg > 1 ? 0 : m);
   KT(hipDeviceSynchronize());
   back(xb.data(), dx, xlen);
   for (int s = 0; s < nseg; ++s)
     std::memcpy(x + (size_t)s * seglen, &xb[(size_t)s * (seglen + gap)], sizeof(double) * seglen);
-  for (void* p : {(void*)dV, (void*)dT, (void*)dx, (void*)dsg, (void*)dd, (void*)work}) (void)hipFree(p);
+  int e = 0;
+  back(&e, err, 1);
+  for (void* p : {(void*)dV, (void*)dT, (void*)dx, (void*)dsg, (void*)dd, (void*)work, (void*)wk, (void*)part,
+                  (void*)sync, (void*)err, (void*)ws})
+    (void)hipFree(p);
+  if (e) { g_err = "barrier wait timed out"; return -1; }
   return 0;
 }
 
@@ -166,14 +193,23 @@ int ktest_bwd(int k, const double* R, int s, const double* A1, double* xs, doubl
   double* dx = dup(xb.data(), xb.size());
   R12Desc<double> q{dA, k, s, (int64_t)k};
   R12Desc<double>* dq = dup(&q, 1);
-  BwdDesc<double> d{dR, k, k, 0, 0, s > 0 ? 1 : 0, trans, 0};
+  const int nwg = k >= 512 ? 4 : 1;
+  unsigned int* sync = dup<unsigned int>(nullptr, 1);
+  int* err = dup<int>(nullptr, 1);
+  BwdDesc<double> d{dR, k, k, 0, 0, s > 0 ? 1 : 0, trans, nwg, sync, err};
   BwdDesc<double>* dd = dup(&d, 1);
-  launch_bwd<double>(0, dd, 1, dq, dx);
+  int zero = 0;
+  int* ws = dup(&zero, 1);
+  launch_bwd<double>(0, dd, ws, 1, nwg, dq, dx);
   KT(hipDeviceSynchronize());
   back(xb.data(), dx, xb.size());
   std::memcpy(xs, xb.data(), sizeof(double) * k);
   if (s > 0) std::memcpy(x1, xb.data() + k, sizeof(double) * s);
-  for (void* p : {(void*)dR, (void*)dA, (void*)dx, (void*)dq, (void*)dd}) (void)hipFree(p);
+  int e = 0;
+  back(&e, err, 1);
+  for (void* p : {(void*)dR, (void*)dA, (void*)dx, (void*)dq, (void*)dd, (void*)sync, (void*)err, (void*)ws})
+    (void)hipFree(p);
+  if (e) { g_err = "barrier wait timed out"; return -1; }
   return 0;
 }
 

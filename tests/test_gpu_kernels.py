@@ -30,7 +30,7 @@ def K(ktest):
                                  C.c_int, C.c_double, C.c_double]
     ktest.ktest_qr.argtypes = [C.c_int, C.c_int, dp, dp, dp, C.c_int, dp]
     ktest.ktest_larft.argtypes = [C.c_int, C.c_int, dp, dp, dp]
-    ktest.ktest_qp3.argtypes = [C.c_int, C.c_int, dp, dp, dp, C.c_double]
+    ktest.ktest_qp3.argtypes = [C.c_int, C.c_int, dp, dp, dp, C.c_double, C.c_int]
     ktest.ktest_refl.argtypes = [C.c_int, C.c_int, dp, dp, dp, C.c_int, C.c_int, C.c_int]
     ktest.ktest_bwd.argtypes = [C.c_int, dp, C.c_int, dp, dp, dp, C.c_int]
     ktest.ktest_trsm.argtypes = [C.c_int, C.c_int, dp, dp]
@@ -150,9 +150,10 @@ def test_larft(K, oracle, mr):
         assert rel(np.triu(Tb[:w, blk]), Tref[:w, blk]) < 1e-12
 
 
+@pytest.mark.parametrize("nwg", [1, 3])
 @pytest.mark.parametrize("kN,true_rank,tol", [((20, 100), 6, 1e-3), ((64, 300), 20, 1e-6), ((50, 40), 50, 1e-10),
-                                              ((30, 30), 10, 1e-2), ((1, 5), 1, 1e-3)])
-def test_qp3(K, oracle, kN, true_rank, tol):
+                                              ((30, 30), 10, 1e-2), ((1, 5), 1, 1e-3), ((120, 2000), 90, 1e-8)])
+def test_qp3(K, oracle, kN, true_rank, tol, nwg):
     k, N = kN
     rng = np.random.default_rng(k * N)
     r = min(true_rank, k, N)
@@ -167,7 +168,7 @@ def test_qp3(K, oracle, kN, true_rank, tol):
     Ct = np.asfortranarray(Cmat.T.copy())
     V = np.zeros((k, k), order="F")
     tau = np.zeros(k)
-    rank = K.ktest_qp3(k, N, d(Ct), d(V), d(tau), tol)
+    rank = K.ktest_qp3(k, N, d(Ct), d(V), d(tau), tol, min(nwg, N))
     assert rank == rank_ref
     Crank = Ct[:, :rank].T  # rank x N, original column order
     # reference Crank = triu(R[:rank]) P^T
@@ -183,13 +184,16 @@ def test_qp3(K, oracle, kN, true_rank, tol):
     well = dd / dd[0] > 1e-6
     assert np.allclose(tau[:rank][well], tau_ref[:rank][well], rtol=1e-9, atol=1e-14)
     Vref = np.tril(Rref[:, :rank], -1) + np.eye(k, rank)
-    assert rel(np.tril(V[:, :rank], -1) + np.eye(k, rank), Vref) < 1e-10
+    assert rel((np.tril(V[:, :rank], -1) + np.eye(k, rank))[:, well], Vref[:, well]) < 1e-9
 
 
 @pytest.mark.parametrize("trans", [1, 0])
-@pytest.mark.parametrize("mk,nseg", [((40, 40), 1), ((150, 70), 3), ((600, 100), 4), ((20000, 40), 2)])
+# m >= 2048 runs the row-sliced multi-workgroup form (4 workgroups)
+@pytest.mark.parametrize("mk,nseg", [((40, 40), 1), ((150, 70), 3), ((600, 100), 4), ((20000, 40), 2),
+                                     ((5100, 300), 3)])
 def test_refl(K, oracle, mk, nseg, trans):
     m, k = mk
+    assert m % nseg == 0  # the test splits x into nseg equal segments
     rng = np.random.default_rng(m + k)
     oracle.set_backend("own")
     QR, tau = oracle.geqrf(rng.standard_normal((m, k)))
@@ -204,11 +208,12 @@ def test_refl(K, oracle, mk, nseg, trans):
 
 
 @pytest.mark.parametrize("trans", [0, 1])
-@pytest.mark.parametrize("ks", [(1, 0), (50, 30), (300, 10)])
+# k >= 512 runs the multi-workgroup form (4 workgroups over whole 32-row blocks)
+@pytest.mark.parametrize("ks", [(1, 0), (50, 30), (300, 10), (700, 40), (1000, 0)])
 def test_bwd(K, ks, trans):
     k, s = ks
     rng = np.random.default_rng(k)
-    R = np.asfortranarray(np.triu(rng.standard_normal((k, k))) + 5 * np.eye(k))
+    R = np.asfortranarray(np.triu(rng.standard_normal((k, k))) / np.sqrt(k) + 4 * np.eye(k))
     A1 = np.asfortranarray(rng.standard_normal((k, max(s, 1))))
     xs = rng.standard_normal(k)
     x1 = rng.standard_normal(max(s, 1))
