@@ -35,6 +35,7 @@ class Engine final : public EngineBase {
     qp3_err_ = (int*)perm_.alloc(256);
     SPQ_HIP(hipMemset(qp3_err_, 0, 256));
     if (const char* e = std::getenv("SPAQR_QP3_MAX_WG")) qp3_max_wg_ = std::max(1, std::atoi(e));
+    if (const char* e = std::getenv("SPAQR_QP3_REGISTER")) qp3_reg_ = std::atoi(e) != 0;
     SPQ_HIP(hipDeviceGetAttribute(&num_cus_, hipDeviceAttributeMultiprocessorCount, device));
     if (const char* e = std::getenv("SPAQR_SOLVE_STAGES")) stage_prof_ = std::atoi(e) != 0;
   }
@@ -423,6 +424,15 @@ class Engine final : public EngineBase {
   double* xout_ = nullptr;
   void run_program(const double* b, double* x, bool prof = false);
   bool stage_prof_ = false;  // SPAQR_SOLVE_STAGES=1: time every solve stage (uncaptured)
+  bool qp3_reg_ = true;      // SPAQR_QP3_REGISTER=0: streaming truncated QR only
+  int qp3r_cap_[2] = {-1, -1};
+  // resident workgroups of the register-form QR with nt threads per
+  // workgroup (largest LDS footprint, G = 64)
+  int qp3r_cap(int nt) {
+    int& c = qp3r_cap_[nt == 1024 ? 1 : 0];
+    if (c < 0) c = qp3r_capacity<T>(64, nt);
+    return c;
+  }
 };
 
 // ======================================================================
@@ -623,21 +633,19 @@ void Engine<T>::eliminate_level(const std::vector<int>& cs) {
   }
   init_blocks(zero, 0);
   // waves: a cluster waits for every earlier cluster sharing a target block
+  // (wave = 1 + the largest wave of an earlier cluster touching one of its
+  // target blocks, tracked per block)
   std::vector<int> wave(plans.size(), 0);
   int nw = plans.empty() ? 0 : 1;
-  for (size_t i = 0; i < plans.size(); ++i) {
-    for (size_t j = 0; j < i; ++j) {
-      const auto& a = plans[i].touched;
-      const auto& b = plans[j].touched;
-      size_t x = 0, y = 0;
-      bool hit = false;
-      while (x < a.size() && y < b.size()) {
-        if (a[x] == b[y]) { hit = true; break; }
-        if (a[x] < b[y]) ++x; else ++y;
-      }
-      if (hit) wave[i] = std::max(wave[i], wave[j] + 1);
+  {
+    std::vector<int> maxw(ed_.size(), -1);
+    for (size_t i = 0; i < plans.size(); ++i) {
+      int w = 0;
+      for (int f : plans[i].touched) w = std::max(w, maxw[f] + 1);
+      wave[i] = w;
+      for (int f : plans[i].touched) maxw[f] = std::max(maxw[f], w);
+      nw = std::max(nw, w + 1);
     }
-    nw = std::max(nw, wave[i] + 1);
   }
   info_[ilvl_].elim_waves = nw;
   info_[ilvl_].n_elim = (int)plans.size();
@@ -776,20 +784,26 @@ void Engine<T>::sparsify_level(const std::vector<int>& cs) {
   std::vector<int> S;
   for (int c : cs)
     if (cl_[c].csize > 0) S.push_back(c);
+  // waves: an interface waits for every earlier adjacent interface (an edge
+  // either way); neighbours come from the out- and in-edge lists
+  std::vector<int> slot(cl_.size(), -1);
+  for (size_t i = 0; i < S.size(); ++i) slot[S[i]] = (int)i;
   std::vector<int> wave(S.size(), 0);
   int nw = S.empty() ? 0 : 1;
   for (size_t i = 0; i < S.size(); ++i) {
-    for (size_t j = 0; j < i; ++j) {
-      const bool adj = find_out(S[i], S[j]) >= 0 || find_out(S[j], S[i]) >= 0;
-      if (adj) wave[i] = std::max(wave[i], wave[j] + 1);
-    }
-    nw = std::max(nw, wave[i] + 1);
+    int w = 0;
+    auto see = [&](int n) {
+      const int jj = n >= 0 ? slot[n] : -1;
+      if (jj >= 0 && jj < (int)i) w = std::max(w, wave[jj] + 1);
+    };
+    for (int e : cl_[S[i]].out) if (ed_[e].n2 != S[i]) see(ed_[e].n2);
+    for (int e : cl_[S[i]].in) if (ed_[e].n1 != S[i]) see(ed_[e].n1);
+    wave[i] = w;
+    nw = std::max(nw, w + 1);
   }
   info_[ilvl_].sprs_waves = nw;
   info_[ilvl_].n_sprs = (int)S.size();
   std::vector<SparsifyInfo> recs(S.size());
-  std::vector<int> slot(cl_.size(), -1);
-  for (size_t i = 0; i < S.size(); ++i) slot[S[i]] = (int)i;
   for (int w = 0; w < nw; ++w) {
     scratch_.rewind();
     struct Job {
@@ -804,6 +818,9 @@ void Engine<T>::sparsify_level(const std::vector<int>& cs) {
     std::vector<Qp3Desc<T>> qd;
     int kmax = 0;
     int* ranks_dev = (int*)scratch_.alloc(sizeof(int) * S.size());
+    std::vector<int> sync_off;  // barrier counters of each problem (words), one array cleared by one launch
+    int sync_words = 0;
+    std::vector<int> nts;       // threads per workgroup of each register-form problem
     std::vector<int> wave_idx;
     for (size_t i = 0; i < S.size(); ++i)
       if (wave[i] == w) { wave_idx.push_back((int)i); kmax = std::max(kmax, cl_[S[i]].csize); }
@@ -821,12 +838,23 @@ void Engine<T>::sparsify_level(const std::vector<int>& cs) {
         if (!cl_[ed_[e].n2].elim && ed_[e].n2 != c) { jb.amp.push_back(e); jb.smp += ed_[e].a.rows; }
       jb.N = jb.smp + spm;
       const int N = std::max(jb.N, 1);
-      const int nwg = lds_ok ? std::max(1, std::min(qp3_max_wg_, jb.N / 256)) : 1;
+      // register form when the columns fit the resident workgroups, else
+      // the streaming form
+      int G = 0, nwg = 1, nt = 256;
+      if (qp3_reg_ && jb.k <= 64 * qp3r_rows()) {
+        G = 1;
+        while (G * qp3r_rows() < jb.k) G <<= 1;
+        // large problems: 1024-thread workgroups (one per CU, fewer to meet)
+        nt = iceil(N, 256 / G) > 32 ? 1024 : 256;
+        nwg = iceil(N, nt / G);
+        if (nwg > qp3r_cap(nt)) G = 0;
+      }
+      if (G == 0) nwg = lds_ok ? std::max(1, std::min(qp3_max_wg_, jb.N / 256)) : 1;
       jb.Ct = alloc_scr((size_t)N * jb.k);
       jb.V = alloc_scr((size_t)jb.k * jb.k);
       jb.tau = alloc_scr(jb.k);
-      jb.vn = (double*)scratch_.alloc(sizeof(double) * 2 * N);
-      jb.pos = (int*)scratch_.alloc(sizeof(int) * 2 * (size_t)N * nwg);
+      jb.vn = G ? nullptr : (double*)scratch_.alloc(sizeof(double) * 2 * N);
+      jb.pos = (int*)scratch_.alloc(sizeof(int) * (G ? (size_t)N : 2 * (size_t)N * nwg));
       int off = 0;
       for (int e : jb.amp) {
         const Block& b = ed_[e].a;
@@ -843,10 +871,13 @@ void Engine<T>::sparsify_level(const std::vector<int>& cs) {
       d.part_v = (double*)scratch_.alloc(sizeof(double) * 2 * nwg);
       d.part_p = (int*)scratch_.alloc(sizeof(int) * 4 * nwg);
       d.beta = (double*)scratch_.alloc(sizeof(double) * std::max(jb.k, 1));
-      d.sync = (unsigned int*)scratch_.alloc(256);
+      sync_off.push_back(sync_words);
+      sync_words += barrier_words(nwg);
+      nts.push_back(G ? nt : 0);
       d.err = qp3_err_;
       d.rank_out = ranks_dev + jobs.size();
-      d.ldc = N; d.k = jb.k; d.N = jb.N; d.ldv = jb.k; d.tol = tol_; d.nwg = nwg;
+      d.ldc = N; d.k = jb.k; d.N = jb.N; d.ldv = jb.k; d.tol = tol_; d.nwg = nwg; d.G = G;
+      d.part_col = G ? alloc_scr((size_t)2 * nwg * jb.k) : nullptr;
       qd.push_back(d);
       const int kk = std::min(jb.k, jb.N);
       flops_ += 4.0 * jb.k * jb.N * kk - 2.0 * (jb.k + jb.N) * kk * kk + 4.0 / 3.0 * kk * kk * kk;
@@ -858,36 +889,52 @@ void Engine<T>::sparsify_level(const std::vector<int>& cs) {
       std::fill(ranks.begin(), ranks.end(), 0);  // util.cpp:798-799
     } else {
       // barrier counters start at zero
-      for (auto& d : qd) launch_zero(stream_, 4, d.sync, 1);
-      // single-workgroup problems in one plain launch; multi-workgroup ones
-      // cooperatively, packed within the resident-workgroup capacity
-      const int cap = qp3_capacity<T>(kmax);
-      std::vector<std::vector<Qp3Desc<T>>> groups(1);
-      std::vector<Qp3Desc<T>> multi;
-      for (auto& d : qd) (d.nwg == 1 ? groups[0] : multi).push_back(d);
-      std::vector<int> coop_flag{0};
-      int used = 0;
-      for (auto& d : multi) {
-        if (d.nwg > cap) d.nwg = std::max(1, cap);
-        if (coop_flag.size() == 1 || used + d.nwg > cap) { groups.push_back({}); coop_flag.push_back(1); used = 0; }
-        groups.back().push_back(d);
-        used += d.nwg;
+      unsigned int* syncs = (unsigned int*)scratch_.alloc(sizeof(unsigned int) * std::max(sync_words, 1));
+      for (size_t q = 0; q < qd.size(); ++q) qd[q].sync = syncs + sync_off[q];
+      launch_zero(stream_, 4, syncs, sync_words);
+      // per kernel form: single-workgroup problems in one plain launch;
+      // multi-workgroup ones cooperatively, packed within the resident-
+      // workgroup capacity
+      struct Group { int nt, coop, gmax; std::vector<Qp3Desc<T>> d; };
+      std::vector<Group> groups;
+      for (int nt : {0, 256, 1024}) {  // 0: streaming form
+        const int cap = nt ? qp3r_cap(nt) : qp3_capacity<T>(kmax);
+        Group single{nt, 0, 1, {}};
+        Group cur{nt, 1, 1, {}};
+        int used = 0;
+        for (size_t q = 0; q < qd.size(); ++q) {
+          if (nts[q] != nt) continue;
+          Qp3Desc<T> d = qd[q];
+          if (d.nwg == 1) {
+            single.d.push_back(d);
+            single.gmax = std::max(single.gmax, d.G);
+            continue;
+          }
+          if (!nt && d.nwg > cap) d.nwg = std::max(1, cap);
+          if (used + d.nwg > cap && !cur.d.empty()) { groups.push_back(cur); cur = Group{nt, 1, 1, {}}; used = 0; }
+          cur.d.push_back(d);
+          cur.gmax = std::max(cur.gmax, d.G);
+          used += d.nwg;
+        }
+        if (!single.d.empty()) groups.push_back(single);
+        if (!cur.d.empty()) groups.push_back(cur);
       }
-      for (size_t q = 0; q < groups.size(); ++q) {
-        auto& gq = groups[q];
-        if (gq.empty()) continue;
+      for (auto& gq : groups) {
         std::vector<int> cnt, st;
         double qfl = 0, qby = 0;
-        for (auto& d : gq) {
+        for (auto& d : gq.d) {
           cnt.push_back(d.nwg);
           qfl += 4.0 * d.k * d.N * std::min(d.k, d.N);  // upper bound: full rank steps
           qby += 2.0 * sizeof(T) * (double)d.k * d.N;
         }
         const int nt = prefix_tiles(cnt, st);
-        const Qp3Desc<T>* dqd = stager_.push(gq);
+        const Qp3Desc<T>* dqd = stager_.push(gq.d);
         const int* dst = stager_.push(st);
         int rc = 0;
-        timed("qp3", qfl, qby, [&] { rc = launch_qp3<T>(stream_, dqd, dst, (int)gq.size(), nt, kmax, coop_flag[q]); });
+        timed("qp3", qfl, qby, [&] {
+          rc = gq.nt ? launch_qp3r<T>(stream_, dqd, dst, (int)gq.d.size(), nt, gq.gmax, gq.nt, gq.coop)
+                     : launch_qp3<T>(stream_, dqd, dst, (int)gq.d.size(), nt, kmax, gq.coop);
+        });
         if (rc != 0) throw Error(kErrHip, std::string("qp3 launch failed: ") + hipGetErrorString((hipError_t)rc));
       }
       SPQ_HIP(hipMemcpyAsync(rank_host_, ranks_dev, sizeof(int) * jobs.size(), hipMemcpyDeviceToHost, stream_));
@@ -895,6 +942,19 @@ void Engine<T>::sparsify_level(const std::vector<int>& cs) {
       sync();
       if (rank_host_[65535] != 0) throw Error(kErrNumeric, "qp3: an inter-workgroup barrier wait timed out");
       for (size_t q = 0; q < jobs.size(); ++q) ranks[q] = rank_host_[q];
+    }
+    if (verbose_ > 1) {
+      int nmax = 0, nwgs = 0, rsum = 0, ksum = 0;
+      double kn = 0;
+      for (size_t q = 0; q < jobs.size(); ++q) {
+        nmax = std::max(nmax, jobs[q].N);
+        nwgs += qd[q].nwg;
+        rsum += ranks[q];
+        ksum += jobs[q].k;
+        kn += (double)jobs[q].k * jobs[q].N;
+      }
+      fprintf(stderr, "sprs lvl %d wave %d: problems %zu kmax %d Nmax %d sum_kN %.3g workgroups %d ranks %d/%d\n", ilvl_,
+              w, jobs.size(), kmax, nmax, kn, nwgs, rsum, ksum);
     }
     std::vector<CopyDesc> scatter;
     std::vector<LarftDesc<T>> lf;
@@ -1045,6 +1105,19 @@ int Engine<T>::factorize() {  // tree.cpp:1309-1437
     const double t6 = wall();
     li.t_fill = t1 - t0; li.t_elmn = t2 - t1; li.t_shift = t3 - t2; li.t_scale = t4 - t3;
     li.t_sprsfy = t5 - t4; li.t_merge = t6 - t5;
+    if (verbose_ > 1 && kprof_) {
+      static thread_local std::map<std::string, double> prev;
+      if (ilvl_ == 0) prev.clear();
+      std::string line;
+      for (auto& k : kernel_stats()) {
+        char buf[96];
+        snprintf(buf, sizeof buf, " %s %.1f", k.name.c_str(), k.ms - prev[k.name]);
+        line += buf;
+        prev[k.name] = k.ms;
+      }
+      fprintf(stderr, "lvl %d kernel ms:%s | wall %.1f | elim %d in %d waves, sprs %d in %d waves\n", ilvl_,
+              line.c_str(), 1e3 * (t6 - t0), li.n_elim, li.elim_waves, li.n_sprs, li.sprs_waves);
+    }
     auto ts = topsize();
     li.top_rows = ts.first; li.top_cols = ts.second;
     if (verbose_)

@@ -7,6 +7,12 @@
 
 namespace spq {
 
+// Inter-workgroup barriers: groups of more than kFanIn workgroups use a
+// two-level counter tree (fan-in counters on separate 64-byte lines).
+// barrier_words(nwg) = unsigned ints of counter storage a group needs.
+constexpr int kFanIn = 16;
+inline int barrier_words(int nwg) { return nwg <= kFanIn ? 16 : 16 * (1 + (nwg + kFanIn - 1) / kFanIn); }
+
 // 2-D block copy: dst(i,j) = src(i,j) (trans=0) or dst(j,i) = src(i,j)
 // (trans=1) for i < rows, j < cols of src.  Col-major, element offsets.
 // Source and destination may have different element types (double/float).
@@ -72,22 +78,29 @@ struct TrsmDesc {
 // col-major (ld = ldc >= N), so row r of C is column r of Ct.  A problem is
 // processed by `nwg` workgroups that own contiguous column slices and meet at
 // one inter-workgroup barrier per pivot step (nwg == 1: no barrier).
+//
+// Two kernels share the descriptor.  The streaming form (G == 0) keeps C in
+// memory and uses vn, pos (nwg * 2N: per-workgroup perm / ipos copies).  The
+// register form (G > 0) holds each column in the registers of G lanes and uses
+// pos (N: position -> column table written by workgroup 0) and part_col
+// (2 * nwg * k: every workgroup's local-best column, double-buffered).
 template <typename T>
 struct Qp3Desc {
   T* Ct;
   T* V;               // k x k (ld ldv): reflector i in column i (unit lower)
   T* tau;             // k
-  double* vn;         // 2*N partial norms (FP64)
-  int* pos;           // nwg * 2N: per-workgroup copy of perm / ipos
+  double* vn;         // 2*N partial norms (FP64), streaming form
+  int* pos;
   double* part_v;     // 2 * nwg partial argmax values (double-buffered by step parity)
   int* part_p;        // 2 * nwg * 2: (position, column) of the partial argmax
   double* beta;       // k: R(i,i) of each step
   unsigned int* sync; // barrier counter (zero on entry)
   int* err;           // set when a barrier wait gives up
   int* rank_out;      // device int
+  T* part_col;        // register form: published local-best columns
   int ldc, k, N, ldv;
   double tol;
-  int nwg, pad_;
+  int nwg, G;
 };
 
 // Solve-side vector segment (cluster x buffer offset, length).

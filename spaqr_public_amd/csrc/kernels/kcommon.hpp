@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "descs.hpp"
+
 namespace spq {
 
 constexpr int kWave = 64;
@@ -52,7 +54,9 @@ __device__ __forceinline__ void block_sum_n(T (&v)[N], T* red, int n) {
 
 // Inter-workgroup barrier on a monotonically increasing counter (target =
 // arrivals expected so far).  All workgroups of the group must be resident;
-// the wait is bounded so the kernel always terminates (err is set instead).
+// the wait is bounded so the kernel always terminates: a wait that gives up
+// sets err, and a workgroup that sees err set stops waiting as well (the
+// results are then invalid and the host reports the error).
 __device__ __forceinline__ void group_barrier(unsigned int* ctr, unsigned int target, int* err) {
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -61,13 +65,45 @@ __device__ __forceinline__ void group_barrier(unsigned int* ctr, unsigned int ta
     unsigned int spins = 0;
     while (__hip_atomic_load(ctr, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
       __builtin_amdgcn_s_sleep(2);
-      if (++spins > (1u << 24)) { atomicOr(err, 1); break; }
+      ++spins;
+      if ((spins & 255u) == 0 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;
+      if (spins > (1u << 20)) { atomicOr(err, 1); break; }
     }
   }
   __syncthreads();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
 }
 
+// Two-level form for large groups.  Workgroup g arrives on fan-in counter
+// 1 + g/16 (counters 64 bytes apart in `sync`); the last arrival of a fan-in
+// group in a phase forwards one arrival to the root counter sync[0], which
+// every workgroup polls.  All counters start at zero; `phase` numbers the
+// barriers 1, 2, ...  Waits are bounded as in group_barrier.
+__device__ __forceinline__ void group_barrier2(unsigned int* sync, int g, int nwg, unsigned int phase, int* err) {
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int grp = g / kFanIn, ngrp = (nwg + kFanIn - 1) / kFanIn;
+    const unsigned int sz = (unsigned int)min(kFanIn, nwg - grp * kFanIn);
+    __threadfence();
+    const unsigned int old = atomicAdd(sync + 16 * (1 + grp), 1u);
+    if (old + 1 == phase * sz) {
+      // last of the fan-in group: acquire its members' writes, release them on
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+      __threadfence();
+      atomicAdd(sync, 1u);
+    }
+    const unsigned int target = phase * (unsigned int)ngrp;
+    unsigned int spins = 0;
+    while (__hip_atomic_load(sync, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+      __builtin_amdgcn_s_sleep(2);
+      ++spins;
+      if ((spins & 255u) == 0 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;
+      if (spins > (1u << 20)) { atomicOr(err, 1); break; }
+    }
+  }
+  __syncthreads();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+}
 // Find the descriptor owning flattened tile `t` given exclusive prefix sums
 // `start[0..n)` (start[0] == 0, nondecreasing).
 __device__ __forceinline__ int find_owner(const int* start, int n, int t) {

@@ -199,6 +199,278 @@ __global__ __launch_bounds__(kQThreads) void k_qp3(const Qp3Desc<T>* __restrict_
   if (g == 0 && tid == 0) *d.rank_out = rank;
 }
 
+// ---------------------------------------------------------------------------
+// Register form.  Column j of C lives in the registers of an aligned group of
+// G lanes (G a power of two): lane l of the group holds the kRC contiguous
+// rows [l*kRC, (l+1)*kRC) (rows >= k zero), so C is read once and written
+// once and a pivot step moves only O(k) values per workgroup.  Per step every
+// workgroup publishes its local-best column next to its (norm, position,
+// column) partial; after one inter-workgroup barrier all workgroups pick the
+// same winner and read its column from the winner's slot.  Positions are
+// tracked per column in registers; the position -> column table `pos` is
+// written by workgroup 0 only (entry p is read at step p, after the barrier
+// that follows its last write).  Arithmetic, pivot rule, downdate and stop
+// test are those of the streaming form.
+constexpr int kRC = 32;
+constexpr int kRP = kRC + 1;  // LDS stride per lane's rows (one pad element)
+
+__device__ __forceinline__ void argmax4(double& v, int& p, int& c, int& w, double ov, int op, int oc, int ow) {
+  if (ov > v || (ov == v && op < p)) { v = ov; p = op; c = oc; w = ow; }
+}
+// Butterfly sum over aligned groups of G lanes; every lane of a group ends
+// with the same bits (IEEE addition is commutative).
+__device__ __forceinline__ double group_sum(double v, int G) {
+  for (int o = 1; o < G; o <<= 1) v += __shfl_xor(v, o, kWave);
+  return v;
+}
+// LDS slot of row r in the padded reflector layout
+__device__ __forceinline__ int vslot(int r) { return r + r / kRC; }
+
+template <typename T, int NT>
+__global__ __launch_bounds__(NT, 1024 / NT) void k_qp3r(const Qp3Desc<T>* __restrict__ descs,
+                                                     const int* __restrict__ wstart, int nd) {
+  extern __shared__ unsigned char smem_raw[];
+  constexpr int NW = NT / kWave;
+  __shared__ double redv[NW];
+  __shared__ int redp[NW], redc[NW], redw[NW];
+  __shared__ double sred[NW];
+  __shared__ double s_bv;
+  __shared__ int s_bp, s_bc, s_piv, s_pos, s_gw, s_q;
+
+  const int p = find_owner(wstart, nd, blockIdx.x);
+  const Qp3Desc<T> d = descs[p];
+  const int g = blockIdx.x - wstart[p], nwg = d.nwg;
+  const int k = d.k, N = d.N, tid = threadIdx.x, lane = tid & (kWave - 1);
+  const int G = d.G, gl = tid & (G - 1), rbase = gl * kRC;
+  const int j = g * (NT / G) + tid / G;  // this group's column
+  const bool has = j < N;
+  T* vsh = reinterpret_cast<T*>(smem_raw);  // G*kRP entries: pivot column, then reflector
+  const T* vmy = vsh + gl * kRP;           // this lane's rows
+  const int kmin = min(k, N);
+  const double tol3z = sqrt(ldexp(1.0, -53));
+
+  T c[kRC];
+  {
+    const T* src = d.Ct + j + (int64_t)rbase * d.ldc;
+#pragma unroll
+    for (int q = 0; q < kRC; ++q) c[q] = (has && rbase + q < k) ? src[(int64_t)q * d.ldc] : T(0);
+  }
+  for (int r = k + tid; r < G * kRC; r += NT) vsh[vslot(r)] = T(0);
+  double s = 0;
+#pragma unroll
+  for (int q = 0; q < kRC; ++q) s += (double)c[q] * (double)c[q];
+  s = group_sum(s, G);
+  double vn1 = sqrt(s), vn2 = vn1, beta_own = 0;
+  int pos = j;
+  if (has && gl == 0) d.pos[j] = j;
+  double bv = -2.0;
+  int bp = INT_MAX, bc = -1;
+  if (has && gl == 0) { bv = (vn1 == vn1) ? vn1 : -1.0; bp = j; bc = j; }
+
+  double r00 = 0;
+  int rank = kmin;
+  unsigned int target = 0;
+  for (int i = 0; i < kmin; ++i) {
+    const int buf = i & 1;
+    // ---- local argmax of the workgroup
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) {
+      const double ov = __shfl_xor(bv, o, kWave);
+      const int op = __shfl_xor(bp, o, kWave), oc = __shfl_xor(bc, o, kWave);
+      argmax_combine(bv, bp, bc, ov, op, oc);
+    }
+    if (lane == 0) { redv[tid / kWave] = bv; redp[tid / kWave] = bp; redc[tid / kWave] = bc; }
+    __syncthreads();
+    if (tid == 0) {
+      double v = redv[0];
+      int pp = redp[0], cc = redc[0];
+      for (int w = 1; w < NW; ++w) argmax_combine(v, pp, cc, redv[w], redp[w], redc[w]);
+      s_bv = v; s_bp = pp; s_bc = cc;
+    }
+    __syncthreads();
+    // ---- publish the local-best column and its partial
+    if (has && j == s_bc) {
+      T* __restrict__ pc = d.part_col + ((int64_t)buf * nwg + g) * k + rbase;
+#pragma unroll
+      for (int q = 0; q < kRC; ++q)
+        if (rbase + q < k) pc[q] = c[q];
+      if (nwg > 1) __threadfence();
+    }
+    if (tid == 0) {
+      d.part_v[buf * nwg + g] = s_bv;
+      d.part_p[2 * (buf * nwg + g)] = s_bp;
+      d.part_p[2 * (buf * nwg + g) + 1] = s_bc;
+    }
+    if (nwg > kFanIn) {
+      group_barrier2(d.sync, g, nwg, (unsigned int)(i + 1), d.err);
+    } else if (nwg > 1) {
+      target += nwg;
+      group_barrier(d.sync, target, d.err);
+    } else {
+      __syncthreads();
+    }
+    // ---- the same winner in every workgroup
+    {
+      double v = -3.0;
+      int pp = INT_MAX, cc = -1, ww = -1;
+      for (int w = tid; w < nwg; w += NT)
+        argmax4(v, pp, cc, ww, d.part_v[buf * nwg + w], d.part_p[2 * (buf * nwg + w)],
+                d.part_p[2 * (buf * nwg + w) + 1], w);
+#pragma unroll
+      for (int o = kWave / 2; o > 0; o >>= 1) {
+        const double ov = __shfl_xor(v, o, kWave);
+        const int op = __shfl_xor(pp, o, kWave), oc = __shfl_xor(cc, o, kWave), ow = __shfl_xor(ww, o, kWave);
+        argmax4(v, pp, cc, ww, ov, op, oc, ow);
+      }
+      if (lane == 0) { redv[tid / kWave] = v; redp[tid / kWave] = pp; redc[tid / kWave] = cc; redw[tid / kWave] = ww; }
+      __syncthreads();
+      if (tid == 0) {
+        for (int w = 1; w < NW; ++w) argmax4(v, pp, cc, ww, redv[w], redp[w], redc[w], redw[w]);
+        s_piv = cc; s_pos = pp; s_gw = ww;
+        s_q = d.pos[i];
+      }
+      __syncthreads();
+    }
+    const int piv = s_piv, ppos = s_pos, gw = s_gw, qcol = s_q;
+    // ---- Householder vector of the pivot column (identical in every workgroup)
+    const T* __restrict__ pcol = d.part_col + ((int64_t)buf * nwg + gw) * k;
+    for (int r = tid; r < k; r += NT) vsh[vslot(r)] = pcol[r];
+    __syncthreads();
+    double ss = 0;
+    for (int r = i + 1 + tid; r < k; r += NT) { const double v = (double)vsh[vslot(r)]; ss += v * v; }
+    ss = block_sum(ss, sred);
+    const double alpha = (double)vsh[vslot(i)];
+    double beta = alpha, tau = 0, scal = 0;
+    if (ss > 0) {
+      beta = -copysign(hypot(alpha, sqrt(ss)), alpha);
+      tau = (beta - alpha) / beta;
+      scal = 1.0 / (alpha - beta);
+    }
+    if (i == 0) r00 = fabs(beta);
+    if (i >= 1 && !(fabs(beta) / r00 >= d.tol)) { rank = i; break; }
+    __syncthreads();  // alpha read everywhere before vsh is overwritten
+    for (int r = tid; r < k; r += NT) {
+      const T v = (r < i) ? T(0) : (r == i ? T(1) : (T)((double)vsh[vslot(r)] * scal));
+      vsh[vslot(r)] = v;
+      if (g == 0) d.V[r + (int64_t)i * d.ldv] = v;
+    }
+    if (g == 0 && tid == 0) {
+      d.tau[i] = (T)tau;
+      d.beta[i] = beta;
+      d.pos[ppos] = qcol;  // LAPACK swap of positions i and ppos
+    }
+    if (has) {
+      if (j == qcol) pos = ppos;
+      if (j == piv) { pos = i; beta_own = beta; }
+    }
+    __syncthreads();
+    // ---- apply H_i to the live columns, downdate, next local candidate
+    bv = -2.0;
+    bp = INT_MAX;
+    bc = -1;
+    if (has && pos > i) {  // uniform over the group
+      const int li = i - rbase;  // row i relative to this lane's first row
+      // dot product in chunks of 8 LDS reads (bounds the live registers)
+      double w = 0;
+#pragma unroll
+      for (int q0 = 0; q0 < kRC; q0 += 8) {
+#pragma unroll
+        for (int q = q0; q < q0 + 8; ++q) w += (double)vmy[q] * (double)c[q];
+        asm volatile("" ::: "memory");
+      }
+      w = group_sum(w, G) * tau;
+      if (w != 0.0) {
+#pragma unroll
+        for (int q0 = 0; q0 < kRC; q0 += 8) {
+#pragma unroll
+          for (int q = q0; q < q0 + 8; ++q)
+            if (q >= li) c[q] = (T)((double)c[q] - w * (double)vmy[q]);
+          asm volatile("" ::: "memory");
+        }
+      }
+      T mine = T(0);
+#pragma unroll
+      for (int q = 0; q < kRC; ++q)
+        if (q == li) mine = c[q];
+      const double cii = (double)__shfl(mine, (lane & ~(G - 1)) | (i / kRC), kWave);
+      double a = vn1;
+      if (a != 0.0) {
+        double tt = fabs(cii) / a;
+        tt = 1.0 - tt * tt;
+        tt = tt < 0.0 ? 0.0 : tt;
+        const double qq = a / vn2;
+        const double tt2 = tt * qq * qq;
+        if (tt2 <= tol3z) {
+          if (i < k - 1) {
+            double s2 = 0;
+#pragma unroll
+            for (int q = 0; q < kRC; ++q)
+              if (q > li) s2 += (double)c[q] * (double)c[q];
+            a = sqrt(group_sum(s2, G));
+            vn2 = a;
+          } else {
+            a = 0.0;
+            vn2 = 0.0;
+          }
+        } else {
+          a = a * sqrt(tt);
+        }
+        vn1 = a;
+      }
+      if (gl == 0) { bv = (a == a) ? a : -1.0; bp = pos; bc = j; }
+    }
+  }
+  // pivoted columns: R(0:pos, j), beta on the pivot row, zeros below; the
+  // others keep their updated values
+  if (has) {
+    int64_t ldc = d.ldc;
+    asm volatile("" : "+s"(ldc));  // recompute the addresses here instead of keeping them live
+    T* dst = d.Ct + j + (int64_t)rbase * ldc;
+    const int lp = (pos < rank) ? pos - rbase : kRC;  // first row (relative) replaced by beta / zero
+#pragma unroll
+    for (int q = 0; q < kRC; ++q) {
+      if (rbase + q < k) {
+        T v = c[q];
+        if (q >= lp) v = (q == lp) ? (T)beta_own : T(0);
+        dst[(int64_t)q * ldc] = v;
+      }
+    }
+  }
+  if (g == 0 && tid == 0) *d.rank_out = rank;
+}
+
+int qp3r_rows() { return kRC; }
+
+template <typename T>
+int qp3r_capacity(int G, int nt) {
+  int per_cu = 0, dev = 0, cus = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+  const void* f = nt == 1024 ? (const void*)k_qp3r<T, 1024> : (const void*)k_qp3r<T, 256>;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, nt, sizeof(T) * (size_t)G * kRP) != hipSuccess) return 0;
+  return per_cu * cus;
+}
+
+template <typename T>
+int launch_qp3r(hipStream_t s, const Qp3Desc<T>* d, const int* wstart, int nd, int nwg_total, int gmax, int nt,
+                int coop) {
+  if (nd <= 0) return 0;
+  if (nt != 256 && nt != 1024) return (int)hipErrorInvalidValue;
+  const size_t shm = sizeof(T) * (size_t)gmax * kRP;
+  const void* f = nt == 1024 ? (const void*)k_qp3r<T, 1024> : (const void*)k_qp3r<T, 256>;
+  if (!coop) {
+    if (nt == 1024) hipLaunchKernelGGL((k_qp3r<T, 1024>), dim3(nwg_total), dim3(1024), shm, s, d, wstart, nd);
+    else hipLaunchKernelGGL((k_qp3r<T, 256>), dim3(nwg_total), dim3(256), shm, s, d, wstart, nd);
+    return (int)hipGetLastError();
+  }
+  void* args[] = {(void*)&d, (void*)&wstart, (void*)&nd};
+  return (int)hipLaunchCooperativeKernel(f, dim3(nwg_total), dim3(nt), args, shm, s);
+}
+template int qp3r_capacity<double>(int, int);
+template int qp3r_capacity<float>(int, int);
+template int launch_qp3r<double>(hipStream_t, const Qp3Desc<double>*, const int*, int, int, int, int, int);
+template int launch_qp3r<float>(hipStream_t, const Qp3Desc<float>*, const int*, int, int, int, int, int);
+
 int qp3_threads() { return kQThreads; }
 
 // Workgroups of one launch that can be resident at once (cooperative bound).

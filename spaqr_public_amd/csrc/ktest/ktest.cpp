@@ -111,25 +111,41 @@ int ktest_larft(int m, int r, const double* V, const double* tau, double* Tb) {
   return 0;
 }
 
-// Truncated QP3 of C (k x N): Ct is N x k (ld N) in/out; V k x k, tau k; the
-// problem is split over nwg workgroups (cooperative launch when nwg > 1).
-// Returns the rank, or -1 on a launch/run error.
-int ktest_qp3(int k, int N, double* Ct, double* V, double* tau, double tol, int nwg) {
+// Truncated QP3 of C (k x N): Ct is N x k (ld N) in/out; V k x k, tau k.
+// form 0: streaming kernel split over nwg workgroups; form 1 / 2: register
+// kernel with 256 / 1024-thread workgroups (group size and workgroup count
+// follow from k and N).  Multi-
+// workgroup problems are launched cooperatively.  Returns the rank, or -1 on
+// a launch/run error.
+int ktest_qp3(int k, int N, double* Ct, double* V, double* tau, double tol, int nwg, int form) {
+  int G = 0, nt = 256;
+  if (form >= 1) {
+    if (k > 64 * qp3r_rows()) { g_err = "k too large for the register form"; return -1; }
+    G = 1;
+    while (G * qp3r_rows() < k) G <<= 1;
+    nt = form == 2 ? 1024 : 256;
+    nwg = (N + nt / G - 1) / (nt / G);
+    if (nwg > qp3r_capacity<double>(G, nt)) { g_err = "nwg exceeds resident capacity"; return -1; }
+  } else if (nwg > qp3_capacity<double>(k)) {
+    g_err = "nwg exceeds resident capacity";
+    return -1;
+  }
   double *dC = dup(Ct, (size_t)N * k), *dV = dup<double>(nullptr, (size_t)k * k), *dt = dup<double>(nullptr, k);
   double* vn = dup<double>(nullptr, 2 * (size_t)N);
   int* pos = dup<int>(nullptr, 2 * (size_t)N * nwg);
   double* pv = dup<double>(nullptr, 2 * (size_t)nwg);
   int* pp = dup<int>(nullptr, 4 * (size_t)nwg);
   double* beta = dup<double>(nullptr, k);
-  unsigned int* sync = dup<unsigned int>(nullptr, 1);
+  unsigned int* sync = dup<unsigned int>(nullptr, barrier_words(nwg));
   int* err = dup<int>(nullptr, 1);
   int* rk = dup<int>(nullptr, 1);
-  Qp3Desc<double> d{dC, dV, dt, vn, pos, pv, pp, beta, sync, err, rk, N, k, N, k, tol, nwg, 0};
+  double* pcol = dup<double>(nullptr, 2 * (size_t)nwg * k);
+  Qp3Desc<double> d{dC, dV, dt, vn, pos, pv, pp, beta, sync, err, rk, pcol, N, k, N, k, tol, nwg, G};
   Qp3Desc<double>* dd = dup(&d, 1);
   int zero = 0;
   int* ws = dup(&zero, 1);
-  if (nwg > qp3_capacity<double>(k)) { g_err = "nwg exceeds resident capacity"; return -1; }
-  int rc = launch_qp3<double>(0, dd, ws, 1, nwg, k, nwg > 1 ? 1 : 0);
+  int rc = form >= 1 ? launch_qp3r<double>(0, dd, ws, 1, nwg, G, nt, nwg > 1 ? 1 : 0)
+                     : launch_qp3<double>(0, dd, ws, 1, nwg, k, nwg > 1 ? 1 : 0);
   if (rc != 0) { g_err = hipGetErrorString((hipError_t)rc); return -1; }
   KT(hipDeviceSynchronize());
   int rank = 0, e = 0;
@@ -139,7 +155,68 @@ int ktest_qp3(int k, int N, double* Ct, double* V, double* tau, double tol, int 
   back(V, dV, (size_t)k * k);
   back(tau, dt, k);
   for (void* q : {(void*)dC, (void*)dV, (void*)dt, (void*)vn, (void*)pos, (void*)pv, (void*)pp, (void*)beta,
-                  (void*)sync, (void*)err, (void*)rk, (void*)dd, (void*)ws})
+                  (void*)sync, (void*)err, (void*)rk, (void*)dd, (void*)ws, (void*)pcol})
+    (void)hipFree(q);
+  if (e) { g_err = "barrier wait timed out"; return -1; }
+  return rank;
+}
+
+// Timing of one truncated QP3 problem (either form): `reps` launches on a
+// fresh copy of C each, the average kernel time in *ms.  Returns the rank.
+int ktest_qp3_bench(int k, int N, const double* Ct, double tol, int form, int nwg, int reps, double* ms) {
+  int G = 0, nt = 256;
+  if (form >= 1) {
+    if (k > 64 * qp3r_rows()) { g_err = "k too large for the register form"; return -1; }
+    G = 1;
+    while (G * qp3r_rows() < k) G <<= 1;
+    nt = form == 2 ? 1024 : 256;
+    nwg = (N + nt / G - 1) / (nt / G);
+    if (nwg > qp3r_capacity<double>(G, nt)) { g_err = "nwg exceeds resident capacity"; return -1; }
+  } else if (nwg > qp3_capacity<double>(k)) {
+    g_err = "nwg exceeds resident capacity";
+    return -1;
+  }
+  double *d0 = dup(Ct, (size_t)N * k), *dC = dup<double>(nullptr, (size_t)N * k);
+  double *dV = dup<double>(nullptr, (size_t)k * k), *dt = dup<double>(nullptr, k);
+  double* vn = dup<double>(nullptr, 2 * (size_t)N);
+  int* pos = dup<int>(nullptr, 2 * (size_t)N * nwg);
+  double* pv = dup<double>(nullptr, 2 * (size_t)nwg);
+  int* pp = dup<int>(nullptr, 4 * (size_t)nwg);
+  double* beta = dup<double>(nullptr, k);
+  unsigned int* sync = dup<unsigned int>(nullptr, barrier_words(nwg));
+  int* err = dup<int>(nullptr, 1);
+  int* rk = dup<int>(nullptr, 1);
+  double* pcol = dup<double>(nullptr, 2 * (size_t)nwg * k);
+  Qp3Desc<double> d{dC, dV, dt, vn, pos, pv, pp, beta, sync, err, rk, pcol, N, k, N, k, tol, nwg, G};
+  Qp3Desc<double>* dd = dup(&d, 1);
+  int zero = 0;
+  int* ws = dup(&zero, 1);
+  hipEvent_t a, b;
+  KT(hipEventCreate(&a));
+  KT(hipEventCreate(&b));
+  double tot = 0;
+  int rc = 0;
+  for (int r = 0; r < reps && rc == 0; ++r) {
+    KT(hipMemcpy(dC, d0, sizeof(double) * (size_t)N * k, hipMemcpyDeviceToDevice));
+    KT(hipMemset(sync, 0, sizeof(unsigned int) * barrier_words(nwg)));
+    KT(hipEventRecord(a, 0));
+    rc = form >= 1 ? launch_qp3r<double>(0, dd, ws, 1, nwg, G, nt, nwg > 1 ? 1 : 0)
+                   : launch_qp3<double>(0, dd, ws, 1, nwg, k, nwg > 1 ? 1 : 0);
+    KT(hipEventRecord(b, 0));
+    KT(hipEventSynchronize(b));
+    float t = 0;
+    KT(hipEventElapsedTime(&t, a, b));
+    tot += t;
+  }
+  if (rc != 0) { g_err = hipGetErrorString((hipError_t)rc); return -1; }
+  *ms = tot / reps;
+  int rank = 0, e = 0;
+  back(&rank, rk, 1);
+  back(&e, err, 1);
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  for (void* q : {(void*)d0, (void*)dC, (void*)dV, (void*)dt, (void*)vn, (void*)pos, (void*)pv, (void*)pp,
+                  (void*)beta, (void*)sync, (void*)err, (void*)rk, (void*)dd, (void*)ws, (void*)pcol})
     (void)hipFree(q);
   if (e) { g_err = "barrier wait timed out"; return -1; }
   return rank;

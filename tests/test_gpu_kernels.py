@@ -30,7 +30,7 @@ def K(ktest):
                                  C.c_int, C.c_double, C.c_double]
     ktest.ktest_qr.argtypes = [C.c_int, C.c_int, dp, dp, dp, C.c_int, dp]
     ktest.ktest_larft.argtypes = [C.c_int, C.c_int, dp, dp, dp]
-    ktest.ktest_qp3.argtypes = [C.c_int, C.c_int, dp, dp, dp, C.c_double, C.c_int]
+    ktest.ktest_qp3.argtypes = [C.c_int, C.c_int, dp, dp, dp, C.c_double, C.c_int, C.c_int]
     ktest.ktest_refl.argtypes = [C.c_int, C.c_int, dp, dp, dp, C.c_int, C.c_int, C.c_int]
     ktest.ktest_bwd.argtypes = [C.c_int, dp, C.c_int, dp, dp, dp, C.c_int]
     ktest.ktest_trsm.argtypes = [C.c_int, C.c_int, dp, dp]
@@ -150,10 +150,16 @@ def test_larft(K, oracle, mr):
         assert rel(np.triu(Tb[:w, blk]), Tref[:w, blk]) < 1e-12
 
 
-@pytest.mark.parametrize("nwg", [1, 3])
+# form 0 = streaming kernel over nwg workgroups; form 1 / 2 = register kernel
+# with 256 / 1024-thread workgroups (lane group size and workgroup count follow
+# from k and N: with 256 threads (120, 2000) runs on 32 workgroups with 4 lanes
+# per column and (300, 1500) on 94 with 16, both through the two-level
+# barrier; with 1024 threads on 8 and 24)
+@pytest.mark.parametrize("form,nwg", [(0, 1), (0, 3), (1, 0), (2, 0)])
 @pytest.mark.parametrize("kN,true_rank,tol", [((20, 100), 6, 1e-3), ((64, 300), 20, 1e-6), ((50, 40), 50, 1e-10),
-                                              ((30, 30), 10, 1e-2), ((1, 5), 1, 1e-3), ((120, 2000), 90, 1e-8)])
-def test_qp3(K, oracle, kN, true_rank, tol, nwg):
+                                              ((30, 30), 10, 1e-2), ((1, 5), 1, 1e-3), ((120, 2000), 90, 1e-8),
+                                              ((300, 1500), 150, 1e-7)])
+def test_qp3(K, oracle, kN, true_rank, tol, form, nwg):
     k, N = kN
     rng = np.random.default_rng(k * N)
     r = min(true_rank, k, N)
@@ -168,7 +174,7 @@ def test_qp3(K, oracle, kN, true_rank, tol, nwg):
     Ct = np.asfortranarray(Cmat.T.copy())
     V = np.zeros((k, k), order="F")
     tau = np.zeros(k)
-    rank = K.ktest_qp3(k, N, d(Ct), d(V), d(tau), tol, min(nwg, N))
+    rank = K.ktest_qp3(k, N, d(Ct), d(V), d(tau), tol, min(nwg, N), form)
     assert rank == rank_ref
     Crank = Ct[:, :rank].T  # rank x N, original column order
     # reference Crank = triu(R[:rank]) P^T
