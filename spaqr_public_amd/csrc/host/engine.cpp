@@ -309,28 +309,70 @@ class Engine final : public EngineBase {
   void blocked_qr(std::vector<QrJob>& jobs) {
     int kmax = 0;
     for (auto& j : jobs) kmax = std::max(kmax, std::min(j.m, j.k));
+    // panel steps run in the register form: rows [jb, m) over nwg
+    // workgroups; barrier counters of all multi-workgroup steps come from one
+    // pool cleared by one launch
+    const int rpw = panel_rows_per_wg();
+    size_t words = 0;
+    for (int jb = 0; jb < kmax; jb += nb_)
+      for (auto& j : jobs)
+        if (jb < std::min(j.m, j.k) && iceil(j.m - jb, rpw) > 1) words += barrier_words(iceil(j.m - jb, rpw));
+    unsigned int* pool = words ? (unsigned int*)scratch_.alloc(sizeof(unsigned int) * words) : nullptr;
+    if (words) launch_zero(stream_, 4, pool, words);
+    size_t pool_used = 0;
+    const int cap = panel_r_cap();
     for (int jb = 0; jb < kmax; jb += nb_) {
-      std::vector<PanelDesc<T>> pd;
+      ws_.rewind();
+      std::vector<PanelDesc<T>> single, multi;
       std::vector<int> live;
+      double pfl = 0, pby = 0;
       for (int q = 0; q < (int)jobs.size(); ++q) {
         auto& j = jobs[q];
         if (jb >= std::min(j.m, j.k)) continue;
         PanelDesc<T> d{};
         d.P = j.P; d.tau = j.tau; d.Tb = j.Tb; d.ld = j.ld; d.m = j.m; d.k = j.k;
-        pd.push_back(d);
+        d.nwg = iceil(j.m - jb, rpw);
+        d.err = qp3_err_;
+        if (d.nwg > 1) {
+          d.part = (T*)ws_.alloc(sizeof(T) * 2 * (size_t)d.nwg * 2 * nb_);
+          d.sync = pool + pool_used;
+          pool_used += barrier_words(d.nwg);
+          multi.push_back(d);
+        } else {
+          single.push_back(d);
+        }
         live.push_back(q);
-      }
-      if (pd.empty()) break;
-      double pfl = 0, pby = 0;
-      for (auto& d : pd) {
-        const int w = std::min(nb_, std::min(d.m, d.k) - jb), r = d.m - jb, c = std::min(nb_, d.k - jb);
+        const int w = std::min(nb_, std::min(j.m, j.k) - jb), r = j.m - jb, c = std::min(nb_, j.k - jb);
         pfl += 4.0 * r * w * c + 2.0 * r * w * w;  // column steps: dots + updates, then T
         pby += 2.0 * sizeof(T) * (double)r * c;
       }
-      const PanelDesc<T>* dpd = stager_.push(pd);
-      timed("panel", pfl, pby, [&] { launch_panel<T>(stream_, dpd, (int)pd.size(), jb); });
+      if (live.empty()) break;
+      // single-workgroup panels in one plain launch; the others cooperatively,
+      // packed within the resident capacity
+      std::vector<std::pair<std::vector<PanelDesc<T>>, int>> launches;
+      if (!single.empty()) launches.push_back({single, 0});
+      std::vector<PanelDesc<T>> cur;
+      int used = 0;
+      for (auto& d : multi) {
+        if (d.nwg > cap) throw Error(kErrState, "panel: problem exceeds the resident workgroup capacity");
+        if (used + d.nwg > cap && !cur.empty()) { launches.push_back({cur, 1}); cur.clear(); used = 0; }
+        cur.push_back(d);
+        used += d.nwg;
+      }
+      if (!cur.empty()) launches.push_back({cur, 1});
+      for (size_t li = 0; li < launches.size(); ++li) {
+        auto& L = launches[li];
+        std::vector<int> cnt, st;
+        for (auto& d : L.first) cnt.push_back(d.nwg);
+        const int nt = prefix_tiles(cnt, st);
+        const PanelDesc<T>* dpd = stager_.push(L.first);
+        const int* dst = stager_.push(st);
+        int rc = 0;
+        timed("panel", li == 0 ? pfl : 0.0, li == 0 ? pby : 0.0,
+              [&] { rc = launch_panel_r<T>(stream_, dpd, dst, (int)L.first.size(), nt, jb, L.second); });
+        if (rc != 0) throw Error(kErrHip, std::string("panel launch failed: ") + hipGetErrorString((hipError_t)rc));
+      }
       // trailing matrices (Y, Z temporaries live in ws_, rewound per step)
-      ws_.rewind();
       std::vector<GemmDesc<T>> g1, g2, g3;
       for (int q : live) {
         auto& j = jobs[q];
@@ -425,6 +467,11 @@ class Engine final : public EngineBase {
   void run_program(const double* b, double* x, bool prof = false);
   bool stage_prof_ = false;  // SPAQR_SOLVE_STAGES=1: time every solve stage (uncaptured)
   bool qp3_reg_ = true;      // SPAQR_QP3_REGISTER=0: streaming truncated QR only
+  int panel_cap_ = -1;
+  int panel_r_cap() {
+    if (panel_cap_ < 0) panel_cap_ = panel_r_capacity<T>();
+    return panel_cap_;
+  }
   int qp3r_cap_[2] = {-1, -1};
   // resident workgroups of the register-form QR with nt threads per
   // workgroup (largest LDS footprint, G = 64)
@@ -1127,7 +1174,10 @@ int Engine<T>::factorize() {  // tree.cpp:1309-1437
              (double)ts.first / (double)ts.second);
   }
   build_solve_program();
+  // a multi-workgroup kernel whose barrier wait gave up leaves the flag set
+  SPQ_HIP(hipMemcpyAsync(rank_host_ + 65535, qp3_err_, sizeof(int), hipMemcpyDeviceToHost, stream_));
   sync();
+  if (rank_host_[65535] != 0) throw Error(kErrNumeric, "factorize: an inter-workgroup barrier wait timed out");
   t_factorize_ = wall() - fstart;
   top_size_ = cl_[bottoms_[nlevels_ - 1][0]].csize;
   factored_ = true;

@@ -46,14 +46,19 @@ struct GemmDesc {
 
 // Householder panel block: factor columns [jb, jb+w) of P (m x k, ld), writing
 // reflectors below the diagonal, R on/above it, tau[j] and the nb x nb block
-// T factor of this block into Tb + jb*nb (ld nb).
+// T factor of this block into Tb + jb*nb (ld nb).  The register form splits
+// rows [jb, m) over nwg workgroups (part: 2 * nwg * 64 partial sums and the
+// published pivot row; sync: barrier_words(nwg) counters, zero on entry).
 template <typename T>
 struct PanelDesc {
   T* P;
   T* tau;
   T* Tb;
   int ld, m, k;
-  int pad_;
+  int nwg;
+  T* part;
+  unsigned int* sync;
+  int* err;
 };
 
 // Block T factors of an existing reflector set (V m x r unit lower, tau).

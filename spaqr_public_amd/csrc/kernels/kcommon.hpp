@@ -52,6 +52,31 @@ __device__ __forceinline__ void block_sum_n(T (&v)[N], T* red, int n) {
   }
 }
 
+// Butterfly reduce-scatter of 32 per-lane partial sums over a wavefront: each
+// step exchanges half of the live values with lane ^ H, so 31 shuffles leave
+// lane l with the sum over its 32-lane half of a[l & 31]; one more folds the
+// two halves.  All indices are compile-time, so a[] stays in registers.
+template <typename T, int H>
+__device__ __forceinline__ void rs_step(T (&a)[32], int lane) {
+  const bool up = (lane & H) != 0;
+#pragma unroll
+  for (int j = 0; j < H; ++j) {
+    const T keep = up ? a[j + H] : a[j];
+    const T send = up ? a[j] : a[j + H];
+    a[j] = keep + __shfl_xor(send, H, kWave);
+  }
+}
+template <typename T>
+__device__ __forceinline__ T wave_reduce_scatter32(T (&a)[32]) {
+  const int lane = threadIdx.x & (kWave - 1);
+  rs_step<T, 16>(a, lane);
+  rs_step<T, 8>(a, lane);
+  rs_step<T, 4>(a, lane);
+  rs_step<T, 2>(a, lane);
+  rs_step<T, 1>(a, lane);
+  return a[0] + __shfl_xor(a[0], 32, kWave);
+}
+
 // Inter-workgroup barrier on a monotonically increasing counter (target =
 // arrivals expected so far).  All workgroups of the group must be resident;
 // the wait is bounded so the kernel always terminates: a wait that gives up

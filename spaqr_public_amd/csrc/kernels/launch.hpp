@@ -19,10 +19,15 @@ template <typename T>
 void launch_gemm(hipStream_t s, const GemmDesc<T>* d, const int* tstart, int nd, int ntiles);
 
 // panel.hip
+// Register form of the panel step: rows per workgroup, resident capacity,
+// launch (multi-workgroup problems need coop = 1 or a grid within capacity).
+int panel_rows_per_wg();
+template <typename T>
+int panel_r_capacity();
+template <typename T>
+int launch_panel_r(hipStream_t s, const PanelDesc<T>* d, const int* wstart, int nd, int nwg_total, int jb, int coop);
 int panel_nb();
 int trsm_rows_per_tile();
-template <typename T>
-void launch_panel(hipStream_t s, const PanelDesc<T>* d, int nd, int jb);
 template <typename T>
 void launch_larft(hipStream_t s, const LarftDesc<T>* d, const int* tstart, int nd, int ntiles);
 template <typename T>

@@ -1,15 +1,11 @@
 // Householder panel kernels (gfx950).
 //
-// k_panel: factors one NB-wide column block [jb, jb+w) of every panel in the
-// batch (one workgroup per panel): dlarfg-convention reflectors
-// (beta = -sign(alpha) * ||x||, tau = (beta - alpha) / beta, v(0) = 1), the
-// reflector applied to the remaining columns of the block, and the block's
-// compact-WY factor T (forward, columnwise: T(0:j,j) = -tau_j T(0:j,0:j) V^T v_j).
-// Each column step needs two workgroup reductions: the norm of the column
-// below the diagonal and the NB dot products of v_j with every column of the
-// block (those against earlier reflectors feed T, those against later columns
-// feed the update).  The caller applies the block to the trailing columns with
-// the batched GEMM.
+// k_panel_r: factors one NB-wide column block [jb, jb+w) of every panel in
+// the batch: dlarfg-convention reflectors (beta = -sign(alpha) * ||x||,
+// tau = (beta - alpha) / beta, v(0) = 1), the reflector applied to the
+// remaining columns of the block, and the block's compact-WY factor T
+// (forward, columnwise: T(0:j,j) = -tau_j T(0:j,0:j) V^T v_j).  The caller
+// applies the block to the trailing columns with the batched GEMM.
 #include "kcommon.hpp"
 #include "descs.hpp"
 #include "launch.hpp"
@@ -31,73 +27,123 @@ __device__ __forceinline__ void tcol_from_dots(T* Ts, int jj, T tau, const T* g)
   if ((int)threadIdx.x == jj) Ts[jj + jj * kNB] = tau;
 }
 
+// Register form of the panel step.  Rows [jb, m) of the block are split over
+// nwg workgroups, one row per thread with the block's 32 columns in
+// registers, so the panel is read once and written once.  Column step j needs
+// one combined reduction of d_c = sum_{r>j} x_r A(r, c) for all 32 columns
+// (x = column j below the diagonal): d_j is ||x||^2, and with the pivot row
+// A(j, :) (held by workgroup 0, published with the partial sums)
+//   v_j^T A(:, c) = A(j, c) + scal * d_c
+// gives both the update of the later columns and the T column from the
+// earlier reflectors.  One inter-workgroup barrier per column; a single
+// workgroup needs none.  Same dlarfg convention as k_panel.
 template <typename T>
-__global__ __launch_bounds__(kThreads) void k_panel(const PanelDesc<T>* __restrict__ descs, int jb) {
-  const PanelDesc<T> d = descs[blockIdx.x];
-  const int m = d.m;
-  const int kk = min(d.m, d.k);
-  if (jb >= kk) return;
-  const int w = min(kNB, kk - jb);
-  const int ncb = min(kNB, d.k - jb);
-  T* __restrict__ P = d.P;
-  const int64_t ld = d.ld;
-  const int tid = threadIdx.x;
-
-  __shared__ T red[kNB * (kThreads / kWave)];
+__global__ __launch_bounds__(kThreads) void k_panel_r(const PanelDesc<T>* __restrict__ descs,
+                                                      const int* __restrict__ wstart, int nd, int jb) {
+  constexpr int NW = kThreads / kWave;
+  __shared__ T red[kNB * NW];
+  __shared__ T Dsh[kNB], rowsh[kNB], wsh[kNB], gsh[kNB];
   __shared__ T Ts[kNB * kNB];
-  __shared__ T gsh[kNB];
-
-  for (int i = tid; i < kNB * kNB; i += kThreads) Ts[i] = T(0);
-
-  for (int j = jb; j < jb + w; ++j) {
-    const int jj = j - jb;
-    T* __restrict__ pj = P + (int64_t)j * ld;
-    // (1) squared norm of the column below the diagonal
-    T s = 0;
-    for (int i = j + 1 + tid; i < m; i += kThreads) { const T v = pj[i]; s += v * v; }
-    s = block_sum(s, red);
-    const T alpha = pj[j];
-    T tau = 0, beta = alpha, scal = 0;
-    if (s > T(0)) {
-      const T xn = sqrt(s);
-      beta = -copysign(hypot(alpha, xn), alpha);
+  const int p = find_owner(wstart, nd, blockIdx.x);
+  const PanelDesc<T> d = descs[p];
+  const int g = blockIdx.x - wstart[p], nwg = d.nwg;
+  const int m = d.m, kk = min(d.m, d.k);
+  if (jb >= kk) return;  // uniform over the problem's workgroups
+  const int w = min(kNB, kk - jb), ncb = min(kNB, d.k - jb);
+  const int tid = threadIdx.x, lane = tid & (kWave - 1), wv = tid / kWave;
+  const int r = jb + g * kThreads + tid;  // this thread's row
+  const bool has = r < m;
+  const int64_t ld = d.ld;
+  T a[kNB];
+#pragma unroll
+  for (int c = 0; c < kNB; ++c) a[c] = (has && c < ncb) ? d.P[r + (int64_t)(jb + c) * ld] : T(0);
+  if (g == 0)
+    for (int i = tid; i < kNB * kNB; i += kThreads) Ts[i] = T(0);
+  unsigned int target = 0;
+  for (int jj = 0; jj < w; ++jj) {
+    const int j = jb + jj;
+    T x = T(0);
+#pragma unroll
+    for (int c = 0; c < kNB; ++c)
+      if (c == jj) x = a[c];
+    const bool below = has && r > j;
+    T dd[kNB];
+#pragma unroll
+    for (int c = 0; c < kNB; ++c) dd[c] = below ? x * a[c] : T(0);
+    const T sw = wave_reduce_scatter32(dd);
+    if (lane < kNB) red[wv * kNB + lane] = sw;
+    if (g == 0 && tid == jj) {
+#pragma unroll
+      for (int c = 0; c < kNB; ++c) rowsh[c] = a[c];
+    }
+    __syncthreads();
+    T sum = T(0);
+    if (tid < kNB) {
+#pragma unroll
+      for (int q = 0; q < NW; ++q) sum += red[q * kNB + tid];
+    }
+    if (nwg > 1) {
+      const int buf = jj & 1;
+      // written by wave 0, the wave that releases the barrier
+      if (tid < kNB) {
+        T* __restrict__ pp = d.part + ((int64_t)buf * nwg + g) * (2 * kNB);
+        pp[tid] = sum;
+        if (g == 0) pp[kNB + tid] = rowsh[tid];
+      }
+      if (nwg > kFanIn) {
+        group_barrier2(d.sync, g, nwg, (unsigned int)(jj + 1), d.err);
+      } else {
+        target += nwg;
+        group_barrier(d.sync, target, d.err);
+      }
+      if (tid < kNB) {
+        T s = T(0);
+        for (int q = 0; q < nwg; ++q) s += d.part[((int64_t)buf * nwg + q) * (2 * kNB) + tid];
+        Dsh[tid] = s;
+        if (g != 0) rowsh[tid] = d.part[(int64_t)buf * nwg * (2 * kNB) + kNB + tid];
+      }
+    } else if (tid < kNB) {
+      Dsh[tid] = sum;
+    }
+    __syncthreads();
+    // reflector scalars (identical in every thread and workgroup)
+    const T ssq = Dsh[jj], alpha = rowsh[jj];
+    T tau = T(0), beta = alpha, scal = T(0);
+    if (ssq > T(0)) {
+      beta = -copysign(hypot(alpha, sqrt(ssq)), alpha);
       tau = (beta - alpha) / beta;
       scal = T(1) / (alpha - beta);
-      for (int i = j + 1 + tid; i < m; i += kThreads) pj[i] *= scal;
     }
-    // row j of the block, read before anyone updates it
-    T rowj[kNB];
-#pragma unroll
-    for (int c = 0; c < kNB; ++c) rowj[c] = (c < ncb && c != jj) ? P[j + (jb + c) * ld] : T(0);
+    if (tid < kNB) {
+      const T vdot = rowsh[tid] + scal * Dsh[tid];
+      wsh[tid] = (tid > jj && tid < ncb) ? tau * vdot : T(0);
+      gsh[tid] = (tid < jj) ? vdot : T(0);
+    }
     __syncthreads();
-    // (2) dots of v_j with every column of the block (rows > j)
-    T acc[kNB];
-#pragma unroll
-    for (int c = 0; c < kNB; ++c) acc[c] = T(0);
-    for (int i = j + 1 + tid; i < m; i += kThreads) {
-      const T vi = pj[i];
-#pragma unroll
-      for (int c = 0; c < kNB; ++c)
-        if (c < ncb && c != jj) acc[c] += vi * P[i + (jb + c) * ld];
+    if (g == 0) {
+      tcol_from_dots(Ts, jj, tau, gsh);
+      if (tid == 0) d.tau[j] = tau;
     }
-    block_sum_n<T, kNB>(acc, red, ncb);
-    if (tid == 0) { pj[j] = beta; d.tau[j] = tau; }
-    // (3) apply H_j to the later columns of the block
-    for (int c = jj + 1; c < ncb; ++c) {
-      const T wc = tau * (rowj[c] + acc[c]);
-      if (wc == T(0)) continue;
-      T* __restrict__ pc = P + (int64_t)(jb + c) * ld;
-      if (tid == 0) pc[j] -= wc;
-      for (int i = j + 1 + tid; i < m; i += kThreads) pc[i] -= wc * pj[i];
+    // apply H_j to the later columns of this row; column j becomes v_j / beta
+    if (has && r >= j) {
+      const T v = (r == j) ? T(1) : ((ssq > T(0)) ? x * scal : x);
+#pragma unroll
+      for (int c = 0; c < kNB; ++c) {
+        if (c > jj) a[c] -= wsh[c] * v;
+        else if (c == jj) a[c] = (r == j) ? beta : v;
+      }
     }
-    // (4) T column from the dots against earlier reflectors
-    if (tid < kNB) gsh[tid] = (tid < jj) ? (rowj[tid] + acc[tid]) : T(0);
-    __syncthreads();
-    tcol_from_dots(Ts, jj, tau, gsh);
     __syncthreads();
   }
-  T* __restrict__ Tb = d.Tb + (int64_t)jb * kNB;
-  for (int i = tid; i < kNB * kNB; i += kThreads) Tb[i] = Ts[i];
+  if (has) {
+#pragma unroll
+    for (int c = 0; c < kNB; ++c)
+      if (c < ncb) d.P[r + (int64_t)(jb + c) * ld] = a[c];
+  }
+  if (g == 0) {
+    T* __restrict__ Tb = d.Tb + (int64_t)jb * kNB;
+    for (int i = tid; i < kNB * kNB; i += kThreads) Tb[i] = Ts[i];
+  }
 }
 
 // Block T factors for an existing reflector set (QP3 output): one workgroup
@@ -176,12 +222,32 @@ __global__ __launch_bounds__(kThreads) void k_trsm_block(const TrsmDesc<T>* __re
 }
 
 int panel_nb() { return kNB; }
+int panel_rows_per_wg() { return kThreads; }
 
 template <typename T>
-void launch_panel(hipStream_t s, const PanelDesc<T>* d, int nd, int jb) {
-  if (nd <= 0) return;
-  hipLaunchKernelGGL((k_panel<T>), dim3(nd), dim3(kThreads), 0, s, d, jb);
+int panel_r_capacity() {
+  int per_cu = 0, dev = 0, cus = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_panel_r<T>, kThreads, 0) != hipSuccess)
+    return 0;
+  return per_cu * cus;
 }
+template <typename T>
+int launch_panel_r(hipStream_t s, const PanelDesc<T>* d, const int* wstart, int nd, int nwg_total, int jb, int coop) {
+  if (nd <= 0) return 0;
+  if (!coop) {
+    hipLaunchKernelGGL((k_panel_r<T>), dim3(nwg_total), dim3(kThreads), 0, s, d, wstart, nd, jb);
+    return (int)hipGetLastError();
+  }
+  void* args[] = {(void*)&d, (void*)&wstart, (void*)&nd, (void*)&jb};
+  return (int)hipLaunchCooperativeKernel((const void*)k_panel_r<T>, dim3(nwg_total), dim3(kThreads), args, 0, s);
+}
+template int panel_r_capacity<double>();
+template int panel_r_capacity<float>();
+template int launch_panel_r<double>(hipStream_t, const PanelDesc<double>*, const int*, int, int, int, int);
+template int launch_panel_r<float>(hipStream_t, const PanelDesc<float>*, const int*, int, int, int, int);
+
 template <typename T>
 void launch_larft(hipStream_t s, const LarftDesc<T>* d, const int* tstart, int nd, int ntiles) {
   if (nd <= 0 || ntiles <= 0) return;
@@ -194,8 +260,6 @@ void launch_trsm_block(hipStream_t s, const TrsmDesc<T>* d, const int* tstart, i
 }
 int trsm_rows_per_tile() { return kThreads; }
 
-template void launch_panel<double>(hipStream_t, const PanelDesc<double>*, int, int);
-template void launch_panel<float>(hipStream_t, const PanelDesc<float>*, int, int);
 template void launch_larft<double>(hipStream_t, const LarftDesc<double>*, const int*, int, int);
 template void launch_larft<float>(hipStream_t, const LarftDesc<float>*, const int*, int, int);
 template void launch_trsm_block<double>(hipStream_t, const TrsmDesc<double>*, const int*, int, int, int);

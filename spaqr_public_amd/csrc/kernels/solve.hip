@@ -24,31 +24,6 @@ constexpr int kSWaves = kSThreads / kWave;
 constexpr int kRNB = 32;      // reflector / triangular block width (matches the panel kernel)
 constexpr int kTLd = kRNB + 1;  // padded LDS leading dimension of a staged 32x32 block
 
-// Butterfly reduce-scatter of 32 per-lane partial sums over a wavefront: each
-// step exchanges half of the live values with lane ^ H, so 31 shuffles leave
-// lane l with the sum over its 32-lane half of a[l & 31]; one more folds the
-// two halves.  All indices are compile-time, so a[] stays in registers.
-template <typename T, int H>
-__device__ __forceinline__ void rs_step(T (&a)[kRNB], int lane) {
-  const bool up = (lane & H) != 0;
-#pragma unroll
-  for (int j = 0; j < H; ++j) {
-    const T keep = up ? a[j + H] : a[j];
-    const T send = up ? a[j] : a[j + H];
-    a[j] = keep + __shfl_xor(send, H, kWave);
-  }
-}
-template <typename T>
-__device__ __forceinline__ T wave_reduce_scatter32(T (&a)[kRNB]) {
-  const int lane = threadIdx.x & (kWave - 1);
-  rs_step<T, 16>(a, lane);
-  rs_step<T, 8>(a, lane);
-  rs_step<T, 4>(a, lane);
-  rs_step<T, 2>(a, lane);
-  rs_step<T, 1>(a, lane);
-  return a[0] + __shfl_xor(a[0], 32, kWave);
-}
-
 // Entry (i, col) of a unit-lower reflector block; the load is in bounds for
 // every i < m, col < k (the strict upper part holds R or is unused).
 template <typename T>

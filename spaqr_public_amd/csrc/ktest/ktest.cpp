@@ -61,12 +61,21 @@ int ktest_qr(int m, int k, double* A, double* tau, double* Tb, int nc, double* C
   double *dA = dup(A, (size_t)m * k), *dt = dup<double>(nullptr, k), *dT = dup<double>(nullptr, (size_t)nb * nb * nblk);
   double* dC = dup(Cm, (size_t)m * (nc > 0 ? nc : 1));
   double *dY = dup<double>(nullptr, (size_t)nb * (k + nc + 1)), *dZ = dup<double>(nullptr, (size_t)nb * (k + nc + 1));
-  PanelDesc<double> pd{dA, dt, dT, m, m, k, 0};
-  PanelDesc<double>* dpd = dup(&pd, 1);
   int zero = 0;
   int* ds = dup(&zero, 1);
+  int* err = dup<int>(nullptr, 1);
   for (int jb = 0; jb < kk; jb += nb) {
-    launch_panel<double>(0, dpd, 1, jb);
+    // register-form panel step over ceil((m - jb) / rows-per-workgroup) workgroups
+    const int nwg = (m - jb + panel_rows_per_wg() - 1) / panel_rows_per_wg();
+    if (nwg > panel_r_capacity<double>()) { g_err = "nwg exceeds resident capacity"; return -1; }
+    double* part = dup<double>(nullptr, 2 * (size_t)nwg * 2 * nb);
+    unsigned int* sync = dup<unsigned int>(nullptr, barrier_words(nwg));
+    PanelDesc<double> pd{dA, dt, dT, m, m, k, nwg, part, sync, err};
+    PanelDesc<double>* dpd = dup(&pd, 1);
+    const int rc = launch_panel_r<double>(0, dpd, ds, 1, nwg, jb, nwg > 1 ? 1 : 0);
+    if (rc != 0) { g_err = hipGetErrorString((hipError_t)rc); return -1; }
+    KT(hipDeviceSynchronize());
+    for (void* q : {(void*)part, (void*)sync, (void*)dpd}) (void)hipFree(q);
     const int w = (nb < kk - jb) ? nb : kk - jb, rows = m - jb;
     const double* Vb = dA + jb + (size_t)jb * m;
     const double* Tbb = dT + (size_t)jb * nb;
@@ -92,7 +101,10 @@ int ktest_qr(int m, int k, double* A, double* tau, double* Tb, int nc, double* C
   back(tau, dt, kk);
   back(Tb, dT, (size_t)nb * nb * nblk);
   if (nc > 0) back(Cm, dC, (size_t)m * nc);
-  for (void* p : {(void*)dA, (void*)dt, (void*)dT, (void*)dC, (void*)dY, (void*)dZ, (void*)dpd, (void*)ds}) (void)hipFree(p);
+  int e = 0;
+  back(&e, err, 1);
+  for (void* p : {(void*)dA, (void*)dt, (void*)dT, (void*)dC, (void*)dY, (void*)dZ, (void*)ds, (void*)err}) (void)hipFree(p);
+  if (e) { g_err = "barrier wait timed out"; return -1; }
   return 0;
 }
 

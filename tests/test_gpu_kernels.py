@@ -101,7 +101,10 @@ def test_gemm(K, mnk, ta, tb, sa, sb):
     assert rel(Cm[:m, :n], ref) < 1e-13
 
 
-@pytest.mark.parametrize("mk", [(1, 1), (5, 3), (40, 40), (97, 33), (150, 64), (300, 100), (64, 80), (513, 65)])
+# panel steps run on ceil((m - jb) / 256) workgroups: (513, 65), (2000, 70)
+# use the flat inter-workgroup barrier, (5000, 40) the two-level one
+@pytest.mark.parametrize("mk", [(1, 1), (5, 3), (40, 40), (97, 33), (150, 64), (300, 100), (64, 80), (513, 65),
+                                (2000, 70), (5000, 40)])
 def test_blocked_qr(K, oracle, mk):
     m, k = mk
     rng = np.random.default_rng(m + 1000 * k)
