@@ -23,6 +23,27 @@ sys.path.insert(0, ROOT)
 
 PEAK = {"fp64_mfma_tflops": 78.6, "fp32_mfma_tflops": 157.3, "hbm_gbs": 8000.0}
 
+# kernel-timing family -> kernel symbol (as rocprofv3 names it), {T} = double/float
+FAMILY_KERNEL = {
+    "qp3r1024": "spq::k_qp3r<{T}, 1024>", "qp3r256": "spq::k_qp3r<{T}, 256>", "qp3s": "spq::k_qp3<{T}>",
+    "panel": "spq::k_panel_r<{T}>", "gemm": "spq::k_gemm<{T}>", "larft": "spq::k_larft<{T}>",
+    "trsm": "spq::k_trsm_block<{T}>", "copy": "spq::k_copy", "init": "spq::k_init_blocks",
+    "apply": "solve program (hipGraph of k_refl/k_bwd/k_move)",
+}
+
+
+def traffic_for(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary
+    (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration), or None."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            rec = json.load(f)
+    except (OSError, ValueError):
+        return None
+    k = rec.get("kernels", {}).get(kernel)
+    return None if k is None else k.get("hbm_bytes_per_launch")
+
 
 def cpu_info():
     model = "unknown"
@@ -165,21 +186,25 @@ def main():
     fac = float(np.mean(fac_t))
     kry = float(np.mean(kry_t))
     # dominant kernel family over the timed steps
+    tname = "double" if precision == sq.FP64 else "float"
     dom = max(stats_acc.items(), key=lambda kv: kv[1]["ms"]) if stats_acc else None
     roofline = None
     if dom:
         k, v = dom
+        kernel = FAMILY_KERNEL.get(k, k).format(T=tname)
         avg_ms = v["ms"] / max(v["launches"], 1)
         if v["flops"] > 0:
             ach = v["flops"] / (v["ms"] * 1e-3) / 1e12
             peak = PEAK["fp64_mfma_tflops"] if precision == sq.FP64 else PEAK["fp32_mfma_tflops"]
-            roofline = dict(kernel=k, bound="mfma", achieved=ach, peak=peak, unit="TFLOP/s", frac=ach / peak,
-                            traffic=None, avg_launch_ms=avg_ms, launches=v["launches"])
+            roofline = dict(bound="mfma", achieved=ach, peak=peak, unit="TFLOP/s", frac=ach / peak,
+                            traffic=traffic_for(kernel), kernel=kernel, family=k, avg_launch_ms=avg_ms,
+                            launches=v["launches"], flops_per_launch=v["flops"] / max(v["launches"], 1))
         else:
             ach = v["bytes"] / (v["ms"] * 1e-3) / 1e9
-            roofline = dict(kernel=k, bound="hbm", achieved=ach, peak=PEAK["hbm_gbs"], unit="GB/s",
-                            frac=ach / PEAK["hbm_gbs"], traffic=None, avg_launch_ms=avg_ms,
-                            launches=v["launches"])
+            roofline = dict(bound="hbm", achieved=ach, peak=PEAK["hbm_gbs"], unit="GB/s",
+                            frac=ach / PEAK["hbm_gbs"], traffic=traffic_for(kernel), kernel=kernel, family=k,
+                            avg_launch_ms=avg_ms, launches=v["launches"],
+                            bytes_per_launch=v["bytes"] / max(v["launches"], 1))
     cpu = None
     if not args.no_cpu_baseline:
         try:

@@ -38,9 +38,11 @@ class Engine final : public EngineBase {
     if (const char* e = std::getenv("SPAQR_QP3_REGISTER")) qp3_reg_ = std::atoi(e) != 0;
     SPQ_HIP(hipDeviceGetAttribute(&num_cus_, hipDeviceAttributeMultiprocessorCount, device));
     if (const char* e = std::getenv("SPAQR_SOLVE_STAGES")) stage_prof_ = std::atoi(e) != 0;
+    if (const char* e = std::getenv("SPAQR_SOLVE_GRAPH")) solve_graph_ = std::atoi(e) != 0;
   }
   ~Engine() override {
-    reset_kernel_stats();
+    (void)hipStreamSynchronize(stream_);
+    for (auto e : ev_pool_) (void)hipEventDestroy(e);
     if (graph_exec_fwd_) hipGraphExecDestroy(graph_exec_fwd_);
     if (rank_host_) hipHostFree(rank_host_);
     if (xbuf_) hipFree(xbuf_);
@@ -76,9 +78,9 @@ class Engine final : public EngineBase {
     if (e) throw Error(kErrNumeric, "solve: an inter-workgroup barrier wait timed out");
   }
   void reset_kernel_stats() override {
-    for (auto& kv : kev_)
-      for (auto& e : kv.second.ev) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
+    SPQ_HIP(hipStreamSynchronize(stream_));
     kev_.clear();
+    ev_used_ = 0;  // the pool's events are recorded again
   }
   std::vector<KernelStat> kernel_stats() override {
     SPQ_HIP(hipStreamSynchronize(stream_));
@@ -230,9 +232,7 @@ class Engine final : public EngineBase {
   template <typename F>
   void timed(const char* name, double flops, double bytes, F&& launch) {
     if (!kprof_) { launch(); return; }
-    hipEvent_t a, b;
-    SPQ_HIP(hipEventCreate(&a));
-    SPQ_HIP(hipEventCreate(&b));
+    hipEvent_t a = pooled_event(), b = pooled_event();
     SPQ_HIP(hipEventRecord(a, stream_));
     launch();
     SPQ_HIP(hipEventRecord(b, stream_));
@@ -437,6 +437,17 @@ class Engine final : public EngineBase {
   bool factored_ = false;
   bool kprof_ = false;
   struct KEv { std::vector<std::pair<hipEvent_t, hipEvent_t>> ev; double flops = 0, bytes = 0; };
+  std::map<const int*, const char*> qp3_family_;  // problem -> timing family of its launch
+  std::vector<hipEvent_t> ev_pool_;               // reused by kernel timing
+  size_t ev_used_ = 0;
+  hipEvent_t pooled_event() {
+    if (ev_used_ == ev_pool_.size()) {
+      hipEvent_t e;
+      SPQ_HIP(hipEventCreate(&e));
+      ev_pool_.push_back(e);
+    }
+    return ev_pool_[ev_used_++];
+  }
   std::map<std::string, KEv> kev_;
 
   // solve program (device descriptor tables, built once after factorize)
@@ -466,6 +477,7 @@ class Engine final : public EngineBase {
   double* xout_ = nullptr;
   void run_program(const double* b, double* x, bool prof = false);
   bool stage_prof_ = false;  // SPAQR_SOLVE_STAGES=1: time every solve stage (uncaptured)
+  bool solve_graph_ = true;  // SPAQR_SOLVE_GRAPH=0: launch the solve program directly (no hipGraph)
   bool qp3_reg_ = true;      // SPAQR_QP3_REGISTER=0: streaming truncated QR only
   int panel_cap_ = -1;
   int panel_r_cap() {
@@ -968,17 +980,20 @@ void Engine<T>::sparsify_level(const std::vector<int>& cs) {
       }
       for (auto& gq : groups) {
         std::vector<int> cnt, st;
-        double qfl = 0, qby = 0;
+        double qby = 0;
         for (auto& d : gq.d) {
           cnt.push_back(d.nwg);
-          qfl += 4.0 * d.k * d.N * std::min(d.k, d.N);  // upper bound: full rank steps
-          qby += 2.0 * sizeof(T) * (double)d.k * d.N;
+          qby += 2.0 * sizeof(T) * (double)d.k * d.N;  // C read once, written once
         }
         const int nt = prefix_tiles(cnt, st);
         const Qp3Desc<T>* dqd = stager_.push(gq.d);
         const int* dst = stager_.push(st);
         int rc = 0;
-        timed("qp3", qfl, qby, [&] {
+        // one timing family per kernel: streaming, register 256 / 1024 threads;
+        // flops are added once the ranks are known
+        const char* fam = gq.nt == 0 ? "qp3s" : (gq.nt == 1024 ? "qp3r1024" : "qp3r256");
+        for (auto& d : gq.d) qp3_family_[d.rank_out] = fam;
+        timed(fam, 0.0, qby, [&] {
           rc = gq.nt ? launch_qp3r<T>(stream_, dqd, dst, (int)gq.d.size(), nt, gq.gmax, gq.nt, gq.coop)
                      : launch_qp3<T>(stream_, dqd, dst, (int)gq.d.size(), nt, kmax, gq.coop);
         });
@@ -989,6 +1004,16 @@ void Engine<T>::sparsify_level(const std::vector<int>& cs) {
       sync();
       if (rank_host_[65535] != 0) throw Error(kErrNumeric, "qp3: an inter-workgroup barrier wait timed out");
       for (size_t q = 0; q < jobs.size(); ++q) ranks[q] = rank_host_[q];
+      if (kprof_) {
+        // truncated RRQR: 4kNr - 2(k+N)r^2 + 4r^3/3 + 2kN (initial norms)
+        for (size_t q = 0; q < jobs.size(); ++q) {
+          const double k = jobs[q].k, N = jobs[q].N, r = ranks[q] < jobs[q].k ? ranks[q] + 1 : ranks[q];
+          auto it = qp3_family_.find(qd[q].rank_out);
+          if (it != qp3_family_.end())
+            kev_[it->second].flops += 4 * k * N * r - 2 * (k + N) * r * r + 4.0 / 3.0 * r * r * r + 2 * k * N;
+        }
+      }
+      qp3_family_.clear();
     }
     if (verbose_ > 1) {
       int nmax = 0, nwgs = 0, rsum = 0, ksum = 0;
@@ -1488,8 +1513,8 @@ void Engine<T>::solve_dev(const double* b, double* x) {
   // b is taken by value (tree.cpp:1440): copy into the program's fixed input,
   // replay the captured apply, copy the fixed output out.
   SPQ_HIP(hipMemcpyAsync(bin_, b, sizeof(double) * nrows_, hipMemcpyDeviceToDevice, stream_));
-  if (stage_prof_) {
-    run_program(bin_, xout_, true);
+  if (stage_prof_ || !solve_graph_) {
+    timed("apply", 0.0, solve_bytes_, [&] { run_program(bin_, xout_, stage_prof_); });
     SPQ_HIP(hipMemcpyAsync(x, xout_, sizeof(double) * ncols_, hipMemcpyDeviceToDevice, stream_));
     return;
   }
