@@ -115,6 +115,80 @@ def cpu_baseline(workload_flops, gpu_iters, threads, budget_s=40.0):
     }
 
 
+def _csv_rows(d, suffix):
+    import csv
+    import glob
+    files = sorted(glob.glob(os.path.join(d, "*" + suffix)))
+    if not files:
+        raise SystemExit(f"no *{suffix} under {d}")
+    with open(files[0]) as f:
+        return list(csv.DictReader(f))
+
+
+def trace_summary(d, out, cmd, top=14):
+    """Markdown table of a rocprofv3 --stats kernel summary."""
+    rows = _csv_rows(d, "kernel_stats.csv")
+    tot = sum(float(r["TotalDurationNs"]) for r in rows)
+    lines = [f"# Kernel time: {os.path.basename(out).rsplit('.', 1)[0]}", "",
+             "Command (from the repository root, on one MI355X):", "", "```", cmd, "```", "",
+             f"Total kernel time {tot / 1e6:.1f} ms over {sum(int(r['Calls']) for r in rows)} dispatches.", "",
+             "| kernel | calls | total ms | avg us | % |", "|---|---|---|---|---|"]
+    for r in rows[:top]:
+        name = r["Name"].split("(")[0].replace("void ", "")
+        lines.append(f"| `{name}` | {r['Calls']} | {float(r['TotalDurationNs']) / 1e6:.1f} | "
+                     f"{float(r['AverageNs']) / 1e3:.1f} | {float(r['Percentage']):.1f} |")
+    with open(out, "w") as f:
+        f.write("\n".join(lines) + "\n")
+
+
+def pmc_summary(fetch_dir, write_dir, cal_fetch_dir, cal_write_dir, out, cmds, copy_bytes=4096 * 4096 * 8):
+    """Per-kernel HBM bytes per dispatch from FETCH_SIZE / WRITE_SIZE (KiB),
+    each scaled so that a plain copy of copy_bytes reads and writes copy_bytes."""
+    import collections
+
+    def per_kernel(d):
+        agg = collections.defaultdict(lambda: [0, 0.0])
+        for r in _csv_rows(d, "counter_collection.csv"):
+            a = agg[r["Kernel_Name"].split("(")[0].replace("void ", "")]
+            a[0] += 1
+            a[1] += float(r["Counter_Value"])
+        return {k: (n, v / n) for k, (n, v) in agg.items()}
+
+    def copy_kib(d):
+        return max(v for k, (n, v) in per_kernel(d).items() if "k_copy" in k)
+
+    sf = copy_bytes / 1024.0 / copy_kib(cal_fetch_dir)
+    sw = copy_bytes / 1024.0 / copy_kib(cal_write_dir)
+    fe, wr = per_kernel(fetch_dir), per_kernel(write_dir)
+    kernels = {}
+    for k in sorted(set(fe) & set(wr)):
+        if not k.startswith("spq::"):
+            continue
+        kernels[k] = dict(dispatches=fe[k][0], fetch_kib=fe[k][1], write_kib=wr[k][1],
+                          hbm_bytes_per_launch=1024.0 * (fe[k][1] * sf + wr[k][1] * sw))
+    rec = dict(commands=cmds, copy_calibration=dict(bytes=copy_bytes, fetch_scale=sf, write_scale=sw),
+               kernels=kernels)
+    with open(out, "w") as f:
+        json.dump(rec, f, indent=1)
+
+
+def copy_calibration():
+    """One 4096 x 4096 FP64 block copy through the library's copy kernel (the
+    known-size reference for the PMC byte counters)."""
+    import ctypes as C
+    import numpy as np
+    lib = C.CDLL(os.path.join(ROOT, "spaqr_public_amd", "_lib", "libspaqr_ktest.so"))
+    dp = C.POINTER(C.c_double)
+    lib.ktest_copy.argtypes = [C.c_int, C.c_int, dp, C.c_int, dp, C.c_int, C.c_int]
+    n = 4096
+    src = np.asfortranarray(np.random.default_rng(0).standard_normal((n, n)))
+    dst = np.zeros((n, n), order="F")
+    rc = lib.ktest_copy(n, n, src.ctypes.data_as(dp), n, dst.ctypes.data_as(dp), n, 0)
+    if rc != 0 or not np.array_equal(src, dst):
+        raise SystemExit("copy calibration failed")
+    print(f"copy calibration: {n * n * 8} bytes read and written", flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -124,7 +198,21 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-2)
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    # measurement bookkeeping (no benchmark run)
+    ap.add_argument("--copy-calibration", action="store_true", help="run the PMC copy calibration and exit")
+    ap.add_argument("--trace-summary", nargs=2, metavar=("STATS_DIR", "OUT_MD"))
+    ap.add_argument("--pmc-summary", nargs=5, metavar=("FETCH", "WRITE", "CAL_FETCH", "CAL_WRITE", "OUT_JSON"))
+    ap.add_argument("--cmd", action="append", default=[], help="profiling command(s) recorded in the summary")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
+    if args.trace_summary:
+        trace_summary(args.trace_summary[0], args.trace_summary[1], " && ".join(args.cmd))
+        return
+    if args.pmc_summary:
+        pmc_summary(*args.pmc_summary[:4], args.pmc_summary[4], args.cmd)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build measures one MI355X; --gpus {args.gpus} is not supported "
               "(the multi-GPU sharding is a design note only)", file=sys.stderr)
