@@ -152,7 +152,9 @@ class Engine final : public EngineBase {
     int level() const { return id.self.lvl; }
   };
   // Solve op records (sizes captured at creation, operations.h / toperations.h)
-  enum OpKind { OpQR, OpScaleD, OpOrthoD, OpSplitD, OpMerge };
+  // OpFineQR: the QR op of an order-1.5 fine cluster (tree.cpp:1252-1253):
+  // tau = 0, so only its backward R12 / R sweep does work
+  enum OpKind { OpQR, OpScaleD, OpOrthoD, OpSplitD, OpMerge, OpFineQR };
   struct Op {
     OpKind kind;
     int level, c = -1, f = -1;
@@ -167,6 +169,7 @@ class Engine final : public EngineBase {
     std::vector<std::pair<Block, std::pair<int, int>>> r12;  // block, (cluster, s)
     // SplitD
     int crows = 0, ccols = 0, rank = 0;
+    int wave = 0;  // sparsify wave (OrthoD, SplitD, FineQR)
   };
 
   // ------------------------------------------------------------ helpers
@@ -865,9 +868,11 @@ void Engine<T>::sparsify_level(const std::vector<int>& cs) {
   std::vector<SparsifyInfo> recs(S.size());
   for (int w = 0; w < nw; ++w) {
     scratch_.rewind();
+    const bool imp = order_ == 1.5f;  // sparsifyD_imp (tree.cpp:1117-1257)
     struct Job {
-      int c, k, N, smp;
+      int c, k, N, smp, spm = 0;
       std::vector<int> amp, apm;
+      T* W = nullptr;  // order 1.5: top k rows of the in-edge blocks, k x spm (ld k)
       T *Ct, *V, *tau;
       double* vn;
       int* pos;
@@ -924,6 +929,17 @@ void Engine<T>::sparsify_level(const std::vector<int>& cs) {
         const Block& b = ed_[e].a;
         gather.push_back(CopyDesc{b.p, jb.Ct + off, b.ld, N, jb.k, b.cols, 1, 0});
         off += b.cols;
+      }
+      jb.spm = spm;
+      if (imp && spm > 0) {
+        // Apmc_top (tree.cpp:1130-1140), kept before the blocks are rewritten
+        jb.W = alloc_scr((size_t)jb.k * spm);
+        int q = 0;
+        for (int e : jb.apm) {
+          const Block& b = ed_[e].a;
+          gather.push_back(CopyDesc{b.p, jb.W + (int64_t)q * jb.k, b.ld, jb.k, jb.k, b.cols, 0, 0});
+          q += b.cols;
+        }
       }
       Qp3Desc<T> d{};
       d.Ct = jb.Ct; d.V = jb.V; d.tau = jb.tau; d.vn = jb.vn; d.pos = jb.pos;
@@ -1028,9 +1044,11 @@ void Engine<T>::sparsify_level(const std::vector<int>& cs) {
       fprintf(stderr, "sprs lvl %d wave %d: problems %zu kmax %d Nmax %d sum_kN %.3g workgroups %d ranks %d/%d\n", ilvl_,
               w, jobs.size(), kmax, nmax, kn, nwgs, rsum, ksum);
     }
-    std::vector<CopyDesc> scatter;
+    std::vector<CopyDesc> scatter, afm_copies;
     std::vector<LarftDesc<T>> lf;
     std::vector<int> lcnt;
+    struct QtJob { const T* V; const T* Tb; T* W; int k, rank, spm; };
+    std::vector<QtJob> qt;  // order 1.5: Apmc_top <- Q^T Apmc_top (ormqr_left, tree.cpp:1234)
     for (size_t q = 0; q < jobs.size(); ++q) {
       Job& jb = jobs[q];
       const int c = jb.c, k = jb.k, rank = ranks[q];
@@ -1053,7 +1071,7 @@ void Engine<T>::sparsify_level(const std::vector<int>& cs) {
       }
       nnzQ_ += (unsigned long long)(k * rank);
       Op od;
-      od.kind = OpOrthoD; od.level = ilvl_; od.c = c;
+      od.kind = OpOrthoD; od.level = ilvl_; od.c = c; od.wave = w;
       od.V = V; od.ldv = k; od.m = k; od.k = rank; od.Tb = Tb;
       od.segs.push_back({c, k});
       ops_.push_back(std::move(od));
@@ -1063,7 +1081,7 @@ void Engine<T>::sparsify_level(const std::vector<int>& cs) {
       alloc_x(cf, cfn);
       cl_[cf].elim = true;
       Op sp;
-      sp.kind = OpSplitD; sp.level = ilvl_; sp.c = c; sp.f = cf;
+      sp.kind = OpSplitD; sp.level = ilvl_; sp.c = c; sp.f = cf; sp.wave = w;
       sp.crows = cl_[c].rsize; sp.ccols = k; sp.rank = rank;
       ops_.push_back(std::move(sp));
       // new blocks (tree.cpp:1082-1110)
@@ -1086,6 +1104,32 @@ void Engine<T>::sparsify_level(const std::vector<int>& cs) {
         off += b.cols;
       }
       const int es = find_out(c, c);
+      if (imp) {
+        // fine cluster of the order-1.5 scheme (tree.cpp:1196-1254): self block
+        // Aff = top-left cf x cf of the self block, in-edges E2 = rows
+        // [rank, k) of Q^T Apmc_top, and a QR op with tau = 0
+        const Block self = ed_[es].a;
+        const int eff = add_edge(cf, cf, nullptr);
+        scatter.push_back(CopyDesc{self.p, ed_[eff].a.p, self.ld, ed_[eff].a.ld, cfn, cfn, 0, 0});
+        Op fq;
+        fq.kind = OpFineQR; fq.level = ilvl_; fq.c = cf; fq.wave = w;
+        fq.R = ed_[eff].a.p; fq.ldr = ed_[eff].a.ld; fq.kr = cfn;
+        int qc = 0;
+        for (int e : jb.apm) {
+          const int m = ed_[e].n1, mc = cl_[m].csize;
+          const int ne = add_edge(m, cf, nullptr);
+          const Block& nb2 = ed_[ne].a;
+          if (mc > 0)
+            afm_copies.push_back(CopyDesc{jb.W + rank + (int64_t)qc * k, nb2.p, k, nb2.ld, cfn, mc, 0, 0});
+          fq.r12.push_back({nb2, {m, mc}});
+          qc += mc;
+        }
+        ops_.push_back(std::move(fq));
+        if (rank > 0 && jb.spm > 0) qt.push_back(QtJob{V, Tb, jb.W, k, rank, jb.spm});
+        else if (jb.spm > 0 && rank == 0) {
+          // Q = I: the in-edges are the top rows unchanged (copies above)
+        }
+      }
       ed_[es].a.rows = rank_rows;
       ed_[es].a.cols = rank;
       recs[slot[c]] = {ilvl_, cl_[c].id.self.lvl, cl_[c].id.self.sep, before, rank};
@@ -1098,6 +1142,30 @@ void Engine<T>::sparsify_level(const std::vector<int>& cs) {
       const int* dst = stager_.push(st);
       timed("larft", 0.0, 0.0, [&] { launch_larft<T>(stream_, dl, dst, (int)lf.size(), nt); });
     }
+    if (!qt.empty()) {
+      // W <- H^T W block by block: Y = V_b^T W, Z = T_b^T Y, W -= V_b Z
+      int rmax = 0;
+      for (auto& j : qt) rmax = std::max(rmax, j.rank);
+      for (int jb = 0; jb < rmax; jb += nb_) {
+        ws_.rewind();
+        std::vector<GemmDesc<T>> g1, g2, g3;
+        for (auto& j : qt) {
+          if (jb >= j.rank) continue;
+          const int wdt = std::min(nb_, j.rank - jb), rows = j.k - jb;
+          const T* Vb = j.V + jb + (int64_t)jb * j.k;
+          const T* Tbb = j.Tb + (int64_t)jb * nb_;
+          T* Y = (T*)ws_.alloc(sizeof(T) * (size_t)nb_ * j.spm);
+          T* Z = (T*)ws_.alloc(sizeof(T) * (size_t)nb_ * j.spm);
+          g1.push_back(gd(Vb, j.k, j.W + jb, j.k, Y, nb_, wdt, j.spm, rows, gemm_flags(true, false, kStructUnitLower), T(1), T(0)));
+          g2.push_back(gd(Tbb, nb_, Y, nb_, Z, nb_, wdt, j.spm, wdt, gemm_flags(true, false, kStructUpper), T(1), T(0)));
+          g3.push_back(gd(Vb, j.k, Z, nb_, j.W + jb, j.k, rows, j.spm, wdt, gemm_flags(false, false, kStructUnitLower), T(-1), T(1)));
+        }
+        gemm_batch(g1);
+        gemm_batch(g2);
+        gemm_batch(g3);
+      }
+    }
+    copy_blocks(afm_copies);
   }
   sprs_.insert(sprs_.end(), recs.begin(), recs.end());
 }
@@ -1167,7 +1235,6 @@ int Engine<T>::factorize() {  // tree.cpp:1309-1437
     const double t4 = wall();
     if (ilvl_ >= skip_ && ilvl_ < nlevels_ - 2 && tol_ != 0) {
       if (!scale_) throw Error(kErrState, "--scale 0 sparsification is not available in this build");
-      if (order_ == 1.5f) throw Error(kErrState, "--order 1.5 sparsification is not available in this build");
       sparsify_level(itf);
     }
     if (timing_) sync();
@@ -1254,7 +1321,8 @@ void Engine<T>::build_solve_program() {
       else if (o.kind == OpScaleD) sc.push_back(&o);
       else if (o.kind == OpOrthoD) od.push_back(&o);
       else if (o.kind == OpSplitD) sd.push_back(&o);
-      else mg.push_back(&o);
+      else if (o.kind == OpMerge) mg.push_back(&o);
+      // OpFineQR: tau = 0, no forward work
     }
     // QR forward waves: ops sharing an x segment keep their order
     std::vector<int> wv(qr.size(), 0);
@@ -1300,14 +1368,15 @@ void Engine<T>::build_solve_program() {
     if (!smg.mv.empty()) hf.push_back(std::move(smg));
   }
   for (int lvl = nlevels_ - 1; lvl >= 0; --lvl) {
-    std::vector<const Op*> qr, sc, od, sd, mg;
+    std::vector<const Op*> qr, sc, od, sd, mg, fq;
     for (auto& o : ops_) {
       if (o.level != lvl) continue;
       if (o.kind == OpQR) qr.push_back(&o);
       else if (o.kind == OpScaleD) sc.push_back(&o);
       else if (o.kind == OpOrthoD) od.push_back(&o);
       else if (o.kind == OpSplitD) sd.push_back(&o);
-      else mg.push_back(&o);
+      else if (o.kind == OpMerge) mg.push_back(&o);
+      else fq.push_back(&o);
     }
     HStage smg; smg.type = 2;
     for (auto o : mg) {
@@ -1318,15 +1387,31 @@ void Engine<T>::build_solve_program() {
       }
     }
     if (!smg.mv.empty()) hb.push_back(std::move(smg));
-    HStage ssd; ssd.type = 2;
-    for (auto o : sd) {
-      const int n = o->ccols - o->rank;
-      if (n > 0) ssd.mv.push_back(MoveDesc{cl_[o->f].xoff, cl_[o->c].xoff + o->rank, n, 0});
+    // sparsification ops in reverse: grouped by kind, or, when order-1.5
+    // fine-cluster QRs read neighbouring interfaces, wave by wave (latest
+    // wave first: FineQR, SplitD, OrthoD), which keeps the tape's order
+    // between adjacent interfaces
+    int wmax = 0;
+    for (auto o : sd) wmax = std::max(wmax, o->wave);
+    for (int w = fq.empty() ? 0 : wmax; w >= 0; --w) {
+      const bool all = fq.empty();
+      HStage sfq; sfq.type = 1;
+      for (auto o : fq) if (o->wave == w) sfq.bwd.push_back(bwd_of(*o));
+      if (!sfq.bwd.empty()) hb.push_back(std::move(sfq));
+      HStage ssd; ssd.type = 2;
+      for (auto o : sd) {
+        if (!all && o->wave != w) continue;
+        const int n = o->ccols - o->rank;
+        if (n > 0) ssd.mv.push_back(MoveDesc{cl_[o->f].xoff, cl_[o->c].xoff + o->rank, n, 0});
+      }
+      if (!ssd.mv.empty()) hb.push_back(std::move(ssd));
+      HStage sod; sod.type = 0;
+      for (auto o : od) {
+        if (!all && o->wave != w) continue;
+        if (o->k > 0) { sod.refl.push_back(refl_of(*o, 0)); sod.mmax = std::max(sod.mmax, o->m); }
+      }
+      if (!sod.refl.empty()) hb.push_back(std::move(sod));
     }
-    if (!ssd.mv.empty()) hb.push_back(std::move(ssd));
-    HStage sod; sod.type = 0;
-    for (auto o : od) if (o->k > 0) { sod.refl.push_back(refl_of(*o, 0)); sod.mmax = std::max(sod.mmax, o->m); }
-    if (!sod.refl.empty()) hb.push_back(std::move(sod));
     HStage ssc; ssc.type = 1;
     for (auto o : sc) ssc.bwd.push_back(BwdDesc<T>{o->R, o->ldr, o->k, cl_[o->c].xoff, 0, 0, 0, 0});
     if (!ssc.bwd.empty()) hb.push_back(std::move(ssc));

@@ -91,3 +91,26 @@ def test_solve_device_pointer_alias(oracle, gpu_lib):
     t.solve_device(buf.ptr, buf.ptr)
     out = buf.to_host()
     assert np.allclose(out, x_host, rtol=0, atol=1e-13 * np.abs(x_host).max())
+
+
+@pytest.mark.parametrize("name,n,dim,tol", [
+    ("advdiff2d_128_central", 128, 2, 1e-3),
+    ("advdiff3d_16", 16, 3, 1e-2),
+    ("advdiff3d_32", 32, 3, 1e-2),
+])
+def test_order_1_5(oracle, gpu_lib, name, n, dim, tol):
+    """--order 1.5 (sparsifyD_imp, tree.cpp:1117-1257): fine clusters keep the
+    coupling E2 = (Q^T Apm_top)[rank:, :] and a tau = 0 QR op."""
+    from spaqr_public_amd import gen
+    A = gen.config_matrix(name)
+    gpu, ref = _run_both(oracle, A, n, dim, tol, order=1.5)
+    _check_tree(gpu, ref)
+    assert gpu["oneshot_res"] <= 2.0 * ref["oneshot_res"]
+    assert abs(gpu["iters"] - ref["iters"]) <= 1
+    assert gpu["final_res"] <= 1e-12
+    sg, so = gpu["tree"].sparsified(), ref["tree"].sparsified()
+    assert sg.shape == so.shape
+    assert np.mean(sg[:, 4] == so[:, 4]) >= 0.95
+    if name == "advdiff2d_128_central":
+        assert np.array_equal(sg, so)
+        assert gpu["oneshot_res"] < 0.8 * 1.338e-3  # below the order-1 README value

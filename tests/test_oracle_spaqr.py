@@ -127,3 +127,24 @@ def test_colscale(oracle):
     norms = np.sqrt(np.asarray(As.multiply(As).sum(axis=0))).ravel()
     assert np.allclose(norms, 10.0, rtol=1e-14)
     assert np.allclose((A @ sp.diags(D) * 10).toarray(), As.toarray(), rtol=1e-15)
+
+
+@pytest.mark.parametrize("name,n,dim,tol", [("advdiff2d_128_central", 128, 2, 1e-3), ("advdiff3d_12", 12, 3, 1e-2)])
+def test_order_1_5_keeps_coupling(oracle, name, n, dim, tol):
+    """--order 1.5 (sparsifyD_imp, tree.cpp:1117-1257) runs the same RRQR and
+    makes the same rank decisions as order 1 (identical sparsified records,
+    top separator and nnzQ); it only keeps the coupling E2 = (Q^T Apm_top)
+    [rank:, :] that order 1 drops, so its one-shot residual is smaller.  A
+    restatement that dropped E2, or applied Q instead of Q^T to Apm_top,
+    would lose the improvement."""
+    from spaqr_public_amd import gen
+    oracle.set_backend("own")
+    A = gen.config_matrix(name)
+    r1 = oracle.run_reference_flow(A, n, dim, tol, order=1.0)
+    r15 = oracle.run_reference_flow(A, n, dim, tol, order=1.5)
+    assert np.array_equal(r1["tree"].sparsified(), r15["tree"].sparsified())
+    c1, c15 = r1["tree"].counters(), r15["tree"].counters()
+    assert c1["top_size"] == c15["top_size"] and c1["nnzQ"] == c15["nnzQ"]
+    assert r15["oneshot_res"] < 0.8 * r1["oneshot_res"]
+    assert r15["iters"] <= r1["iters"]
+    assert r15["final_res"] <= 1e-12
