@@ -76,8 +76,12 @@ int spaqr_factorize(spaqr_tree* t);
 int spaqr_solve(spaqr_tree* t, const double* b, double* x);
 /* Tree::solve on device vectors (b may alias x).                          */
 int spaqr_solve_device(spaqr_tree* t, const double* b, double* x);
-/* Tree::solve_nrml(VectorXd b, VectorXd& x)                tree.cpp:1471-1517 */
+/* Tree::solve_nrml(VectorXd b, VectorXd& x)                tree.cpp:1471-1517
+ * x = (W^T W)^{-1} b with the R-only factors of a least-squares tree
+ * (set_square(0)); b and x have length cols.                              */
 int spaqr_solve_nrml(spaqr_tree* t, const double* b, double* x);
+/* Same on device vectors (b may alias x).                                 */
+int spaqr_solve_nrml_device(spaqr_tree* t, const double* b, double* x);
 
 /* Sparse operator for the Krylov drivers (the SpMat `mat` argument of
  * gmres/cgls, is.cpp:95/275), uploaded once to the tree's device.         */
@@ -93,6 +97,17 @@ int spaqr_gmres(spaqr_tree* t, const spaqr_matrix* A, const double* b, double* x
 /* Same on device vectors.                                                 */
 int spaqr_gmres_device(spaqr_tree* t, const spaqr_matrix* A, const double* b, double* x, int iters, int restart,
                        double tol, int verbose, int* iters_out, double* est_out);
+
+/* cgls(mat, rhs, x, precond, iters, tol, verb)            is.cpp:275-346
+ * CGLS on A^T A x = A^T b preconditioned by the tree's normal-equation
+ * apply; stops when ||A^T r||^2 < tol^2 ||A^T b||^2.  A is m x n (uploaded
+ * with its transpose), b has length m, x length n (initial guess on entry).
+ * iters_out: iterations; est_out: ||A^T r|| / ||A^T b||.                 */
+int spaqr_cgls(spaqr_tree* t, const spaqr_matrix* A, const double* b, double* x, int iters, double tol, int verbose,
+               int* iters_out, double* est_out);
+/* Same on device vectors.                                                 */
+int spaqr_cgls_device(spaqr_tree* t, const spaqr_matrix* A, const double* b, double* x, int iters, double tol,
+                      int verbose, int* iters_out, double* est_out);
 
 /* spaQR.cpp:131-147: D_jj = 1/||A(:,j)||_2, A <- (A D) * 10 in place; D may be NULL. */
 int spaqr_colscale(int m, int n, const int* colptr, const int* rowind, double* val, double* D);

@@ -38,7 +38,7 @@ SYMBOLS = [
     "spaqr_colscale", "spaqr_random", "spaqr_perms", "spaqr_level_stats", "spaqr_counters",
     "spaqr_num_sparsified", "spaqr_sparsified", "spaqr_num_clusters", "spaqr_cluster_table",
     "spaqr_set_phase_sync", "spaqr_kernel_timing", "spaqr_kernel_stat_count", "spaqr_kernel_stat",
-    "spaqr_kernel_stats_reset",
+    "spaqr_kernel_stats_reset", "spaqr_solve_nrml_device", "spaqr_cgls", "spaqr_cgls_device",
 ]
 
 
@@ -68,6 +68,10 @@ def load_library(path: str = LIB_PATH):
     L.spaqr_solve.argtypes = [C.c_void_p, _dp, _dp]
     L.spaqr_solve_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
     L.spaqr_solve_nrml.argtypes = [C.c_void_p, _dp, _dp]
+    L.spaqr_solve_nrml_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+    L.spaqr_cgls.argtypes = [C.c_void_p, C.c_void_p, _dp, _dp, C.c_int, C.c_double, C.c_int, _ip, _dp]
+    L.spaqr_cgls_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_double, C.c_int,
+                                    _ip, _dp]
     L.spaqr_matrix_create.restype = C.c_void_p
     L.spaqr_matrix_create.argtypes = [C.c_void_p, C.c_int, C.c_int, _ip, _ip, _dp]
     L.spaqr_matrix_destroy.argtypes = [C.c_void_p]
@@ -199,10 +203,15 @@ class Tree:
         _check(load_library().spaqr_solve_device(self.h, C.c_void_p(b_ptr), C.c_void_p(x_ptr)), "solve_device")
 
     def solve_nrml(self, b):
+        """tree.cpp:1471-1517: x = (W^T W)^{-1} b (least-squares tree, b of length cols)."""
         b = np.ascontiguousarray(b, dtype=np.float64)
         x = np.zeros(self.n)
         _check(load_library().spaqr_solve_nrml(self.h, _d(b), _d(x)), "solve_nrml")
         return x
+
+    def solve_nrml_device(self, b_ptr, x_ptr):
+        _check(load_library().spaqr_solve_nrml_device(self.h, C.c_void_p(b_ptr), C.c_void_p(x_ptr)),
+               "solve_nrml_device")
 
     # inspection
     def perms(self):
@@ -297,11 +306,34 @@ def gmres_device(tree: Tree, M: Matrix, b_ptr, x_ptr, iters=300, restart=None, t
     return it.value, est.value
 
 
+def cgls(tree: Tree, M: Matrix, b, x0=None, iters=300, tol=1e-10, verbose=0):
+    """is.cpp:275-346 preconditioned by the tree's normal-equation apply;
+    returns (x, iters, est) with est = ||A^T r|| / ||A^T b||."""
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    x = np.zeros(M.n) if x0 is None else np.array(x0, dtype=np.float64)
+    it = C.c_int(0)
+    est = C.c_double(0)
+    _check(load_library().spaqr_cgls(tree.h, M.h, _d(b), _d(x), iters, tol, verbose, C.byref(it), C.byref(est)),
+           "cgls")
+    return x, it.value, est.value
+
+
+def cgls_device(tree: Tree, M: Matrix, b_ptr, x_ptr, iters=300, tol=1e-10, verbose=0):
+    it = C.c_int(0)
+    est = C.c_double(0)
+    _check(load_library().spaqr_cgls_device(tree.h, M.h, C.c_void_p(b_ptr), C.c_void_p(x_ptr), iters, tol, verbose,
+                                            C.byref(it), C.byref(est)), "cgls_device")
+    return it.value, est.value
+
+
 def run_flow(A, n, dim, tol, nlevels=None, skip=0, scale=1, order=1.0, iters=300, res=1e-12, precision=FP64,
              rhs=None, verbose=0, device=0):
-    """The driver flow of spaQR.cpp:85-320 on the GPU (square systems)."""
+    """The driver flow of spaQR.cpp:85-320 on the GPU: GMRES for square
+    systems, CGLS on the normal equations for tall ones."""
     import time
     A = sp.csc_matrix(A)
+    if A.shape[0] < A.shape[1]:
+        A = A.T.tocsc()
     m, ncols = A.shape
     if nlevels is None:
         nlevels = default_levels(ncols)
@@ -320,15 +352,24 @@ def run_flow(A, n, dim, tol, nlevels=None, skip=0, scale=1, order=1.0, iters=300
     t.factorize()
     out = dict(nlevels=nlevels, tree=t, A=As, D=D)
     b = random_vec(m, 2021) if rhs is None else np.asarray(rhs, dtype=np.float64)
-    t0 = time.perf_counter()
-    x = t.solve(b)
-    out["t_solve"] = time.perf_counter() - t0
-    out["oneshot_res"] = float(np.linalg.norm(As @ x - b) / np.linalg.norm(b))
     M = Matrix(t, As)
     t0 = time.perf_counter()
-    x, it, est = gmres(t, M, b, iters=iters, tol=res, verbose=verbose)
-    out["t_krylov"] = time.perf_counter() - t0
-    out["final_res"] = float(np.linalg.norm(As @ x - b) / np.linalg.norm(b))
+    if m == ncols:
+        x = t.solve(b)
+        out["t_solve"] = time.perf_counter() - t0
+        out["oneshot_res"] = float(np.linalg.norm(As @ x - b) / np.linalg.norm(b))
+        t0 = time.perf_counter()
+        x, it, est = gmres(t, M, b, iters=iters, tol=res, verbose=verbose)
+        out["t_krylov"] = time.perf_counter() - t0
+        out["final_res"] = float(np.linalg.norm(As @ x - b) / np.linalg.norm(b))
+    else:
+        x = t.solve_nrml(As.T @ b)
+        out["t_solve"] = time.perf_counter() - t0
+        out["oneshot_res"] = float(np.linalg.norm(As.T @ (As @ x - b)) / np.linalg.norm(As.T @ b))
+        t0 = time.perf_counter()
+        x, it, est = cgls(t, M, b, iters=iters, tol=res, verbose=verbose)
+        out["t_krylov"] = time.perf_counter() - t0
+        out["final_res"] = float(np.linalg.norm(As.T @ (As @ x - b)) / np.linalg.norm(As.T @ b))
     out.update(iters=it, est=est, x=x, b=b, M=M)
     out.update(t.counters())
     out["levels"] = t.level_stats()

@@ -162,7 +162,17 @@ int spaqr_solve_nrml(spaqr_tree* t, const double* b, double* x) {
     SPQ_HIP(hipMemcpy(db, b, sizeof(double) * e.cols(), hipMemcpyHostToDevice));
     e.solve_nrml_dev(db, dx);
     SPQ_HIP(hipStreamSynchronize(e.stream()));
+    e.check_solve();
     SPQ_HIP(hipMemcpy(x, dx, sizeof(double) * e.cols(), hipMemcpyDeviceToHost));
+  });
+}
+
+int spaqr_solve_nrml_device(spaqr_tree* t, const double* b, double* x) {
+  return guard([&] {
+    EngineBase& e = eng(t);
+    e.solve_nrml_dev(b, x);
+    SPQ_HIP(hipStreamSynchronize(e.stream()));
+    e.check_solve();
   });
 }
 
@@ -188,6 +198,38 @@ int spaqr_gmres_device(spaqr_tree* t, const spaqr_matrix* A, const double* b, do
     if (A->d->m != A->d->n || A->d->m != e.rows()) throw Error(kErrArg, "gmres needs the square matrix of the tree");
     KrylovResult r = gmres_dev(e, *A->d, b, x, iters, restart, tol, verbose);
     e.check_solve();
+    if (iters_out) *iters_out = r.iters;
+    if (est_out) *est_out = r.est;
+  });
+}
+int spaqr_cgls_device(spaqr_tree* t, const spaqr_matrix* A, const double* b, double* x, int iters, double tol,
+                      int verbose, int* iters_out, double* est_out) {
+  return guard([&] {
+    EngineBase& e = eng(t);
+    if (!A || !A->d) throw Error(kErrArg, "null matrix");
+    if (A->d->m != e.rows() || A->d->n != e.cols()) throw Error(kErrArg, "cgls needs the matrix of the tree");
+    KrylovResult r = cgls_dev(e, *A->d, b, x, iters, tol, verbose);
+    e.check_solve();
+    if (iters_out) *iters_out = r.iters;
+    if (est_out) *est_out = r.est;
+  });
+}
+int spaqr_cgls(spaqr_tree* t, const spaqr_matrix* A, const double* b, double* x, int iters, double tol, int verbose,
+               int* iters_out, double* est_out) {
+  return guard([&] {
+    EngineBase& e = eng(t);
+    if (!A || !A->d) throw Error(kErrArg, "null matrix");
+    const int n = A->d->n, m = A->d->m;
+    if (m != e.rows() || n != e.cols()) throw Error(kErrArg, "cgls needs the matrix of the tree");
+    double *db = nullptr, *dx = nullptr;
+    SPQ_HIP(hipMalloc((void**)&db, sizeof(double) * std::max(m, 1)));
+    SPQ_HIP(hipMalloc((void**)&dx, sizeof(double) * std::max(n, 1)));
+    struct F { double* p; ~F() { hipFree(p); } } fb{db}, fx{dx};
+    SPQ_HIP(hipMemcpy(db, b, sizeof(double) * m, hipMemcpyHostToDevice));
+    SPQ_HIP(hipMemcpy(dx, x, sizeof(double) * n, hipMemcpyHostToDevice));
+    KrylovResult r = cgls_dev(e, *A->d, db, dx, iters, tol, verbose);
+    e.check_solve();
+    SPQ_HIP(hipMemcpy(x, dx, sizeof(double) * n, hipMemcpyDeviceToHost));
     if (iters_out) *iters_out = r.iters;
     if (est_out) *est_out = r.est;
   });

@@ -106,6 +106,12 @@ void spmv(const Csc& A, const std::vector<double>& x, std::vector<double>& y) {
   for (int c = 0; c < A.n; ++c)
     for (int q = A.p[c]; q < A.p[c + 1]; ++q) y[A.i[q]] += A.x[q] * x[c];
 }
+// y = A^T x
+void spmv_t(const Csc& A, const std::vector<double>& x, std::vector<double>& y) {
+  y.assign(A.n, 0.0);
+  for (int c = 0; c < A.n; ++c)
+    for (int q = A.p[c]; q < A.p[c + 1]; ++q) y[c] += A.x[q] * x[A.i[q]];
+}
 double nrm(const std::vector<double>& v) {
   double s = 0;
   for (double a : v) s += a * a;
@@ -273,8 +279,16 @@ int main(int argc, char** argv) {
       for (int i = 0; i < nrows; ++i) Ax[i] -= b[i];
       printf("<<<<|(Ax-b)|/|b| : %.3e\n", nrm(Ax) / nrm(b));
     } else {
-      fprintf(stderr, "The least-squares solve is not available in this build.\n");
-      return 2;
+      // spaQR.cpp:277-283: one normal-equation apply of A^T b
+      std::vector<double> Atb, r;
+      spmv_t(A, b, Atb);
+      check(spaqr_solve_nrml(t, Atb.data(), x.data()), "solve_nrml");
+      printf("<<<<tsolv=%.3e\n", now() - t0);
+      printf("One-time solve (Random b):\n");
+      spmv(A, x, Ax);
+      for (int i = 0; i < nrows; ++i) Ax[i] -= b[i];
+      spmv_t(A, Ax, r);
+      printf("<<<<|A'(Ax-b)|/|A'b| : %.3e\n", nrm(r) / nrm(Atb));
     }
   }
   {
@@ -290,14 +304,28 @@ int main(int argc, char** argv) {
     int it = 0;
     double est = 0;
     const double g0 = now();
-    check(spaqr_gmres(t, M, b.data(), x.data(), iterations, iterations, residual, 0, &it, &est), "gmres");
-    const double g1 = now();
-    spmv(A, x, Ax);
-    for (int i = 0; i < nrows; ++i) Ax[i] -= b[i];
-    printf("GMRES converged!\n");
-    printf("GMRES: #iterations: %d, residual |Ax-b|/|b|: %.3e\n", it, nrm(Ax) / nrm(b));
-    printf("  GMRES: %.3e s.\n", g1 - g0);
-    printf("<<<<GMRES=%d\n", it);
+    if (nrows == ncols) {
+      check(spaqr_gmres(t, M, b.data(), x.data(), iterations, iterations, residual, 0, &it, &est), "gmres");
+      const double g1 = now();
+      spmv(A, x, Ax);
+      for (int i = 0; i < nrows; ++i) Ax[i] -= b[i];
+      printf("GMRES converged!\n");
+      printf("GMRES: #iterations: %d, residual |Ax-b|/|b|: %.3e\n", it, nrm(Ax) / nrm(b));
+      printf("  GMRES: %.3e s.\n", g1 - g0);
+      printf("<<<<GMRES=%d\n", it);
+    } else {
+      // spaQR.cpp:308-316
+      check(spaqr_cgls(t, M, b.data(), x.data(), iterations, residual, 0, &it, &est), "cgls");
+      const double g1 = now();
+      std::vector<double> r, Atb;
+      spmv(A, x, Ax);
+      for (int i = 0; i < nrows; ++i) Ax[i] -= b[i];
+      spmv_t(A, Ax, r);
+      spmv_t(A, b, Atb);
+      printf("CGLS: #iterations: %d, residual |A'(Ax-b)|/|A'(b)|: %.3e\n", it, nrm(r) / nrm(Atb));
+      printf("  CGLS: %.3e s.\n", g1 - g0);
+      printf("<<<<CGLS=%d\n", it);
+    }
     spaqr_matrix_destroy(M);
   }
   spaqr_tree_destroy(t);

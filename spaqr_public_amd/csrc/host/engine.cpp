@@ -44,6 +44,8 @@ class Engine final : public EngineBase {
     (void)hipStreamSynchronize(stream_);
     for (auto e : ev_pool_) (void)hipEventDestroy(e);
     if (graph_exec_fwd_) hipGraphExecDestroy(graph_exec_fwd_);
+    if (graph_exec_nrml_) hipGraphExecDestroy(graph_exec_nrml_);
+    if (xtbuf_) hipFree(xtbuf_);
     if (rank_host_) hipHostFree(rank_host_);
     if (xbuf_) hipFree(xbuf_);
     if (bin_) hipFree(bin_);
@@ -105,9 +107,7 @@ class Engine final : public EngineBase {
   void assemble(const CscMat& A) override;
   int factorize() override;
   void solve_dev(const double* b, double* x) override;
-  void solve_nrml_dev(const double*, double*) override {
-    throw Error(kErrState, "solve_nrml: the least-squares apply is not available in this build");
-  }
+  void solve_nrml_dev(const double* b, double* x) override;
 
   std::vector<std::array<int, 12>> cluster_table() const override {
     std::vector<std::array<int, 12>> t;
@@ -154,7 +154,9 @@ class Engine final : public EngineBase {
   // Solve op records (sizes captured at creation, operations.h / toperations.h)
   // OpFineQR: the QR op of an order-1.5 fine cluster (tree.cpp:1252-1253):
   // tau = 0, so only its backward R12 / R sweep does work
-  enum OpKind { OpQR, OpScaleD, OpOrthoD, OpSplitD, OpMerge, OpFineQR };
+  // OpReassign: least-squares rows moved from an eliminated cluster to a
+  // neighbour (operations.cpp:59-68, toperations.cpp:56-66)
+  enum OpKind { OpQR, OpScaleD, OpOrthoD, OpSplitD, OpMerge, OpFineQR, OpReassign };
   struct Op {
     OpKind kind;
     int level, c = -1, f = -1;
@@ -170,6 +172,9 @@ class Engine final : public EngineBase {
     // SplitD
     int crows = 0, ccols = 0, rank = 0;
     int wave = 0;  // sparsify wave (OrthoD, SplitD, FineQR)
+    // Reassign: x_f[nstart + t] = x_c[idx[t]]
+    std::vector<int> idx;
+    int nstart = 0;
   };
 
   // ------------------------------------------------------------ helpers
@@ -181,10 +186,16 @@ class Engine final : public EngineBase {
     cl_.push_back(c);
     return (int)cl_.size() - 1;
   }
-  void alloc_x(int c, int cap) {
-    cl_[c].xoff = xsize_;
-    cl_[c].xcap = cap;
-    xsize_ += cap;
+  // A cluster's x segment needs room for the most rows it holds during its
+  // one-level lifetime (reassigned least-squares rows grow it, tree.cpp:365);
+  // offsets are laid out when the solve program is built.
+  void alloc_x(int c, int cap) { cl_[c].xcap = std::max(cl_[c].xcap, cap); }
+  void layout_x() {
+    xsize_ = 0;
+    for (auto& c : cl_) {
+      if (c.xcap > 0) { c.xoff = xsize_; xsize_ += c.xcap; }
+      else c.xoff = 0;
+    }
   }
   T* alloc_perm(size_t n) { return (T*)perm_.alloc(sizeof(T) * std::max<size_t>(n, 1)); }
   T* alloc_scr(size_t n) { return (T*)scratch_.alloc(sizeof(T) * std::max<size_t>(n, 1)); }
@@ -409,6 +420,16 @@ class Engine final : public EngineBase {
   void scale_level(const std::vector<int>& cs);
   void sparsify_level(const std::vector<int>& cs);
   void merge_all();
+  void reassign_level(const std::vector<int>& cs);
+  struct RrqrJob {
+    T* Ct;
+    int k, N;
+    int rank = 0, nwg = 1;
+    T* V = nullptr;
+    T* tau = nullptr;
+  };
+  void rrqr_batch(std::vector<RrqrJob>& jobs);
+  void sparsify_extra_level(const std::vector<int>& cs);
   void build_solve_program();
 
   // ------------------------------------------------------------ state
@@ -457,6 +478,7 @@ class Engine final : public EngineBase {
   struct Stage {
     int type;  // 0 refl, 1 bwd, 2 move
     int nd = 0, mmax = 0, nwg_total = 0;
+    int buf = 0, sbuf = 0;  // x buffer used (0 = x, 1 = xt); moves copy sbuf -> buf
     const ReflDesc<T>* refl = nullptr;
     const BwdDesc<T>* bwd = nullptr;
     const MoveDesc* mv = nullptr;
@@ -478,7 +500,14 @@ class Engine final : public EngineBase {
   hipGraphExec_t graph_exec_fwd_ = nullptr;
   double* bin_ = nullptr;
   double* xout_ = nullptr;
-  void run_program(const double* b, double* x, bool prof = false);
+  void run_program(const double* b, double* x, bool prof = false, bool nrml = false);
+  // normal-equation programs (least squares) and their entry map
+  std::vector<Stage> tfwd_, tbwd_, pfwd_;
+  T* xtbuf_ = nullptr;
+  int64_t* tin_dst_ = nullptr;
+  int* tin_src_ = nullptr;
+  int n_tin_ = 0;
+  hipGraphExec_t graph_exec_nrml_ = nullptr;
   bool stage_prof_ = false;  // SPAQR_SOLVE_STAGES=1: time every solve stage (uncaptured)
   bool solve_graph_ = true;  // SPAQR_SOLVE_GRAPH=0: launch the solve program directly (no hipGraph)
   bool qp3_reg_ = true;      // SPAQR_QP3_REGISTER=0: streaming truncated QR only
@@ -839,6 +868,174 @@ void Engine<T>::scale_level(const std::vector<int>& cs) {
   }
 }
 
+// Truncated column-pivoted QR (the geqp3 + choose_rank of util.cpp:466-477,
+// 795-816) of a batch of independent problems whose transposed inputs Ct
+// (N x k, ld N) are already gathered.  Per problem: the register form when its
+// columns fit the resident workgroups, else the streaming form; single-
+// workgroup problems in plain launches, multi-workgroup ones cooperatively
+// within the resident capacity.  Fills rank, V (k x k, ld k) and tau; on
+// exit Ct(0:rank, j) = triu(R(0:rank,:)) P^T for every column j.
+template <typename T>
+void Engine<T>::rrqr_batch(std::vector<RrqrJob>& jobs) {
+  if (jobs.empty()) return;
+  int kmax = 0;
+  for (auto& j : jobs) kmax = std::max(kmax, j.k);
+  const bool lds_ok = sizeof(T) * (size_t)std::max(kmax, 1) <= (size_t)48 * 1024;
+  int* ranks_dev = (int*)scratch_.alloc(sizeof(int) * jobs.size());
+  std::vector<Qp3Desc<T>> qd;
+  std::vector<int> sync_off, nts;
+  int sync_words = 0;
+  for (size_t q = 0; q < jobs.size(); ++q) {
+    RrqrJob& jb = jobs[q];
+    const int N = std::max(jb.N, 1);
+    int G = 0, nwg = 1, nt = 256;
+    if (qp3_reg_ && jb.k <= 64 * qp3r_rows()) {
+      G = 1;
+      while (G * qp3r_rows() < jb.k) G <<= 1;
+      // large problems: 1024-thread workgroups (one per CU, fewer to meet)
+      nt = iceil(N, 256 / G) > 32 ? 1024 : 256;
+      nwg = iceil(N, nt / G);
+      if (nwg > qp3r_cap(nt)) G = 0;
+    }
+    if (G == 0) nwg = lds_ok ? std::max(1, std::min(qp3_max_wg_, jb.N / 256)) : 1;
+    jb.nwg = nwg;
+    jb.V = alloc_scr((size_t)jb.k * jb.k);
+    jb.tau = alloc_scr(jb.k);
+    Qp3Desc<T> d{};
+    d.Ct = jb.Ct; d.V = jb.V; d.tau = jb.tau;
+    d.vn = G ? nullptr : (double*)scratch_.alloc(sizeof(double) * 2 * N);
+    d.pos = (int*)scratch_.alloc(sizeof(int) * (G ? (size_t)N : 2 * (size_t)N * nwg));
+    d.part_v = (double*)scratch_.alloc(sizeof(double) * 2 * nwg);
+    d.part_p = (int*)scratch_.alloc(sizeof(int) * 4 * nwg);
+    d.beta = (double*)scratch_.alloc(sizeof(double) * std::max(jb.k, 1));
+    sync_off.push_back(sync_words);
+    sync_words += barrier_words(nwg);
+    nts.push_back(G ? nt : 0);
+    d.err = qp3_err_;
+    d.rank_out = ranks_dev + q;
+    d.ldc = N; d.k = jb.k; d.N = jb.N; d.ldv = jb.k; d.tol = tol_; d.nwg = nwg; d.G = G;
+    d.part_col = G ? alloc_scr((size_t)2 * nwg * jb.k) : nullptr;
+    qd.push_back(d);
+    const int kk = std::min(jb.k, jb.N);
+    flops_ += 4.0 * jb.k * jb.N * kk - 2.0 * (jb.k + jb.N) * kk * kk + 4.0 / 3.0 * kk * kk * kk;
+  }
+  if (tol_ >= 1.0) {
+    for (auto& j : jobs) j.rank = 0;  // util.cpp:798-799
+    return;
+  }
+  // barrier counters start at zero
+  unsigned int* syncs = (unsigned int*)scratch_.alloc(sizeof(unsigned int) * std::max(sync_words, 1));
+  for (size_t q = 0; q < qd.size(); ++q) qd[q].sync = syncs + sync_off[q];
+  launch_zero(stream_, 4, syncs, sync_words);
+  struct Group { int nt, coop, gmax; std::vector<Qp3Desc<T>> d; };
+  std::vector<Group> groups;
+  for (int nt : {0, 256, 1024}) {  // 0: streaming form
+    const int cap = nt ? qp3r_cap(nt) : qp3_capacity<T>(kmax);
+    Group single{nt, 0, 1, {}};
+    Group cur{nt, 1, 1, {}};
+    int used = 0;
+    for (size_t q = 0; q < qd.size(); ++q) {
+      if (nts[q] != nt) continue;
+      Qp3Desc<T> d = qd[q];
+      if (d.nwg == 1) {
+        single.d.push_back(d);
+        single.gmax = std::max(single.gmax, d.G);
+        continue;
+      }
+      if (!nt && d.nwg > cap) d.nwg = std::max(1, cap);
+      if (used + d.nwg > cap && !cur.d.empty()) { groups.push_back(cur); cur = Group{nt, 1, 1, {}}; used = 0; }
+      cur.d.push_back(d);
+      cur.gmax = std::max(cur.gmax, d.G);
+      used += d.nwg;
+    }
+    if (!single.d.empty()) groups.push_back(single);
+    if (!cur.d.empty()) groups.push_back(cur);
+  }
+  for (auto& gq : groups) {
+    std::vector<int> cnt, st;
+    double qby = 0;
+    for (auto& d : gq.d) {
+      cnt.push_back(d.nwg);
+      qby += 2.0 * sizeof(T) * (double)d.k * d.N;  // C read once, written once
+    }
+    const int nt = prefix_tiles(cnt, st);
+    const Qp3Desc<T>* dqd = stager_.push(gq.d);
+    const int* dst = stager_.push(st);
+    int rc = 0;
+    // one timing family per kernel: streaming, register 256 / 1024 threads;
+    // flops are added once the ranks are known
+    const char* fam = gq.nt == 0 ? "qp3s" : (gq.nt == 1024 ? "qp3r1024" : "qp3r256");
+    for (auto& d : gq.d) qp3_family_[d.rank_out] = fam;
+    timed(fam, 0.0, qby, [&] {
+      rc = gq.nt ? launch_qp3r<T>(stream_, dqd, dst, (int)gq.d.size(), nt, gq.gmax, gq.nt, gq.coop)
+                 : launch_qp3<T>(stream_, dqd, dst, (int)gq.d.size(), nt, kmax, gq.coop);
+    });
+    if (rc != 0) throw Error(kErrHip, std::string("qp3 launch failed: ") + hipGetErrorString((hipError_t)rc));
+  }
+  SPQ_HIP(hipMemcpyAsync(rank_host_, ranks_dev, sizeof(int) * jobs.size(), hipMemcpyDeviceToHost, stream_));
+  SPQ_HIP(hipMemcpyAsync(rank_host_ + 65535, qp3_err_, sizeof(int), hipMemcpyDeviceToHost, stream_));
+  sync();
+  if (rank_host_[65535] != 0) throw Error(kErrNumeric, "qp3: an inter-workgroup barrier wait timed out");
+  for (size_t q = 0; q < jobs.size(); ++q) jobs[q].rank = rank_host_[q];
+  if (kprof_) {
+    // truncated RRQR: 4kNr - 2(k+N)r^2 + 4r^3/3 + 2kN (initial norms)
+    for (size_t q = 0; q < jobs.size(); ++q) {
+      const double k = jobs[q].k, N = jobs[q].N, r = jobs[q].rank < jobs[q].k ? jobs[q].rank + 1 : jobs[q].rank;
+      auto it = qp3_family_.find(qd[q].rank_out);
+      if (it != qp3_family_.end())
+        kev_[it->second].flops += 4 * k * N * r - 2 * (k + N) * r * r + 4.0 / 3.0 * r * r * r + 2 * k * N;
+    }
+  }
+  qp3_family_.clear();
+}
+
+// tree.cpp:1260-1307 (+ sparsify_extra_rows 226-247), least squares: the
+// extra rows (rows - cols) of an interface's in-edge blocks are compressed by
+// a truncated pivoted QR; they become triu(R(0:r,:)) P^T, r = choose_rank,
+// and the interface keeps cols + r rows.  Interfaces are independent here
+// (each rewrites only its own in-edge blocks), so one batch covers a level.
+template <typename T>
+void Engine<T>::sparsify_extra_level(const std::vector<int>& cs) {
+  struct X { int c, cols, k, N; std::vector<int> apm; T* Ct; };
+  std::vector<X> xs;
+  std::vector<CopyDesc> gather, scatter;
+  std::vector<RrqrJob> rj;
+  scratch_.rewind();
+  for (int c : cs) {
+    if (cl_[c].csize == 0 || cl_[c].rsize <= cl_[c].csize) continue;
+    X x;
+    x.c = c; x.cols = cl_[c].csize; x.k = cl_[c].rsize - x.cols; x.N = 0;
+    for (int e : cl_[c].in)
+      if (!cl_[ed_[e].n1].elim) { x.apm.push_back(e); x.N += ed_[e].a.cols; }
+    x.Ct = alloc_scr((size_t)std::max(x.N, 1) * x.k);
+    int off = 0;
+    for (int e : x.apm) {
+      const Block& b = ed_[e].a;
+      if (b.cols > 0) gather.push_back(CopyDesc{b.p + x.cols, x.Ct + off, b.ld, std::max(x.N, 1), x.k, b.cols, 1, 0});
+      off += b.cols;
+    }
+    if (x.N > 0) rj.push_back(RrqrJob{x.Ct, x.k, x.N});
+    xs.push_back(std::move(x));
+  }
+  if (xs.empty()) return;
+  copy_blocks(gather);
+  rrqr_batch(rj);
+  size_t q = 0;
+  for (auto& x : xs) {
+    const int r = x.N > 0 ? rj[q++].rank : 0;
+    int off = 0;
+    for (int e : x.apm) {
+      Block& b = ed_[e].a;
+      if (r > 0 && b.cols > 0) scatter.push_back(CopyDesc{x.Ct + off, b.p + x.cols, x.N, b.ld, b.cols, r, 1, 0});
+      b.rows = x.cols + r;
+      off += b.cols;
+    }
+    cl_[x.c].rsize = x.cols + r;
+    ed_[find_out(x.c, x.c)].a.rows = x.cols + r;
+  }
+  copy_blocks(scatter);
+}
+
 // tree.cpp:992-1114 (sparsifyD, scale = 1, order = 1) in dependency waves:
 // an interface waits for every earlier interface it shares a block with.
 template <typename T>
@@ -873,24 +1070,17 @@ void Engine<T>::sparsify_level(const std::vector<int>& cs) {
       int c, k, N, smp, spm = 0;
       std::vector<int> amp, apm;
       T* W = nullptr;  // order 1.5: top k rows of the in-edge blocks, k x spm (ld k)
+      T* Rest = nullptr;  // least squares: rows [k, rows) of the in-edge blocks, rest x spm
+      int rest = 0;
       T *Ct, *V, *tau;
-      double* vn;
-      int* pos;
     };
     std::vector<Job> jobs;
     std::vector<CopyDesc> gather;
-    std::vector<Qp3Desc<T>> qd;
+    std::vector<RrqrJob> rj;
     int kmax = 0;
-    int* ranks_dev = (int*)scratch_.alloc(sizeof(int) * S.size());
-    std::vector<int> sync_off;  // barrier counters of each problem (words), one array cleared by one launch
-    int sync_words = 0;
-    std::vector<int> nts;       // threads per workgroup of each register-form problem
     std::vector<int> wave_idx;
     for (size_t i = 0; i < S.size(); ++i)
       if (wave[i] == w) { wave_idx.push_back((int)i); kmax = std::max(kmax, cl_[S[i]].csize); }
-    // workgroups per problem: >= 256 owned columns each, only when the
-    // reflector fits in LDS (multi-workgroup problems keep it there)
-    const bool lds_ok = sizeof(T) * (size_t)std::max(kmax, 1) <= (size_t)48 * 1024;
     for (int i : wave_idx) {
       const int c = S[i];
       Job jb;
@@ -902,23 +1092,7 @@ void Engine<T>::sparsify_level(const std::vector<int>& cs) {
         if (!cl_[ed_[e].n2].elim && ed_[e].n2 != c) { jb.amp.push_back(e); jb.smp += ed_[e].a.rows; }
       jb.N = jb.smp + spm;
       const int N = std::max(jb.N, 1);
-      // register form when the columns fit the resident workgroups, else
-      // the streaming form
-      int G = 0, nwg = 1, nt = 256;
-      if (qp3_reg_ && jb.k <= 64 * qp3r_rows()) {
-        G = 1;
-        while (G * qp3r_rows() < jb.k) G <<= 1;
-        // large problems: 1024-thread workgroups (one per CU, fewer to meet)
-        nt = iceil(N, 256 / G) > 32 ? 1024 : 256;
-        nwg = iceil(N, nt / G);
-        if (nwg > qp3r_cap(nt)) G = 0;
-      }
-      if (G == 0) nwg = lds_ok ? std::max(1, std::min(qp3_max_wg_, jb.N / 256)) : 1;
       jb.Ct = alloc_scr((size_t)N * jb.k);
-      jb.V = alloc_scr((size_t)jb.k * jb.k);
-      jb.tau = alloc_scr(jb.k);
-      jb.vn = G ? nullptr : (double*)scratch_.alloc(sizeof(double) * 2 * N);
-      jb.pos = (int*)scratch_.alloc(sizeof(int) * (G ? (size_t)N : 2 * (size_t)N * nwg));
       int off = 0;
       for (int e : jb.amp) {
         const Block& b = ed_[e].a;
@@ -931,6 +1105,17 @@ void Engine<T>::sparsify_level(const std::vector<int>& cs) {
         off += b.cols;
       }
       jb.spm = spm;
+      jb.rest = cl_[c].rsize - jb.k;
+      if (jb.rest > 0 && spm > 0) {
+        // Apmc_rest (tree.cpp:1016), moved up under the rank rows afterwards
+        jb.Rest = alloc_scr((size_t)jb.rest * spm);
+        int q = 0;
+        for (int e : jb.apm) {
+          const Block& b = ed_[e].a;
+          gather.push_back(CopyDesc{b.p + jb.k, jb.Rest + (int64_t)q * jb.rest, b.ld, jb.rest, jb.rest, b.cols, 0, 0});
+          q += b.cols;
+        }
+      }
       if (imp && spm > 0) {
         // Apmc_top (tree.cpp:1130-1140), kept before the blocks are rewritten
         jb.W = alloc_scr((size_t)jb.k * spm);
@@ -941,102 +1126,23 @@ void Engine<T>::sparsify_level(const std::vector<int>& cs) {
           q += b.cols;
         }
       }
-      Qp3Desc<T> d{};
-      d.Ct = jb.Ct; d.V = jb.V; d.tau = jb.tau; d.vn = jb.vn; d.pos = jb.pos;
-      d.part_v = (double*)scratch_.alloc(sizeof(double) * 2 * nwg);
-      d.part_p = (int*)scratch_.alloc(sizeof(int) * 4 * nwg);
-      d.beta = (double*)scratch_.alloc(sizeof(double) * std::max(jb.k, 1));
-      sync_off.push_back(sync_words);
-      sync_words += barrier_words(nwg);
-      nts.push_back(G ? nt : 0);
-      d.err = qp3_err_;
-      d.rank_out = ranks_dev + jobs.size();
-      d.ldc = N; d.k = jb.k; d.N = jb.N; d.ldv = jb.k; d.tol = tol_; d.nwg = nwg; d.G = G;
-      d.part_col = G ? alloc_scr((size_t)2 * nwg * jb.k) : nullptr;
-      qd.push_back(d);
-      const int kk = std::min(jb.k, jb.N);
-      flops_ += 4.0 * jb.k * jb.N * kk - 2.0 * (jb.k + jb.N) * kk * kk + 4.0 / 3.0 * kk * kk * kk;
+      rj.push_back(RrqrJob{jb.Ct, jb.k, jb.N});
       jobs.push_back(std::move(jb));
     }
     copy_blocks(gather);
+    rrqr_batch(rj);
     std::vector<int> ranks(jobs.size(), 0);
-    if (tol_ >= 1.0) {
-      std::fill(ranks.begin(), ranks.end(), 0);  // util.cpp:798-799
-    } else {
-      // barrier counters start at zero
-      unsigned int* syncs = (unsigned int*)scratch_.alloc(sizeof(unsigned int) * std::max(sync_words, 1));
-      for (size_t q = 0; q < qd.size(); ++q) qd[q].sync = syncs + sync_off[q];
-      launch_zero(stream_, 4, syncs, sync_words);
-      // per kernel form: single-workgroup problems in one plain launch;
-      // multi-workgroup ones cooperatively, packed within the resident-
-      // workgroup capacity
-      struct Group { int nt, coop, gmax; std::vector<Qp3Desc<T>> d; };
-      std::vector<Group> groups;
-      for (int nt : {0, 256, 1024}) {  // 0: streaming form
-        const int cap = nt ? qp3r_cap(nt) : qp3_capacity<T>(kmax);
-        Group single{nt, 0, 1, {}};
-        Group cur{nt, 1, 1, {}};
-        int used = 0;
-        for (size_t q = 0; q < qd.size(); ++q) {
-          if (nts[q] != nt) continue;
-          Qp3Desc<T> d = qd[q];
-          if (d.nwg == 1) {
-            single.d.push_back(d);
-            single.gmax = std::max(single.gmax, d.G);
-            continue;
-          }
-          if (!nt && d.nwg > cap) d.nwg = std::max(1, cap);
-          if (used + d.nwg > cap && !cur.d.empty()) { groups.push_back(cur); cur = Group{nt, 1, 1, {}}; used = 0; }
-          cur.d.push_back(d);
-          cur.gmax = std::max(cur.gmax, d.G);
-          used += d.nwg;
-        }
-        if (!single.d.empty()) groups.push_back(single);
-        if (!cur.d.empty()) groups.push_back(cur);
-      }
-      for (auto& gq : groups) {
-        std::vector<int> cnt, st;
-        double qby = 0;
-        for (auto& d : gq.d) {
-          cnt.push_back(d.nwg);
-          qby += 2.0 * sizeof(T) * (double)d.k * d.N;  // C read once, written once
-        }
-        const int nt = prefix_tiles(cnt, st);
-        const Qp3Desc<T>* dqd = stager_.push(gq.d);
-        const int* dst = stager_.push(st);
-        int rc = 0;
-        // one timing family per kernel: streaming, register 256 / 1024 threads;
-        // flops are added once the ranks are known
-        const char* fam = gq.nt == 0 ? "qp3s" : (gq.nt == 1024 ? "qp3r1024" : "qp3r256");
-        for (auto& d : gq.d) qp3_family_[d.rank_out] = fam;
-        timed(fam, 0.0, qby, [&] {
-          rc = gq.nt ? launch_qp3r<T>(stream_, dqd, dst, (int)gq.d.size(), nt, gq.gmax, gq.nt, gq.coop)
-                     : launch_qp3<T>(stream_, dqd, dst, (int)gq.d.size(), nt, kmax, gq.coop);
-        });
-        if (rc != 0) throw Error(kErrHip, std::string("qp3 launch failed: ") + hipGetErrorString((hipError_t)rc));
-      }
-      SPQ_HIP(hipMemcpyAsync(rank_host_, ranks_dev, sizeof(int) * jobs.size(), hipMemcpyDeviceToHost, stream_));
-      SPQ_HIP(hipMemcpyAsync(rank_host_ + 65535, qp3_err_, sizeof(int), hipMemcpyDeviceToHost, stream_));
-      sync();
-      if (rank_host_[65535] != 0) throw Error(kErrNumeric, "qp3: an inter-workgroup barrier wait timed out");
-      for (size_t q = 0; q < jobs.size(); ++q) ranks[q] = rank_host_[q];
-      if (kprof_) {
-        // truncated RRQR: 4kNr - 2(k+N)r^2 + 4r^3/3 + 2kN (initial norms)
-        for (size_t q = 0; q < jobs.size(); ++q) {
-          const double k = jobs[q].k, N = jobs[q].N, r = ranks[q] < jobs[q].k ? ranks[q] + 1 : ranks[q];
-          auto it = qp3_family_.find(qd[q].rank_out);
-          if (it != qp3_family_.end())
-            kev_[it->second].flops += 4 * k * N * r - 2 * (k + N) * r * r + 4.0 / 3.0 * r * r * r + 2 * k * N;
-        }
-      }
-      qp3_family_.clear();
+    for (size_t q = 0; q < jobs.size(); ++q) {
+      ranks[q] = rj[q].rank;
+      jobs[q].V = rj[q].V;
+      jobs[q].tau = rj[q].tau;
     }
     if (verbose_ > 1) {
       int nmax = 0, nwgs = 0, rsum = 0, ksum = 0;
       double kn = 0;
       for (size_t q = 0; q < jobs.size(); ++q) {
         nmax = std::max(nmax, jobs[q].N);
-        nwgs += qd[q].nwg;
+        nwgs += rj[q].nwg;
         rsum += ranks[q];
         ksum += jobs[q].k;
         kn += (double)jobs[q].k * jobs[q].N;
@@ -1096,12 +1202,15 @@ void Engine<T>::sparsify_level(const std::vector<int>& cs) {
         b.cols = rank;
         off += b.rows;
       }
+      int qr0 = 0;
       for (int e : jb.apm) {
         Block& b = ed_[e].a;
-        if (rest > 0) throw Error(kErrState, "sparsify: rectangular interface blocks are not supported in this build");
         if (rank > 0) scatter.push_back(CopyDesc{jb.Ct + off, b.p, N, b.ld, b.cols, rank, 1, 0});
+        if (rest > 0 && b.cols > 0)
+          scatter.push_back(CopyDesc{jb.Rest + (int64_t)qr0 * rest, b.p + rank, rest, b.ld, rest, b.cols, 0, 0});
         b.rows = rank_rows;
         off += b.cols;
+        qr0 += b.cols;
       }
       const int es = find_out(c, c);
       if (imp) {
@@ -1170,6 +1279,153 @@ void Engine<T>::sparsify_level(const std::vector<int>& cs) {
   sprs_.insert(sprs_.end(), recs.begin(), recs.end());
 }
 
+// tree.cpp:319-411 (least squares), for every cluster eliminated at this
+// level with more rows than columns: each extra row goes to the column
+// neighbour whose in-edge block has the largest norm on that row (first one
+// on ties, else the cluster itself); rows are ordered by target, appended to
+// the target's blocks, and every other in-edge block of a grown cluster is
+// padded with zero rows.  The growth is simulated on the host in the
+// reference's order and each grown block is materialised once: zero, old
+// rows, gathered rows.
+template <typename T>
+void Engine<T>::reassign_level(const std::vector<int>& cs) {
+  std::vector<int> E;
+  for (int c : cs)
+    if (cl_[c].elim && cl_[c].rsize > cl_[c].csize) E.push_back(c);
+  if (E.empty()) return;
+  scratch_.rewind();
+  // (1) norms of the extra rows of the column-neighbour in-edge blocks
+  std::vector<RowNormDesc<T>> nd;
+  std::vector<std::pair<int, int>> nkey;  // (cluster, edge)
+  std::vector<int> ncnt, nstart_host;
+  int total = 0;
+  for (int c : E) {
+    const int rsz = cl_[c].rsize - cl_[c].csize;
+    for (int e : cl_[c].in) {
+      const int n = ed_[e].n1;
+      if (find_out(c, n) < 0) continue;
+      const Block& b = ed_[e].a;
+      nd.push_back(RowNormDesc<T>{b.p, nullptr, b.ld, cl_[c].csize, rsz, b.cols});
+      nkey.push_back({c, e});
+      nstart_host.push_back(total);
+      ncnt.push_back(iceil(rsz, 256));
+      total += rsz;
+    }
+  }
+  std::vector<double> norms(total);
+  if (total > 0) {
+    double* dn = (double*)scratch_.alloc(sizeof(double) * total);
+    for (size_t q = 0; q < nd.size(); ++q) nd[q].out = dn + nstart_host[q];
+    std::vector<int> st;
+    const int nt = prefix_tiles(ncnt, st);
+    const RowNormDesc<T>* dd = stager_.push(nd);
+    const int* dst = stager_.push(st);
+    launch_rownorm<T>(stream_, dd, dst, (int)nd.size(), nt);
+    SPQ_HIP(hipMemcpyAsync(norms.data(), dn, sizeof(double) * total, hipMemcpyDeviceToHost, stream_));
+    sync();
+  }
+  // (2) the reference's sequence on a symbolic copy of the block rows
+  struct Grow {
+    Block base;            // content before this level's reassignments
+    int rows;              // current row count
+    struct App { const T* src; int lds, row0; std::vector<int> idx; };
+    std::vector<App> app;  // appended row runs
+  };
+  std::map<int, Grow> grow;  // edge -> growth
+  auto rows_of = [&](int e) { auto it = grow.find(e); return it == grow.end() ? ed_[e].a.rows : it->second.rows; };
+  auto touch = [&](int e) -> Grow& {
+    auto it = grow.find(e);
+    if (it == grow.end()) it = grow.emplace(e, Grow{ed_[e].a, ed_[e].a.rows, {}}).first;
+    return it->second;
+  };
+  std::vector<Block> zero;
+  size_t ni = 0;
+  for (int c : E) {
+    const int cols = cl_[c].csize, rsz = cl_[c].rsize - cols;
+    std::vector<std::pair<int, double>> r2c(rsz, {c, 0.0});
+    for (; ni < nkey.size() && nkey[ni].first == c; ++ni) {
+      const int n = ed_[nkey[ni].second].n1;
+      const double* nv = norms.data() + nstart_host[ni];
+      for (int i = 0; i < rsz; ++i)
+        if (nv[i] > r2c[i].second) r2c[i] = {n, nv[i]};
+    }
+    std::vector<int> rp(rsz);
+    std::iota(rp.begin(), rp.end(), 0);
+    std::stable_sort(rp.begin(), rp.end(), [&](int a, int b) { return cl_[r2c[a].first].order < cl_[r2c[b].first].order; });
+    const std::vector<int> cin = cl_[c].in;  // add_edge below never adds in-edges to c
+    for (int ein : cin) {
+      const int nold = ed_[ein].n1;
+      const Block src = ed_[ein].a;
+      for (int i = 0; i < rsz;) {
+        const int nnew = r2c[rp[i]].first;
+        int inext = i + 1;
+        while (inext < rsz && r2c[rp[inext]].first == nnew) ++inext;
+        const int rinc = inext - i;
+        int e = find_out(nold, nnew);
+        if (e < 0) {
+          if (nold == nnew) throw Error(kErrState, "reassign: missing self block");
+          e = add_edge(nold, nnew, &zero);
+        }
+        Grow& g = touch(e);
+        typename Grow::App a;
+        a.src = src.p; a.lds = src.ld; a.row0 = g.rows;
+        for (int t = i; t < inext; ++t) a.idx.push_back(cols + rp[t]);
+        g.app.push_back(std::move(a));
+        g.rows += rinc;
+        if (nold == nnew) {
+          Op o;
+          o.kind = OpReassign; o.level = ilvl_; o.c = c; o.f = nnew;
+          o.nstart = cl_[nnew].rsize;
+          for (int t = i; t < inext; ++t) o.idx.push_back(cols + rp[t]);
+          alloc_x(nnew, cl_[nnew].rsize + rinc);
+          ops_.push_back(std::move(o));
+        }
+        i = inext;
+      }
+    }
+    // sizes and zero padding (tree.cpp:394-410)
+    for (int eo : cl_[c].out) {
+      const int n = ed_[eo].n2;
+      if (n == c) continue;
+      const int es = find_out(n, n);
+      cl_[n].rsize = rows_of(es);
+      cl_[n].csize = ed_[es].a.cols;
+      alloc_x(n, cl_[n].rsize);
+      for (int e : cl_[n].in) {
+        if (cl_[ed_[e].n1].elim) continue;
+        if (rows_of(e) != cl_[n].rsize) touch(e).rows = cl_[n].rsize;
+      }
+    }
+  }
+  // (3) materialise: zero the new blocks, copy the old rows, gather the new
+  std::vector<CopyDesc> copies;
+  std::vector<RowGatherDesc<T>> rg;
+  std::vector<int> rgcnt;
+  for (auto& kv : grow) {
+    Grow& g = kv.second;
+    if (g.rows == g.base.rows && g.app.empty()) continue;
+    Block nb = new_block(g.rows, g.base.cols);
+    zero.push_back(nb);
+    if (g.base.rows > 0 && g.base.cols > 0)
+      copies.push_back(CopyDesc{g.base.p, nb.p, g.base.ld, nb.ld, g.base.rows, g.base.cols, 0, 0});
+    for (auto& a : g.app) {
+      const int* didx = stager_.upload(perm_, a.idx);
+      rg.push_back(RowGatherDesc<T>{a.src, nb.p, didx, a.lds, nb.ld, a.row0, (int)a.idx.size(), nb.cols, 0});
+      rgcnt.push_back(rowgather_tiles((int)a.idx.size(), nb.cols));
+    }
+    ed_[kv.first].a = nb;
+  }
+  init_blocks(zero, 0);
+  copy_blocks(copies);
+  if (!rg.empty()) {
+    std::vector<int> st;
+    const int nt = prefix_tiles(rgcnt, st);
+    const RowGatherDesc<T>* dd = stager_.push(rg);
+    const int* dst = stager_.push(st);
+    timed("copy", 0.0, 0.0, [&] { launch_rowgather<T>(stream_, dd, dst, (int)rg.size(), nt); });
+  }
+}
+
 template <typename T>
 void Engine<T>::merge_all() {  // tree.cpp:872-881, 414-475
   cur_++;
@@ -1225,7 +1481,8 @@ int Engine<T>::factorize() {  // tree.cpp:1309-1437
     eliminate_level(lev);
     if (timing_) sync();
     const double t2 = wall();
-    if (!square_ && ilvl_ < nlevels_ - 1) throw Error(kErrState, "least-squares factorization is not available in this build");
+    if (!square_ && ilvl_ < nlevels_ - 1) reassign_level(lev);
+    if (timing_) sync();
     const double t3 = wall();
     std::vector<int> itf;
     for (int s : bottoms_[cur_])
@@ -1235,6 +1492,7 @@ int Engine<T>::factorize() {  // tree.cpp:1309-1437
     const double t4 = wall();
     if (ilvl_ >= skip_ && ilvl_ < nlevels_ - 2 && tol_ != 0) {
       if (!scale_) throw Error(kErrState, "--scale 0 sparsification is not available in this build");
+      if (!square_) sparsify_extra_level(itf);
       sparsify_level(itf);
     }
     if (timing_) sync();
@@ -1287,13 +1545,35 @@ int Engine<T>::factorize() {  // tree.cpp:1309-1437
 // -> SplitD -> Merge, backward stages in the reverse order (tree.cpp:1440-1466).
 template <typename T>
 void Engine<T>::build_solve_program() {
+  layout_x();
+  // temporary segments for SplitD moves whose ranges overlap (least-squares
+  // clusters with extra rows)
+  std::map<const Op*, int64_t> split_tmp;
+  for (auto& o : ops_)
+    if (o.kind == OpSplitD && o.crows > o.ccols) { split_tmp[&o] = xsize_; xsize_ += o.crows - o.ccols; }
   SPQ_HIP(hipMalloc((void**)&xbuf_, sizeof(T) * std::max<int64_t>(xsize_, 1)));
+  if (!square_) SPQ_HIP(hipMalloc((void**)&xtbuf_, sizeof(T) * std::max<int64_t>(xsize_, 1)));
   SPQ_HIP(hipMalloc((void**)&bin_, sizeof(double) * std::max(nrows_, 1)));
   SPQ_HIP(hipMalloc((void**)&xout_, sizeof(double) * std::max(ncols_, 1)));
   std::vector<Seg> segs;
   std::vector<R12Desc<T>> r12;
-  struct HStage { int type; std::vector<ReflDesc<T>> refl; std::vector<BwdDesc<T>> bwd; std::vector<MoveDesc> mv; int mmax = 0; };
-  std::vector<HStage> hf, hb;
+  // buf: the x buffer a stage works on (0 = x, 1 = xt); moves copy sbuf -> buf
+  struct HStage {
+    int type;
+    std::vector<ReflDesc<T>> refl;
+    std::vector<BwdDesc<T>> bwd;
+    std::vector<MoveDesc> mv;
+    int mmax = 0, buf = 0, sbuf = 0;
+  };
+  std::vector<HStage> hf, hb, htf, htb, hpf;
+  auto push = [](std::vector<HStage>& v, HStage&& st) {
+    if (!st.refl.empty() || !st.bwd.empty() || !st.mv.empty()) v.push_back(std::move(st));
+  };
+  auto stage = [](int type, int buf, int sbuf = -1) {
+    HStage st;
+    st.type = type; st.buf = buf; st.sbuf = sbuf < 0 ? buf : sbuf;
+    return st;
+  };
   auto refl_of = [&](const Op& o, int trans) {
     ReflDesc<T> d{};
     d.V = o.V; d.Tb = o.Tb; d.ldv = o.ldv; d.m = o.m; d.k = o.k; d.nb = nb_;
@@ -1302,10 +1582,10 @@ void Engine<T>::build_solve_program() {
     solve_bytes_ += sizeof(T) * ((double)o.m * o.k + (double)nb_ * o.k) + 2.0 * sizeof(T) * o.m;
     return d;
   };
-  auto bwd_of = [&](const Op& o) {
+  auto bwd_of = [&](const Op& o, int trans) {
     BwdDesc<T> d{};
     d.R = o.R; d.ldr = o.ldr; d.k = o.kr; d.xoff = cl_[o.c].xoff;
-    d.blk0 = (int)r12.size(); d.nblk = (int)o.r12.size(); d.trans = 0;
+    d.blk0 = (int)r12.size(); d.nblk = (int)o.r12.size(); d.trans = trans;
     for (auto& q : o.r12) {
       r12.push_back(R12Desc<T>{q.first.p, q.first.ld, q.second.second, cl_[q.second.first].xoff});
       solve_bytes_ += sizeof(T) * (double)o.kr * q.second.second;
@@ -1313,26 +1593,90 @@ void Engine<T>::build_solve_program() {
     solve_bytes_ += sizeof(T) * 0.5 * o.kr * o.kr + 2.0 * sizeof(T) * o.kr;
     return d;
   };
-  for (int lvl = 0; lvl < nlevels_; ++lvl) {
-    std::vector<const Op*> qr, sc, od, sd, mg;
+  auto scaled_of = [&](const Op& o, int trans) {
+    BwdDesc<T> d{};
+    d.R = o.R; d.ldr = o.ldr; d.k = o.k; d.xoff = cl_[o.c].xoff; d.trans = trans;
+    solve_bytes_ += sizeof(T) * 0.5 * o.k * o.k + 2.0 * sizeof(T) * o.k;
+    return d;
+  };
+  // SplitD (operations.cpp:120-131): forward x_f = x[rk:cc), x[rk:rk+rest) =
+  // x[cc:cr); backward x[rk:cc) = x_f after x[cc:cr) = x[rk:rk+rest).  The
+  // overlapping shift of the rest rows goes through a temporary segment, so
+  // each direction is two stages (A then B).
+  auto splitd_fwd = [&](const Op* o, HStage& A, HStage& B) {
+    const int64_t xc = cl_[o->c].xoff, xf = cl_[o->f].xoff;
+    const int n = o->ccols - o->rank, rest = o->crows - o->ccols;
+    if (n > 0) A.mv.push_back(MoveDesc{xc + o->rank, xf, n, 0});
+    if (rest > 0) {
+      const int64_t t = split_tmp.at(o);
+      A.mv.push_back(MoveDesc{xc + o->ccols, t, rest, 0});
+      B.mv.push_back(MoveDesc{t, xc + o->rank, rest, 0});
+    }
+  };
+  auto splitd_bwd = [&](const Op* o, HStage& A, HStage& B) {
+    const int64_t xc = cl_[o->c].xoff, xf = cl_[o->f].xoff;
+    const int n = o->ccols - o->rank, rest = o->crows - o->ccols;
+    if (rest > 0) {
+      const int64_t t = split_tmp.at(o);
+      A.mv.push_back(MoveDesc{xc + o->rank, t, rest, 0});
+      B.mv.push_back(MoveDesc{t, xc + o->ccols, rest, 0});
+    }
+    if (n > 0) B.mv.push_back(MoveDesc{xf, xc + o->rank, n, 0});
+  };
+  // Merge: children -> parent (rows on the way up, cols in the normal-
+  // equation forward pass) and parent -> children
+  auto merge_up = [&](const Op* o, HStage& S, bool rows) {
+    int64_t k = 0;
+    for (int ch : cl_[o->c].children) {
+      const int len = rows ? cl_[ch].rsize : cl_[ch].csize;
+      if (len > 0) S.mv.push_back(MoveDesc{cl_[ch].xoff, cl_[o->c].xoff + k, len, 0});
+      k += len;
+    }
+  };
+  auto merge_down = [&](const Op* o, HStage& S, bool rows) {
+    int64_t k = 0;
+    for (int ch : cl_[o->c].children) {
+      const int len = rows ? cl_[ch].rsize : cl_[ch].csize;
+      if (len > 0) S.mv.push_back(MoveDesc{cl_[o->c].xoff + k, cl_[ch].xoff, len, 0});
+      k += len;
+    }
+  };
+  // Reassign: x_f[nstart + t] = x_c[idx[t]] (forward) and back
+  auto reassign = [&](const Op* o, HStage& S, bool fwd) {
+    for (size_t t = 0; t < o->idx.size(); ++t) {
+      const int64_t c = cl_[o->c].xoff + o->idx[t], f = cl_[o->f].xoff + o->nstart + (int64_t)t;
+      S.mv.push_back(fwd ? MoveDesc{c, f, 1, 0} : MoveDesc{f, c, 1, 0});
+    }
+  };
+  struct LevelOps { std::vector<const Op*> qr, sc, od, sd, mg, fq, ra; };
+  auto level_ops = [&](int lvl) {
+    LevelOps L;
     for (auto& o : ops_) {
       if (o.level != lvl) continue;
-      if (o.kind == OpQR) qr.push_back(&o);
-      else if (o.kind == OpScaleD) sc.push_back(&o);
-      else if (o.kind == OpOrthoD) od.push_back(&o);
-      else if (o.kind == OpSplitD) sd.push_back(&o);
-      else if (o.kind == OpMerge) mg.push_back(&o);
-      // OpFineQR: tau = 0, no forward work
+      switch (o.kind) {
+        case OpQR: L.qr.push_back(&o); break;
+        case OpScaleD: L.sc.push_back(&o); break;
+        case OpOrthoD: L.od.push_back(&o); break;
+        case OpSplitD: L.sd.push_back(&o); break;
+        case OpMerge: L.mg.push_back(&o); break;
+        case OpFineQR: L.fq.push_back(&o); break;
+        case OpReassign: L.ra.push_back(&o); break;
+      }
     }
+    return L;
+  };
+  // ---- x <- M^{-1} x: ops forward, then backward (tree.cpp:1450-1457)
+  for (int lvl = 0; lvl < nlevels_; ++lvl) {
+    LevelOps L = level_ops(lvl);
     // QR forward waves: ops sharing an x segment keep their order
-    std::vector<int> wv(qr.size(), 0);
+    std::vector<int> wv(L.qr.size(), 0);
     int nw = 0;
-    for (size_t i = 0; i < qr.size(); ++i) {
+    for (size_t i = 0; i < L.qr.size(); ++i) {
       for (size_t j = 0; j < i; ++j) {
         bool hit = false;
-        for (auto& a : qr[i]->segs) {
-          for (auto& b : qr[j]->segs)
-            if (a.first == b.first) { hit = true; break; }
+        for (auto& a2 : L.qr[i]->segs) {
+          for (auto& b2 : L.qr[j]->segs)
+            if (a2.first == b2.first) { hit = true; break; }
           if (hit) break;
         }
         if (hit) wv[i] = std::max(wv[i], wv[j] + 1);
@@ -1340,84 +1684,127 @@ void Engine<T>::build_solve_program() {
       nw = std::max(nw, wv[i] + 1);
     }
     for (int w = 0; w < nw; ++w) {
-      HStage s; s.type = 0;
-      for (size_t i = 0; i < qr.size(); ++i)
-        if (wv[i] == w) { s.refl.push_back(refl_of(*qr[i], 1)); s.mmax = std::max(s.mmax, qr[i]->m); }
-      hf.push_back(std::move(s));
+      HStage st = stage(0, 0);
+      for (size_t i = 0; i < L.qr.size(); ++i)
+        if (wv[i] == w) { st.refl.push_back(refl_of(*L.qr[i], 1)); st.mmax = std::max(st.mmax, L.qr[i]->m); }
+      push(hf, std::move(st));
     }
-    HStage ssc; ssc.type = 0;
-    for (auto o : sc) { ssc.refl.push_back(refl_of(*o, 1)); ssc.mmax = std::max(ssc.mmax, o->m); }
-    if (!ssc.refl.empty()) hf.push_back(std::move(ssc));
-    HStage sod; sod.type = 0;
-    for (auto o : od) if (o->k > 0) { sod.refl.push_back(refl_of(*o, 1)); sod.mmax = std::max(sod.mmax, o->m); }
-    if (!sod.refl.empty()) hf.push_back(std::move(sod));
-    HStage ssd; ssd.type = 2;
-    for (auto o : sd) {
-      const int n = o->ccols - o->rank;
-      if (n > 0) ssd.mv.push_back(MoveDesc{cl_[o->c].xoff + o->rank, cl_[o->f].xoff, n, 0});
-    }
-    if (!ssd.mv.empty()) hf.push_back(std::move(ssd));
-    HStage smg; smg.type = 2;
-    for (auto o : mg) {
-      int64_t k = 0;
-      for (int ch : cl_[o->c].children) {
-        if (cl_[ch].rsize > 0) smg.mv.push_back(MoveDesc{cl_[ch].xoff, cl_[o->c].xoff + k, cl_[ch].rsize, 0});
-        k += cl_[ch].rsize;
-      }
-    }
-    if (!smg.mv.empty()) hf.push_back(std::move(smg));
+    HStage sra = stage(2, 0);
+    for (auto o : L.ra) reassign(o, sra, true);
+    push(hf, std::move(sra));
+    HStage ssc = stage(0, 0);
+    for (auto o : L.sc) { ssc.refl.push_back(refl_of(*o, 1)); ssc.mmax = std::max(ssc.mmax, o->m); }
+    push(hf, std::move(ssc));
+    HStage sod = stage(0, 0);
+    for (auto o : L.od) if (o->k > 0) { sod.refl.push_back(refl_of(*o, 1)); sod.mmax = std::max(sod.mmax, o->m); }
+    push(hf, std::move(sod));
+    HStage sa = stage(2, 0), sb = stage(2, 0);
+    for (auto o : L.sd) splitd_fwd(o, sa, sb);
+    push(hf, std::move(sa));
+    push(hf, std::move(sb));
+    HStage smg = stage(2, 0);
+    for (auto o : L.mg) merge_up(o, smg, true);
+    push(hf, std::move(smg));
   }
   for (int lvl = nlevels_ - 1; lvl >= 0; --lvl) {
-    std::vector<const Op*> qr, sc, od, sd, mg, fq;
-    for (auto& o : ops_) {
-      if (o.level != lvl) continue;
-      if (o.kind == OpQR) qr.push_back(&o);
-      else if (o.kind == OpScaleD) sc.push_back(&o);
-      else if (o.kind == OpOrthoD) od.push_back(&o);
-      else if (o.kind == OpSplitD) sd.push_back(&o);
-      else if (o.kind == OpMerge) mg.push_back(&o);
-      else fq.push_back(&o);
-    }
-    HStage smg; smg.type = 2;
-    for (auto o : mg) {
-      int64_t k = 0;
-      for (int ch : cl_[o->c].children) {
-        if (cl_[ch].csize > 0) smg.mv.push_back(MoveDesc{cl_[o->c].xoff + k, cl_[ch].xoff, cl_[ch].csize, 0});
-        k += cl_[ch].csize;
-      }
-    }
-    if (!smg.mv.empty()) hb.push_back(std::move(smg));
+    LevelOps L = level_ops(lvl);
+    HStage smg = stage(2, 0);
+    for (auto o : L.mg) merge_down(o, smg, false);
+    push(hb, std::move(smg));
     // sparsification ops in reverse: grouped by kind, or, when order-1.5
     // fine-cluster QRs read neighbouring interfaces, wave by wave (latest
     // wave first: FineQR, SplitD, OrthoD), which keeps the tape's order
     // between adjacent interfaces
     int wmax = 0;
-    for (auto o : sd) wmax = std::max(wmax, o->wave);
-    for (int w = fq.empty() ? 0 : wmax; w >= 0; --w) {
-      const bool all = fq.empty();
-      HStage sfq; sfq.type = 1;
-      for (auto o : fq) if (o->wave == w) sfq.bwd.push_back(bwd_of(*o));
-      if (!sfq.bwd.empty()) hb.push_back(std::move(sfq));
-      HStage ssd; ssd.type = 2;
-      for (auto o : sd) {
-        if (!all && o->wave != w) continue;
-        const int n = o->ccols - o->rank;
-        if (n > 0) ssd.mv.push_back(MoveDesc{cl_[o->f].xoff, cl_[o->c].xoff + o->rank, n, 0});
-      }
-      if (!ssd.mv.empty()) hb.push_back(std::move(ssd));
-      HStage sod; sod.type = 0;
-      for (auto o : od) {
+    for (auto o : L.sd) wmax = std::max(wmax, o->wave);
+    for (int w = L.fq.empty() ? 0 : wmax; w >= 0; --w) {
+      const bool all = L.fq.empty();
+      HStage sfq = stage(1, 0);
+      for (auto o : L.fq) if (o->wave == w) sfq.bwd.push_back(bwd_of(*o, 0));
+      push(hb, std::move(sfq));
+      HStage sa = stage(2, 0), sb = stage(2, 0);
+      for (auto o : L.sd) if (all || o->wave == w) splitd_bwd(o, sa, sb);
+      push(hb, std::move(sa));
+      push(hb, std::move(sb));
+      HStage sod = stage(0, 0);
+      for (auto o : L.od) {
         if (!all && o->wave != w) continue;
         if (o->k > 0) { sod.refl.push_back(refl_of(*o, 0)); sod.mmax = std::max(sod.mmax, o->m); }
       }
-      if (!sod.refl.empty()) hb.push_back(std::move(sod));
+      push(hb, std::move(sod));
     }
-    HStage ssc; ssc.type = 1;
-    for (auto o : sc) ssc.bwd.push_back(BwdDesc<T>{o->R, o->ldr, o->k, cl_[o->c].xoff, 0, 0, 0, 0});
-    if (!ssc.bwd.empty()) hb.push_back(std::move(ssc));
-    HStage sqr; sqr.type = 1;
-    for (auto o : qr) sqr.bwd.push_back(bwd_of(*o));
-    if (!sqr.bwd.empty()) hb.push_back(std::move(sqr));
+    HStage ssc = stage(1, 0);
+    for (auto o : L.sc) ssc.bwd.push_back(scaled_of(*o, 0));
+    push(hb, std::move(ssc));
+    // Reassign has no backward step on x (operations.cpp:70)
+    HStage sqr = stage(1, 0);
+    for (auto o : L.qr) sqr.bwd.push_back(bwd_of(*o, 0));
+    push(hb, std::move(sqr));
+  }
+  if (!square_) {
+    // ---- normal equations (tree.cpp:1471-1517): tops forward on xt, tops
+    // backward (permutations only) on xt, xt -> x for the leaves, ops
+    // forward (permutations only) on x, then the backward program above
+    for (int lvl = 0; lvl < nlevels_; ++lvl) {
+      LevelOps L = level_ops(lvl);
+      HStage sqr = stage(1, 1);  // R^{-T}, then x_n -= A_n^T x_c (toperations.cpp:12-24)
+      for (auto o : L.qr) sqr.bwd.push_back(bwd_of(*o, 1));
+      push(htf, std::move(sqr));
+      HStage ssc = stage(1, 1);
+      for (auto o : L.sc) ssc.bwd.push_back(scaled_of(*o, 1));
+      push(htf, std::move(ssc));
+      int wmax = 0;
+      for (auto o : L.sd) wmax = std::max(wmax, o->wave);
+      for (int w = 0; w <= (L.fq.empty() ? 0 : wmax); ++w) {
+        const bool all = L.fq.empty();
+        HStage sod = stage(0, 1);
+        for (auto o : L.od) {
+          if (!all && o->wave != w) continue;
+          if (o->k > 0) { sod.refl.push_back(refl_of(*o, 1)); sod.mmax = std::max(sod.mmax, o->m); }
+        }
+        push(htf, std::move(sod));
+        HStage sa = stage(2, 1), sb = stage(2, 1);
+        for (auto o : L.sd) if (all || o->wave == w) splitd_fwd(o, sa, sb);
+        push(htf, std::move(sa));
+        push(htf, std::move(sb));
+        HStage sfq = stage(1, 1);
+        for (auto o : L.fq) if (o->wave == w) sfq.bwd.push_back(bwd_of(*o, 1));
+        push(htf, std::move(sfq));
+      }
+      HStage smg = stage(2, 1);
+      for (auto o : L.mg) merge_up(o, smg, false);
+      push(htf, std::move(smg));
+    }
+    for (int lvl = nlevels_ - 1; lvl >= 0; --lvl) {
+      LevelOps L = level_ops(lvl);
+      HStage smg = stage(2, 1);
+      for (auto o : L.mg) merge_down(o, smg, true);
+      push(htb, std::move(smg));
+      HStage sa = stage(2, 1), sb = stage(2, 1);
+      for (auto o : L.sd) splitd_bwd(o, sa, sb);
+      push(htb, std::move(sa));
+      push(htb, std::move(sb));
+      HStage sra = stage(2, 1);
+      for (auto o : L.ra) reassign(o, sra, false);
+      push(htb, std::move(sra));
+    }
+    HStage cp = stage(2, 0, 1);
+    for (int c : bottoms_[0])
+      if (cl_[c].rsize_org > 0) cp.mv.push_back(MoveDesc{cl_[c].xoff, cl_[c].xoff, cl_[c].rsize_org, 0});
+    push(htb, std::move(cp));
+    for (int lvl = 0; lvl < nlevels_; ++lvl) {
+      LevelOps L = level_ops(lvl);
+      HStage sra = stage(2, 0);
+      for (auto o : L.ra) reassign(o, sra, true);
+      push(hpf, std::move(sra));
+      HStage sa = stage(2, 0), sb = stage(2, 0);
+      for (auto o : L.sd) splitd_fwd(o, sa, sb);
+      push(hpf, std::move(sa));
+      push(hpf, std::move(sb));
+      HStage smg = stage(2, 0);
+      for (auto o : L.mg) merge_up(o, smg, true);
+      push(hpf, std::move(smg));
+    }
   }
   segs_dev_ = stager_.upload(perm_, segs);
   r12_dev_ = stager_.upload(perm_, r12);
@@ -1441,6 +1828,8 @@ void Engine<T>::build_solve_program() {
       Stage s;
       s.type = h.type;
       s.mmax = h.mmax;
+      s.buf = h.buf;
+      s.sbuf = h.sbuf;
       std::vector<int> cnt, st;
       if (h.type == 0) {
         std::vector<int> want;
@@ -1496,6 +1885,9 @@ void Engine<T>::build_solve_program() {
   };
   finish(hf, fwd_);
   finish(hb, bwd_);
+  finish(htf, tfwd_);
+  finish(htb, tbwd_);
+  finish(hpf, pfwd_);
   n_solve_sync_ = nsync;
   solve_sync_ = (unsigned int*)perm_.alloc(sizeof(unsigned int) * std::max(nsync, 1));
   solve_err_ = (int*)perm_.alloc(256);
@@ -1522,6 +1914,9 @@ void Engine<T>::build_solve_program() {
   };
   upload_stage(hf, fwd_);
   upload_stage(hb, bwd_);
+  upload_stage(htf, tfwd_);
+  upload_stage(htb, tbwd_);
+  upload_stage(hpf, pfwd_);
   if (verbose_ > 1) {
     auto dump = [&](const char* dir, std::vector<HStage>& hs) {
       for (auto& h : hs) {
@@ -1551,13 +1946,23 @@ void Engine<T>::build_solve_program() {
   in_src_ = stager_.upload(perm_, isrc);
   out_src_ = stager_.upload(perm_, osrc);
   out_dst_ = stager_.upload(perm_, odst);
+  if (!square_) {
+    // normal equations: b (length cols) enters xt through the column
+    // permutation (tree.cpp:1475-1480)
+    std::vector<int64_t> tdst;
+    std::vector<int> tsrc;
+    for (int c : bottoms_[0])
+      for (int i = 0; i < cl_[c].csize_org; ++i) { tdst.push_back(cl_[c].xoff + i); tsrc.push_back(cperm_[cl_[c].cstart + i]); }
+    n_tin_ = (int)tdst.size();
+    tin_dst_ = stager_.upload(perm_, tdst);
+    tin_src_ = stager_.upload(perm_, tsrc);
+  }
 }
 
 template <typename T>
-void Engine<T>::run_program(const double* b, double* x, bool prof) {
+void Engine<T>::run_program(const double* b, double* x, bool prof, bool nrml) {
   launch_zero(stream_, sizeof(T), xbuf_, xsize_);
   if (n_solve_sync_ > 0) launch_zero(stream_, 4, solve_sync_, n_solve_sync_);
-  launch_gather_in<T>(stream_, in_dst_, in_src_, n_in_, b, xbuf_);
   std::vector<hipEvent_t> ev;
   auto run = [&](const Stage& s) {
     if (prof) {
@@ -1565,12 +1970,22 @@ void Engine<T>::run_program(const double* b, double* x, bool prof) {
       SPQ_HIP(hipEventCreate(&ev.back()));
       SPQ_HIP(hipEventRecord(ev.back(), stream_));
     }
-    if (s.type == 0) launch_refl<T>(stream_, s.refl, s.wstart, s.nd, s.nwg_total, segs_dev_, xbuf_, s.work, s.mmax);
-    else if (s.type == 1) launch_bwd<T>(stream_, s.bwd, s.wstart, s.nd, s.nwg_total, r12_dev_, xbuf_);
-    else launch_move<T>(stream_, s.mv, s.nd, xbuf_, xbuf_);
+    T* xb = s.buf ? xtbuf_ : xbuf_;
+    if (s.type == 0) launch_refl<T>(stream_, s.refl, s.wstart, s.nd, s.nwg_total, segs_dev_, xb, s.work, s.mmax);
+    else if (s.type == 1) launch_bwd<T>(stream_, s.bwd, s.wstart, s.nd, s.nwg_total, r12_dev_, xb);
+    else launch_move<T>(stream_, s.mv, s.nd, xb, s.sbuf ? xtbuf_ : xbuf_);
   };
-  for (auto& s : fwd_) run(s);
-  for (auto& s : bwd_) run(s);
+  std::vector<const std::vector<Stage>*> progs;
+  if (nrml) {
+    launch_zero(stream_, sizeof(T), xtbuf_, xsize_);
+    launch_gather_in<T>(stream_, tin_dst_, tin_src_, n_tin_, b, xtbuf_);
+    progs = {&tfwd_, &tbwd_, &pfwd_, &bwd_};
+  } else {
+    launch_gather_in<T>(stream_, in_dst_, in_src_, n_in_, b, xbuf_);
+    progs = {&fwd_, &bwd_};
+  }
+  for (auto p : progs)
+    for (auto& s : *p) run(s);
   if (prof) {
     ev.emplace_back();
     SPQ_HIP(hipEventCreate(&ev.back()));
@@ -1581,9 +1996,10 @@ void Engine<T>::run_program(const double* b, double* x, bool prof) {
       float ms = 0;
       SPQ_HIP(hipEventElapsedTime(&ms, ev[i], ev[i + 1]));
       tot += ms;
-      const bool f = i < fwd_.size();
-      const Stage& s = f ? fwd_[i] : bwd_[i - fwd_.size()];
-      fprintf(stderr, "stage %s %3zu type %d ops %5d wgs %5d mmax %6d ms %.4f\n", f ? "fwd" : "bwd", i, s.type, s.nd,
+      size_t j = i, pi = 0;
+      while (j >= progs[pi]->size()) j -= progs[pi++]->size();
+      const Stage& s = (*progs[pi])[j];
+      fprintf(stderr, "stage p%zu %3zu type %d ops %5d wgs %5d mmax %6d ms %.4f\n", pi, j, s.type, s.nd,
               s.type == 2 ? s.nd : s.nwg_total, s.mmax, ms);
     }
     fprintf(stderr, "stages total ms %.3f\n", tot);
@@ -1595,6 +2011,7 @@ void Engine<T>::run_program(const double* b, double* x, bool prof) {
 template <typename T>
 void Engine<T>::solve_dev(const double* b, double* x) {
   if (!factored_) throw Error(kErrState, "solve before factorize");
+  if (!square_) throw Error(kErrState, "solve: rectangular (least-squares) tree; use solve_nrml");
   // b is taken by value (tree.cpp:1440): copy into the program's fixed input,
   // replay the captured apply, copy the fixed output out.
   SPQ_HIP(hipMemcpyAsync(bin_, b, sizeof(double) * nrows_, hipMemcpyDeviceToDevice, stream_));
@@ -1612,6 +2029,29 @@ void Engine<T>::solve_dev(const double* b, double* x) {
     hipGraphDestroy(g);
   }
   timed("apply", 0.0, solve_bytes_, [&] { SPQ_HIP(hipGraphLaunch(graph_exec_fwd_, stream_)); });
+  SPQ_HIP(hipMemcpyAsync(x, xout_, sizeof(double) * ncols_, hipMemcpyDeviceToDevice, stream_));
+}
+
+// x = (W^T W)^{-1} b with the R-only factors of a least-squares tree
+// (tree.cpp:1471-1517); b and x have length cols, b may alias x.
+template <typename T>
+void Engine<T>::solve_nrml_dev(const double* b, double* x) {
+  if (!factored_) throw Error(kErrState, "solve_nrml before factorize");
+  if (square_) throw Error(kErrState, "solve_nrml: the tree was factored as a square system (set_square(0) first)");
+  SPQ_HIP(hipMemcpyAsync(bin_, b, sizeof(double) * ncols_, hipMemcpyDeviceToDevice, stream_));
+  if (stage_prof_ || !solve_graph_) {
+    timed("apply", 0.0, solve_bytes_, [&] { run_program(bin_, xout_, stage_prof_, true); });
+  } else {
+    if (!graph_exec_nrml_) {
+      hipGraph_t g = nullptr;
+      SPQ_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+      run_program(bin_, xout_, false, true);
+      SPQ_HIP(hipStreamEndCapture(stream_, &g));
+      SPQ_HIP(hipGraphInstantiate(&graph_exec_nrml_, g, nullptr, nullptr, 0));
+      hipGraphDestroy(g);
+    }
+    timed("apply", 0.0, solve_bytes_, [&] { SPQ_HIP(hipGraphLaunch(graph_exec_nrml_, stream_)); });
+  }
   SPQ_HIP(hipMemcpyAsync(x, xout_, sizeof(double) * ncols_, hipMemcpyDeviceToDevice, stream_));
 }
 

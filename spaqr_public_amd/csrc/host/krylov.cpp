@@ -138,4 +138,83 @@ KrylovResult gmres_dev(EngineBase& M, const DevMatrix& A, const double* b, doubl
   return res;
 }
 
+// Preconditioned CGLS on the normal equations (is.cpp:275-346): the
+// preconditioner is the normal-equation apply of the tree, the stop test is
+// ||A^T r||^2 < tol^2 ||A^T b||^2.  FP64 throughout; b (m), x (n) on the device.
+KrylovResult cgls_dev(EngineBase& M, const DevMatrix& A, const double* b, double* x, int iters, double tol,
+                      int verbose) {
+  KrylovResult res;
+  hipStream_t s = M.stream();
+  const int m = A.m, n = A.n;
+  if (!A.rpt) throw Error(kErrArg, "cgls: the matrix was uploaded without its transpose");
+  const double t0 = wall();
+  double *r = nullptr, *nr = nullptr, *tmp = nullptr, *p = nullptr, *z = nullptr, *h = nullptr, *hh = nullptr;
+  SPQ_HIP(hipMalloc((void**)&r, sizeof(double) * std::max(m, 1)));
+  SPQ_HIP(hipMalloc((void**)&tmp, sizeof(double) * std::max(m, 1)));
+  SPQ_HIP(hipMalloc((void**)&nr, sizeof(double) * std::max(n, 1)));
+  SPQ_HIP(hipMalloc((void**)&p, sizeof(double) * std::max(n, 1)));
+  SPQ_HIP(hipMalloc((void**)&z, sizeof(double) * std::max(n, 1)));
+  SPQ_HIP(hipMalloc((void**)&h, sizeof(double) * 4));
+  SPQ_HIP(hipHostMalloc((void**)&hh, sizeof(double) * 4, hipHostMallocDefault));
+  struct Free {
+    std::vector<void*> d;
+    void* hp;
+    ~Free() { for (void* q : d) hipFree(q); hipHostFree(hp); }
+  } fr{{r, tmp, nr, p, z, h}, hh};
+  auto dot = [&](int len, const double* u, const double* v) {
+    launch_dots(s, len, 1, u, len, v, h);
+    SPQ_HIP(hipMemcpyAsync(hh, h, sizeof(double), hipMemcpyDeviceToHost, s));
+    SPQ_HIP(hipStreamSynchronize(s));
+    return hh[0];
+  };
+  auto At = [&](const double* in, double* out) { launch_spmv(s, n, A.rpt, A.cit, A.vt, in, out, 1.0, 0.0); };
+  auto precond = [&](const double* in, double* out) {
+    const double a = wall();
+    M.solve_nrml_dev(in, out);
+    if (verbose) { SPQ_HIP(hipStreamSynchronize(s)); res.t_precond += wall() - a; }
+    res.n_precond++;
+  };
+  // r = b - A x ; nres = A^T r ; A^T b
+  SPQ_HIP(hipMemcpyAsync(r, b, sizeof(double) * m, hipMemcpyDeviceToDevice, s));
+  launch_spmv(s, m, A.rp, A.ci, A.v, x, r, -1.0, 1.0);
+  At(r, nr);
+  At(b, z);
+  const double rhsn2 = dot(n, z, z);
+  if (rhsn2 == 0.0) {
+    SPQ_HIP(hipMemsetAsync(x, 0, sizeof(double) * n, s));
+    SPQ_HIP(hipStreamSynchronize(s));
+    res.t_total = wall() - t0;
+    return res;
+  }
+  const double thr = tol * tol * rhsn2;
+  double rn2 = dot(n, nr, nr);
+  int i = 0;
+  if (rn2 >= thr) {
+    precond(nr, p);
+    double absNew = dot(n, nr, p);
+    while (i < iters) {
+      launch_spmv(s, m, A.rp, A.ci, A.v, p, tmp, 1.0, 0.0);
+      const double tn = dot(m, tmp, tmp);
+      const double alpha = absNew / tn;
+      launch_axpby(s, n, alpha, p, 1.0, x);
+      launch_axpby(s, m, -alpha, tmp, 1.0, r);
+      At(r, nr);
+      rn2 = dot(n, nr, nr);
+      if (verbose > 1) printf("%d %3.2e \n", i, std::sqrt(rn2 / rhsn2));
+      if (rn2 < thr) break;
+      precond(nr, z);
+      const double absOld = absNew;
+      absNew = dot(n, nr, z);
+      const double beta = absNew / absOld;
+      launch_axpby(s, n, 1.0, z, beta, p);
+      ++i;
+    }
+  }
+  SPQ_HIP(hipStreamSynchronize(s));
+  res.iters = i;
+  res.est = std::sqrt(rn2 / rhsn2);
+  res.t_total = wall() - t0;
+  return res;
+}
+
 }  // namespace spq

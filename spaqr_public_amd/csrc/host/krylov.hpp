@@ -29,4 +29,10 @@ struct KrylovResult {
 KrylovResult gmres_dev(EngineBase& M, const DevMatrix& A, const double* b, double* x, int iters, int restart,
                        double tol, int verbose);
 
+// Preconditioned CGLS on the normal equations (is.cpp:275-346): stop when
+// ||A^T r||^2 < tol^2 ||A^T b||^2; est = ||A^T r|| / ||A^T b||.  b has length
+// m, x length n (device vectors); A must carry its transpose.
+KrylovResult cgls_dev(EngineBase& M, const DevMatrix& A, const double* b, double* x, int iters, double tol,
+                      int verbose);
+
 }  // namespace spq

@@ -75,6 +75,53 @@ void launch_init_blocks(hipStream_t s, int elem, const CopyDesc* d, const int* t
   else hipLaunchKernelGGL((k_init_blocks<float>), g, b, 0, s, d, tstart, nd);
 }
 
+// Row norms of row ranges of blocks: one thread per row, a tile of 256 rows
+// per workgroup (tiles: exclusive prefix sums over descriptors).
+template <typename T>
+__global__ __launch_bounds__(256) void k_rownorm(const RowNormDesc<T>* __restrict__ d, const int* __restrict__ tstart,
+                                                 int nd) {
+  const int p = find_owner(tstart, nd, blockIdx.x);
+  const RowNormDesc<T> q = d[p];
+  const int i = (blockIdx.x - tstart[p]) * 256 + threadIdx.x;
+  if (i >= q.nrows) return;
+  double s = 0;
+  for (int j = 0; j < q.ncols; ++j) {
+    const double v = (double)q.A[(q.row0 + i) + (int64_t)j * q.ld];
+    s += v * v;
+  }
+  q.out[i] = sqrt(s);
+}
+// Row gather: one tile = 32 rows x 32 columns, one thread per (row, column-lane).
+template <typename T>
+__global__ __launch_bounds__(256) void k_rowgather(const RowGatherDesc<T>* __restrict__ d,
+                                                   const int* __restrict__ tstart, int nd) {
+  const int p = find_owner(tstart, nd, blockIdx.x);
+  const RowGatherDesc<T> q = d[p];
+  const int ntc = (q.cols + 31) / 32;
+  const int t = blockIdx.x - tstart[p];
+  const int r0 = (t / ntc) * 32, c0 = (t % ntc) * 32;
+  for (int e = threadIdx.x; e < 32 * 32; e += 256) {
+    const int r = r0 + (e & 31), c = c0 + (e >> 5);
+    if (r < q.nrows && c < q.cols)
+      q.dst[(q.row0 + r) + (int64_t)c * q.ldd] = q.src[q.idx[r] + (int64_t)c * q.lds];
+  }
+}
+int rowgather_tiles(int nrows, int cols) { return ((nrows + 31) / 32) * ((cols + 31) / 32); }
+template <typename T>
+void launch_rownorm(hipStream_t s, const RowNormDesc<T>* d, const int* tstart, int nd, int ntiles) {
+  if (nd <= 0 || ntiles <= 0) return;
+  hipLaunchKernelGGL((k_rownorm<T>), dim3(ntiles), dim3(256), 0, s, d, tstart, nd);
+}
+template <typename T>
+void launch_rowgather(hipStream_t s, const RowGatherDesc<T>* d, const int* tstart, int nd, int ntiles) {
+  if (nd <= 0 || ntiles <= 0) return;
+  hipLaunchKernelGGL((k_rowgather<T>), dim3(ntiles), dim3(256), 0, s, d, tstart, nd);
+}
+template void launch_rownorm<double>(hipStream_t, const RowNormDesc<double>*, const int*, int, int);
+template void launch_rownorm<float>(hipStream_t, const RowNormDesc<float>*, const int*, int, int);
+template void launch_rowgather<double>(hipStream_t, const RowGatherDesc<double>*, const int*, int, int);
+template void launch_rowgather<float>(hipStream_t, const RowGatherDesc<float>*, const int*, int, int);
+
 template <typename T>
 __global__ void k_zero(T* __restrict__ p, int64_t n) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;

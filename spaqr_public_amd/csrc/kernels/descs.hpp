@@ -150,6 +150,24 @@ struct R12Desc {
   int64_t xoff;
 };
 
+// Least-squares row surgery (tree.cpp:319-411).
+// RowNormDesc: out[i] = ||A(row0 + i, 0:ncols)||_2 for i < nrows (FP64).
+template <typename T>
+struct RowNormDesc {
+  const T* A;
+  double* out;
+  int ld, row0, nrows, ncols;
+};
+// RowGatherDesc: dst(row0 + i, 0:cols) = src(idx[i], 0:cols) for i < nrows.
+template <typename T>
+struct RowGatherDesc {
+  const T* src;
+  T* dst;
+  const int* idx;
+  int lds, ldd, row0, nrows, cols;
+  int pad_;
+};
+
 // Vector segment move dst[0:len] = src[0:len] (x-buffer offsets).
 struct MoveDesc {
   int64_t src, dst;

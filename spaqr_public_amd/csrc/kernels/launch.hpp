@@ -12,6 +12,11 @@ namespace spq {
 void launch_copy(hipStream_t s, int elem_src, int elem_dst, const CopyDesc* d, const int* tstart, int nd, int ntiles);
 void launch_init_blocks(hipStream_t s, int elem, const CopyDesc* d, const int* tstart, int nd, int ntiles);
 void launch_zero(hipStream_t s, int elem, void* p, int64_t n);
+int rowgather_tiles(int nrows, int cols);
+template <typename T>
+void launch_rownorm(hipStream_t s, const RowNormDesc<T>* d, const int* tstart, int nd, int ntiles);
+template <typename T>
+void launch_rowgather(hipStream_t s, const RowGatherDesc<T>* d, const int* tstart, int nd, int ntiles);
 
 // gemm.hip
 int gemm_tiles(int m, int n);

@@ -139,7 +139,7 @@ def config_matrix(name: str, **kw) -> sp.csc_matrix:
         n = int(name.split("_")[1])
         return high_contrast_diffusion(n, 3, 8, kw.get("seed", 0))
     if name.startswith("ls_lapgrad_"):
-        n = int(name.split("_")[1])
+        n = int(name.split("_")[2])
         return laplacian_gradient_ls(n, kw.get("mu", 1e-2))
     raise KeyError(name)
 

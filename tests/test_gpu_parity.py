@@ -114,3 +114,27 @@ def test_order_1_5(oracle, gpu_lib, name, n, dim, tol):
     if name == "advdiff2d_128_central":
         assert np.array_equal(sg, so)
         assert gpu["oneshot_res"] < 0.8 * 1.338e-3  # below the order-1 README value
+
+
+@pytest.mark.parametrize("n,tol", [(16, 0.0), (32, 0.0), (32, 1e-2), (64, 1e-2)])
+def test_least_squares(oracle, gpu_lib, n, tol):
+    """Least squares [L_h; mu D_h] (configs[3] family): R-only elimination,
+    reassign_rows, sparsify_extra, the normal-equation apply and CGLS on the
+    GPU against the oracle (tree.cpp:319-411, 1260-1307, 1471-1517;
+    is.cpp:275-346)."""
+    from spaqr_public_amd import gen
+    A = gen.config_matrix(f"ls_lapgrad_{n}")
+    gpu, ref = _run_both(oracle, A, n, 2, tol, res=1e-10)
+    _check_tree(gpu, ref)
+    if tol == 0:
+        As, b = gpu["A"], gpu["b"]
+        x = gpu["tree"].solve_nrml(As.T @ b)
+        xe = np.linalg.lstsq(As.toarray(), b, rcond=None)[0]
+        assert np.linalg.norm(x - xe) / np.linalg.norm(xe) < 1e-11
+    assert gpu["oneshot_res"] <= 2.0 * ref["oneshot_res"] + 1e-14
+    assert abs(gpu["iters"] - ref["iters"]) <= 1
+    assert gpu["final_res"] <= 1e-10
+    sg, so = gpu["tree"].sparsified(), ref["tree"].sparsified()
+    assert sg.shape == so.shape
+    if len(so):
+        assert np.mean(sg[:, 4] == so[:, 4]) >= 0.95

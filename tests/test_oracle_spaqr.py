@@ -148,3 +148,31 @@ def test_order_1_5_keeps_coupling(oracle, name, n, dim, tol):
     assert r15["oneshot_res"] < 0.8 * r1["oneshot_res"]
     assert r15["iters"] <= r1["iters"]
     assert r15["final_res"] <= 1e-12
+
+
+@pytest.mark.parametrize("n", [16, 32])
+def test_least_squares_exact_tol0(oracle, n):
+    """Least squares (square = 0): with tol = 0 the R-only factorization is an
+    exact sparse QR, so one normal-equation apply of A^T b is the least-
+    squares solution.  Covers eliminate_cluster's R-only branch, reassign_rows,
+    the tops tape and solve_nrml (tree.cpp:319-411, 780-832, 1471-1517)."""
+    from spaqr_public_amd import gen
+    oracle.set_backend("own")
+    A = gen.config_matrix(f"ls_lapgrad_{n}")
+    r = oracle.run_reference_flow(A, n, 2, 0.0)
+    As, b = r["A"], r["b"]
+    x = r["tree"].solve_nrml(As.T @ b)
+    xe = np.linalg.lstsq(As.toarray(), b, rcond=None)[0]
+    assert np.linalg.norm(x - xe) / np.linalg.norm(xe) < 1e-11
+
+
+def test_least_squares_cgls(oracle):
+    """With sparsification (t = 1e-2) the apply is a preconditioner: CGLS
+    (is.cpp:275-346) reaches 1e-10 on the normal equations in a few steps."""
+    from spaqr_public_amd import gen
+    oracle.set_backend("own")
+    A = gen.config_matrix("ls_lapgrad_32")
+    r = oracle.run_reference_flow(A, 32, 2, 1e-2, res=1e-10)
+    assert 1e-5 < r["oneshot_res"] < 1e-1   # sparsified: not exact
+    assert r["iters"] <= 20
+    assert r["final_res"] <= 1e-10
