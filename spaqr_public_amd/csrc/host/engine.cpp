@@ -892,8 +892,9 @@ void Engine<T>::rrqr_batch(std::vector<RrqrJob>& jobs) {
     if (qp3_reg_ && jb.k <= 64 * qp3r_rows()) {
       G = 1;
       while (G * qp3r_rows() < jb.k) G <<= 1;
-      // large problems: 1024-thread workgroups (one per CU, fewer to meet)
-      nt = iceil(N, 256 / G) > 32 ? 1024 : 256;
+      // very large problems: 1024-thread workgroups (one per CU, fewer to
+      // meet); below ~256 workgroups the 256-thread form is faster
+      nt = iceil(N, 256 / G) > 256 ? 1024 : 256;
       nwg = iceil(N, nt / G);
       if (nwg > qp3r_cap(nt)) G = 0;
     }
@@ -927,7 +928,7 @@ void Engine<T>::rrqr_batch(std::vector<RrqrJob>& jobs) {
   unsigned int* syncs = (unsigned int*)scratch_.alloc(sizeof(unsigned int) * std::max(sync_words, 1));
   for (size_t q = 0; q < qd.size(); ++q) qd[q].sync = syncs + sync_off[q];
   launch_zero(stream_, 4, syncs, sync_words);
-  struct Group { int nt, coop, gmax; std::vector<Qp3Desc<T>> d; };
+  struct Group { int nt, coop, gmax; std::vector<Qp3Desc<T>> d; int nmax = 0; };
   std::vector<Group> groups;
   for (int nt : {0, 256, 1024}) {  // 0: streaming form
     const int cap = nt ? qp3r_cap(nt) : qp3_capacity<T>(kmax);
@@ -967,7 +968,9 @@ void Engine<T>::rrqr_batch(std::vector<RrqrJob>& jobs) {
     const char* fam = gq.nt == 0 ? "qp3s" : (gq.nt == 1024 ? "qp3r1024" : "qp3r256");
     for (auto& d : gq.d) qp3_family_[d.rank_out] = fam;
     timed(fam, 0.0, qby, [&] {
-      rc = gq.nt ? launch_qp3r<T>(stream_, dqd, dst, (int)gq.d.size(), nt, gq.gmax, gq.nt, gq.coop)
+      int nmax = 0;
+      for (auto& d : gq.d) nmax = std::max(nmax, d.N);
+      rc = gq.nt ? launch_qp3r<T>(stream_, dqd, dst, (int)gq.d.size(), nt, gq.gmax, nmax, gq.nt, gq.coop)
                  : launch_qp3<T>(stream_, dqd, dst, (int)gq.d.size(), nt, kmax, gq.coop);
     });
     if (rc != 0) throw Error(kErrHip, std::string("qp3 launch failed: ") + hipGetErrorString((hipError_t)rc));

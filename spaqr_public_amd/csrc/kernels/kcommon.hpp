@@ -129,6 +129,57 @@ __device__ __forceinline__ void group_barrier2(unsigned int* sync, int g, int nw
   __syncthreads();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
 }
+// ---------------------------------------------------------------------------
+// Barriers without cache maintenance.  Values a workgroup hands to the
+// others go through agent-scope atomic stores and loads (coherent across
+// XCDs without writing back or invalidating the L2); a barrier then needs
+// only each wave's stores to have completed before the arrival and relaxed
+// counter traffic.  Measured on MI355X at 12 / 83 / 170 workgroups: 1.6 /
+// 2.3 / 3.3 us per barrier, against 3.8 / 13.9 / 26.5 us for the fenced
+// form above (whose per-workgroup L2 write-back serialises).  Only data
+// accessed through agent_store / agent_load may cross a *_rx barrier.
+template <typename T>
+__device__ __forceinline__ void agent_store(T* p, T v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ __forceinline__ T agent_load(const T* p) {
+  return __hip_atomic_load(const_cast<T*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ bool barrier_wait_rx(unsigned int* ctr, unsigned int target, int* err) {
+  unsigned int spins = 0;
+  while (__hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+    __builtin_amdgcn_s_sleep(2);
+    ++spins;
+    if ((spins & 255u) == 0 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return false;
+    if (spins > (1u << 20)) { atomicOr(err, 1); return false; }
+  }
+  return true;
+}
+__device__ __forceinline__ void group_barrier_rx(unsigned int* ctr, unsigned int target, int* err) {
+  __builtin_amdgcn_s_waitcnt(0);  // this wave's agent stores have completed
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __hip_atomic_fetch_add(ctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    barrier_wait_rx(ctr, target, err);
+  }
+  __syncthreads();
+}
+// Two-level form (fan-in counters as in group_barrier2).
+__device__ __forceinline__ void group_barrier2_rx(unsigned int* sync, int g, int nwg, unsigned int phase, int* err) {
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int grp = g / kFanIn, ngrp = (nwg + kFanIn - 1) / kFanIn;
+    const unsigned int sz = (unsigned int)min(kFanIn, nwg - grp * kFanIn);
+    const unsigned int old =
+        __hip_atomic_fetch_add(sync + 16 * (1 + grp), 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (old + 1 == phase * sz) __hip_atomic_fetch_add(sync, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    barrier_wait_rx(sync, phase * (unsigned int)ngrp, err);
+  }
+  __syncthreads();
+}
+
 // Find the descriptor owning flattened tile `t` given exclusive prefix sums
 // `start[0..n)` (start[0] == 0, nondecreasing).
 __device__ __forceinline__ int find_owner(const int* start, int n, int t) {

@@ -41,13 +41,13 @@ void launch_trsm_block(hipStream_t s, const TrsmDesc<T>* d, const int* tstart, i
 // qp3.hip
 // Register form of the truncated pivoted QR: rows held per lane, resident
 // workgroups for lane-group size G and nt threads per workgroup (256 or
-// 1024), launch (gmax = largest G of the launch).
+// 1024), launch (gmax = largest G, nmax = largest N of the launch).
 int qp3r_rows();
 template <typename T>
 int qp3r_capacity(int G, int nt);
 template <typename T>
-int launch_qp3r(hipStream_t s, const Qp3Desc<T>* d, const int* wstart, int nd, int nwg_total, int gmax, int nt,
-                int coop);
+int launch_qp3r(hipStream_t s, const Qp3Desc<T>* d, const int* wstart, int nd, int nwg_total, int gmax, int nmax,
+                int nt, int coop);
 int qp3_threads();
 template <typename T>
 int qp3_capacity(int kmax);

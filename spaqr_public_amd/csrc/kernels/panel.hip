@@ -84,23 +84,23 @@ __global__ __launch_bounds__(kThreads) void k_panel_r(const PanelDesc<T>* __rest
     }
     if (nwg > 1) {
       const int buf = jj & 1;
-      // written by wave 0, the wave that releases the barrier
+      // handed over through agent-scope atomics (no cache maintenance at the barrier)
       if (tid < kNB) {
         T* __restrict__ pp = d.part + ((int64_t)buf * nwg + g) * (2 * kNB);
-        pp[tid] = sum;
-        if (g == 0) pp[kNB + tid] = rowsh[tid];
+        agent_store(pp + tid, sum);
+        if (g == 0) agent_store(pp + kNB + tid, rowsh[tid]);
       }
       if (nwg > kFanIn) {
-        group_barrier2(d.sync, g, nwg, (unsigned int)(jj + 1), d.err);
+        group_barrier2_rx(d.sync, g, nwg, (unsigned int)(jj + 1), d.err);
       } else {
         target += nwg;
-        group_barrier(d.sync, target, d.err);
+        group_barrier_rx(d.sync, target, d.err);
       }
       if (tid < kNB) {
         T s = T(0);
-        for (int q = 0; q < nwg; ++q) s += d.part[((int64_t)buf * nwg + q) * (2 * kNB) + tid];
+        for (int q = 0; q < nwg; ++q) s += agent_load(d.part + ((int64_t)buf * nwg + q) * (2 * kNB) + tid);
         Dsh[tid] = s;
-        if (g != 0) rowsh[tid] = d.part[(int64_t)buf * nwg * (2 * kNB) + kNB + tid];
+        if (g != 0) rowsh[tid] = agent_load(d.part + (int64_t)buf * nwg * (2 * kNB) + kNB + tid);
       }
     } else if (tid < kNB) {
       Dsh[tid] = sum;

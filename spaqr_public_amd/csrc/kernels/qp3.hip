@@ -25,6 +25,8 @@
 // up sets `err` and the host reports it.
 #include <limits.h>
 
+#include <algorithm>
+
 #include "kcommon.hpp"
 #include "descs.hpp"
 #include "launch.hpp"
@@ -213,6 +215,13 @@ __global__ __launch_bounds__(kQThreads) void k_qp3(const Qp3Desc<T>* __restrict_
 // test are those of the streaming form.
 constexpr int kRC = 32;
 constexpr int kRP = kRC + 1;  // LDS stride per lane's rows (one pad element)
+constexpr int kPosLds = 4096; // largest N whose position table lives in LDS
+template <typename T>
+__host__ __device__ constexpr int vsh_bytes(int G) { return (int)(((sizeof(T) * G * kRP) + 15) / 16 * 16); }
+template <typename T>
+constexpr size_t qp3r_lds(int gmax, int nmax) {
+  return (size_t)vsh_bytes<T>(gmax) + (nmax <= kPosLds ? sizeof(int) * (size_t)nmax : 0);
+}
 
 __device__ __forceinline__ void argmax4(double& v, int& p, int& c, int& w, double ov, int op, int oc, int ow) {
   if (ov > v || (ov == v && op < p)) { v = ov; p = op; c = oc; w = ow; }
@@ -246,6 +255,10 @@ __global__ __launch_bounds__(NT, 1024 / NT) void k_qp3r(const Qp3Desc<T>* __rest
   const bool has = j < N;
   T* vsh = reinterpret_cast<T*>(smem_raw);  // G*kRP entries: pivot column, then reflector
   const T* vmy = vsh + gl * kRP;           // this lane's rows
+  // position -> column table: in LDS (every workgroup applies the same swaps)
+  // when N <= kPosLds, else in memory (written by workgroup 0)
+  const bool lpos = N <= kPosLds;
+  int* ptab = reinterpret_cast<int*>(smem_raw + vsh_bytes<T>(G));
   const int kmin = min(k, N);
   const double tol3z = sqrt(ldexp(1.0, -53));
 
@@ -262,7 +275,11 @@ __global__ __launch_bounds__(NT, 1024 / NT) void k_qp3r(const Qp3Desc<T>* __rest
   s = group_sum(s, G);
   double vn1 = sqrt(s), vn2 = vn1, beta_own = 0;
   int pos = j;
-  if (has && gl == 0) d.pos[j] = j;
+  if (lpos) {
+    for (int q = tid; q < N; q += NT) ptab[q] = q;
+  } else if (has && gl == 0) {
+    agent_store(d.pos + j, j);
+  }
   double bv = -2.0;
   int bp = INT_MAX, bc = -1;
   if (has && gl == 0) { bv = (vn1 == vn1) ? vn1 : -1.0; bp = j; bc = j; }
@@ -288,54 +305,67 @@ __global__ __launch_bounds__(NT, 1024 / NT) void k_qp3r(const Qp3Desc<T>* __rest
       s_bv = v; s_bp = pp; s_bc = cc;
     }
     __syncthreads();
-    // ---- publish the local-best column and its partial
-    if (has && j == s_bc) {
-      T* __restrict__ pc = d.part_col + ((int64_t)buf * nwg + g) * k + rbase;
+    // ---- publish the local-best column and its partial; find the winner
+    const int wbest = s_bc;
+    if (nwg == 1) {
+      // one workgroup: the winner is local and its column goes straight to LDS
+      if (has && j == wbest) {
 #pragma unroll
-      for (int q = 0; q < kRC; ++q)
-        if (rbase + q < k) pc[q] = c[q];
-      if (nwg > 1) __threadfence();
-    }
-    if (tid == 0) {
-      d.part_v[buf * nwg + g] = s_bv;
-      d.part_p[2 * (buf * nwg + g)] = s_bp;
-      d.part_p[2 * (buf * nwg + g) + 1] = s_bc;
-    }
-    if (nwg > kFanIn) {
-      group_barrier2(d.sync, g, nwg, (unsigned int)(i + 1), d.err);
-    } else if (nwg > 1) {
-      target += nwg;
-      group_barrier(d.sync, target, d.err);
-    } else {
-      __syncthreads();
-    }
-    // ---- the same winner in every workgroup
-    {
-      double v = -3.0;
-      int pp = INT_MAX, cc = -1, ww = -1;
-      for (int w = tid; w < nwg; w += NT)
-        argmax4(v, pp, cc, ww, d.part_v[buf * nwg + w], d.part_p[2 * (buf * nwg + w)],
-                d.part_p[2 * (buf * nwg + w) + 1], w);
-#pragma unroll
-      for (int o = kWave / 2; o > 0; o >>= 1) {
-        const double ov = __shfl_xor(v, o, kWave);
-        const int op = __shfl_xor(pp, o, kWave), oc = __shfl_xor(cc, o, kWave), ow = __shfl_xor(ww, o, kWave);
-        argmax4(v, pp, cc, ww, ov, op, oc, ow);
+        for (int q = 0; q < kRC; ++q)
+          if (rbase + q < k) vsh[vslot(rbase + q)] = c[q];
       }
-      if (lane == 0) { redv[tid / kWave] = v; redp[tid / kWave] = pp; redc[tid / kWave] = cc; redw[tid / kWave] = ww; }
-      __syncthreads();
       if (tid == 0) {
-        for (int w = 1; w < NW; ++w) argmax4(v, pp, cc, ww, redv[w], redp[w], redc[w], redw[w]);
-        s_piv = cc; s_pos = pp; s_gw = ww;
-        s_q = d.pos[i];
+        s_piv = wbest; s_pos = s_bp; s_gw = 0;
+        s_q = lpos ? ptab[i] : agent_load(d.pos + i);
       }
       __syncthreads();
+    } else {
+      // everything handed to the other workgroups goes through agent-scope
+      // atomics, so the barrier needs no cache maintenance
+      if (has && j == wbest) {
+        T* __restrict__ pc = d.part_col + ((int64_t)buf * nwg + g) * k + rbase;
+#pragma unroll
+        for (int q = 0; q < kRC; ++q)
+          if (rbase + q < k) agent_store(pc + q, c[q]);
+      }
+      if (tid == 0) {
+        agent_store(d.part_v + buf * nwg + g, s_bv);
+        agent_store(d.part_p + 2 * (buf * nwg + g), s_bp);
+        agent_store(d.part_p + 2 * (buf * nwg + g) + 1, s_bc);
+      }
+      if (nwg > kFanIn) {
+        group_barrier2_rx(d.sync, g, nwg, (unsigned int)(i + 1), d.err);
+      } else {
+        target += nwg;
+        group_barrier_rx(d.sync, target, d.err);
+      }
+      {
+        double v = -3.0;
+        int pp = INT_MAX, cc = -1, ww = -1;
+        for (int w = tid; w < nwg; w += NT)
+          argmax4(v, pp, cc, ww, agent_load(d.part_v + buf * nwg + w), agent_load(d.part_p + 2 * (buf * nwg + w)),
+                  agent_load(d.part_p + 2 * (buf * nwg + w) + 1), w);
+#pragma unroll
+        for (int o = kWave / 2; o > 0; o >>= 1) {
+          const double ov = __shfl_xor(v, o, kWave);
+          const int op = __shfl_xor(pp, o, kWave), oc = __shfl_xor(cc, o, kWave), ow = __shfl_xor(ww, o, kWave);
+          argmax4(v, pp, cc, ww, ov, op, oc, ow);
+        }
+        if (lane == 0) { redv[tid / kWave] = v; redp[tid / kWave] = pp; redc[tid / kWave] = cc; redw[tid / kWave] = ww; }
+        __syncthreads();
+        if (tid == 0) {
+          for (int w = 1; w < NW; ++w) argmax4(v, pp, cc, ww, redv[w], redp[w], redc[w], redw[w]);
+          s_piv = cc; s_pos = pp; s_gw = ww;
+          s_q = lpos ? ptab[i] : agent_load(d.pos + i);
+        }
+        __syncthreads();
+      }
+      const T* __restrict__ pcol = d.part_col + ((int64_t)buf * nwg + s_gw) * k;
+      for (int r = tid; r < k; r += NT) vsh[vslot(r)] = agent_load(pcol + r);
+      __syncthreads();
     }
-    const int piv = s_piv, ppos = s_pos, gw = s_gw, qcol = s_q;
+    const int piv = s_piv, ppos = s_pos, qcol = s_q;
     // ---- Householder vector of the pivot column (identical in every workgroup)
-    const T* __restrict__ pcol = d.part_col + ((int64_t)buf * nwg + gw) * k;
-    for (int r = tid; r < k; r += NT) vsh[vslot(r)] = pcol[r];
-    __syncthreads();
     double ss = 0;
     for (int r = i + 1 + tid; r < k; r += NT) { const double v = (double)vsh[vslot(r)]; ss += v * v; }
     ss = block_sum(ss, sred);
@@ -354,10 +384,11 @@ __global__ __launch_bounds__(NT, 1024 / NT) void k_qp3r(const Qp3Desc<T>* __rest
       vsh[vslot(r)] = v;
       if (g == 0) d.V[r + (int64_t)i * d.ldv] = v;
     }
-    if (g == 0 && tid == 0) {
-      d.tau[i] = (T)tau;
-      d.beta[i] = beta;
-      d.pos[ppos] = qcol;  // LAPACK swap of positions i and ppos
+    if (tid == 0) {
+      // LAPACK swap of positions i and ppos (entry i is never read again)
+      if (lpos) ptab[ppos] = qcol;
+      else if (g == 0) agent_store(d.pos + ppos, qcol);
+      if (g == 0) { d.tau[i] = (T)tau; d.beta[i] = beta; }
     }
     if (has) {
       if (j == qcol) pos = ppos;
@@ -447,16 +478,17 @@ int qp3r_capacity(int G, int nt) {
   if (hipGetDevice(&dev) != hipSuccess) return 0;
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
   const void* f = nt == 1024 ? (const void*)k_qp3r<T, 1024> : (const void*)k_qp3r<T, 256>;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, nt, sizeof(T) * (size_t)G * kRP) != hipSuccess) return 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, nt, qp3r_lds<T>(G, kPosLds)) != hipSuccess) return 0;
   return per_cu * cus;
 }
 
 template <typename T>
-int launch_qp3r(hipStream_t s, const Qp3Desc<T>* d, const int* wstart, int nd, int nwg_total, int gmax, int nt,
-                int coop) {
+int launch_qp3r(hipStream_t s, const Qp3Desc<T>* d, const int* wstart, int nd, int nwg_total, int gmax, int nmax,
+                int nt, int coop) {
   if (nd <= 0) return 0;
   if (nt != 256 && nt != 1024) return (int)hipErrorInvalidValue;
-  const size_t shm = sizeof(T) * (size_t)gmax * kRP;
+  // the LDS table is sized for the largest N that uses it
+  const size_t shm = qp3r_lds<T>(gmax, std::min(nmax, kPosLds));
   const void* f = nt == 1024 ? (const void*)k_qp3r<T, 1024> : (const void*)k_qp3r<T, 256>;
   if (!coop) {
     if (nt == 1024) hipLaunchKernelGGL((k_qp3r<T, 1024>), dim3(nwg_total), dim3(1024), shm, s, d, wstart, nd);
@@ -468,8 +500,8 @@ int launch_qp3r(hipStream_t s, const Qp3Desc<T>* d, const int* wstart, int nd, i
 }
 template int qp3r_capacity<double>(int, int);
 template int qp3r_capacity<float>(int, int);
-template int launch_qp3r<double>(hipStream_t, const Qp3Desc<double>*, const int*, int, int, int, int, int);
-template int launch_qp3r<float>(hipStream_t, const Qp3Desc<float>*, const int*, int, int, int, int, int);
+template int launch_qp3r<double>(hipStream_t, const Qp3Desc<double>*, const int*, int, int, int, int, int, int);
+template int launch_qp3r<float>(hipStream_t, const Qp3Desc<float>*, const int*, int, int, int, int, int, int);
 
 int qp3_threads() { return kQThreads; }
 

@@ -135,16 +135,16 @@ __global__ __launch_bounds__(kSThreads) void k_refl(const ReflDesc<T>* __restric
       T y = 0;
 #pragma unroll
       for (int q = 0; q < kSWaves; ++q) y += red[q * kRNB + tid];
-      if (multi) d.part[((int64_t)(t & 1) * d.nwg + g) * kRNB + tid] = y;
+      if (multi) agent_store(d.part + ((int64_t)(t & 1) * d.nwg + g) * kRNB + tid, y);
       else ysh[tid] = y;
     }
     if (multi) {
-      // partial sums are written by wave 0, the wave that releases the barrier
+      // partial sums go through agent-scope atomics (no cache maintenance)
       target += d.nwg;
-      group_barrier(d.sync, target, d.err);
+      group_barrier_rx(d.sync, target, d.err);
       if (tid < kRNB) {
         T y = 0;
-        for (int q = 0; q < d.nwg; ++q) y += d.part[((int64_t)(t & 1) * d.nwg + q) * kRNB + tid];
+        for (int q = 0; q < d.nwg; ++q) y += agent_load(d.part + ((int64_t)(t & 1) * d.nwg + q) * kRNB + tid);
         ysh[tid] = y;
       }
     }
@@ -169,7 +169,7 @@ __global__ __launch_bounds__(kSThreads) void k_refl(const ReflDesc<T>* __restric
 // xs[jb:jb+w] with the first wavefront (lane l holds x_{jb+l});
 // trans: R^T (forward substitution).
 template <typename T>
-__device__ __forceinline__ void diag_solve(const T* rsh, T* __restrict__ xs, int jb, int w, int trans) {
+__device__ __forceinline__ void diag_solve(const T* rsh, T* __restrict__ xs, int jb, int w, int trans, bool shared) {
   if (threadIdx.x >= kWave) return;
   const int l = threadIdx.x;
   T xl = (l < w) ? xs[jb + l] : T(0);
@@ -188,7 +188,10 @@ __device__ __forceinline__ void diag_solve(const T* rsh, T* __restrict__ xs, int
       if (l > i && l < w) xl -= rsh[i + l * kTLd] * xi;
     }
   }
-  if (l < w) xs[jb + l] = xl;
+  if (l < w) {
+    if (shared) agent_store(xs + jb + l, xl);  // read by the other workgroups after the barrier
+    else xs[jb + l] = xl;
+  }
 }
 
 template <typename T>
@@ -208,14 +211,16 @@ __global__ __launch_bounds__(kSThreads) void k_bwd(const BwdDesc<T>* __restrict_
   const int b0 = (int)((int64_t)g * nblk / nwg), b1 = (int)((int64_t)(g + 1) * nblk / nwg);
   const int r0 = b0 * kRNB, r1 = min(k, b1 * kRNB);
   unsigned int target = 0;
+  const bool shared = nwg > 1;
   auto meet = [&]() {
-    if (nwg > 1) {
+    if (shared) {
       target += nwg;
-      group_barrier(d.sync, target, d.err);
+      group_barrier_rx(d.sync, target, d.err);
     } else {
       __syncthreads();
     }
   };
+  auto xget = [&](int r) { return shared ? agent_load(xs + r) : xs[r]; };
   auto stage_diag = [&](int jb, int w) {
     for (int e = tid; e < kRNB * kRNB; e += kSThreads) {
       const int r = e & (kRNB - 1), c = e / kRNB;
@@ -251,10 +256,10 @@ __global__ __launch_bounds__(kSThreads) void k_bwd(const BwdDesc<T>* __restrict_
       const int jb = bi * kRNB, w = min(kRNB, k - jb);
       if (bi >= b0 && bi < b1) {
         stage_diag(jb, w);
-        diag_solve(rsh, xs, jb, w, 0);
+        diag_solve(rsh, xs, jb, w, 0, shared);
       }
       meet();
-      if (tid < w) xsh[tid] = xs[jb + tid];
+      if (tid < w) xsh[tid] = xget(jb + tid);
       __syncthreads();
       for (int r = r0 + tid; r < min(r1, jb); r += kSThreads) {
         T s = 0;
@@ -270,10 +275,10 @@ __global__ __launch_bounds__(kSThreads) void k_bwd(const BwdDesc<T>* __restrict_
       const int jb = bi * kRNB, w = min(kRNB, k - jb);
       if (bi >= b0 && bi < b1) {
         stage_diag(jb, w);
-        diag_solve(rsh, xs, jb, w, 1);
+        diag_solve(rsh, xs, jb, w, 1, shared);
       }
       meet();
-      if (tid < w) xsh[tid] = xs[jb + tid];
+      if (tid < w) xsh[tid] = xget(jb + tid);
       __syncthreads();
       // x[r] -= sum_c R[jb+c, r] x[jb+c]: a half-wave reads the contiguous
       // 32-entry column segment of R for one row r
@@ -286,9 +291,9 @@ __global__ __launch_bounds__(kSThreads) void k_bwd(const BwdDesc<T>* __restrict_
       }
       __syncthreads();
     }
-    if (nwg > 1) {
+    if (shared) {
       target += nwg;
-      group_barrier(d.sync, target, d.err);
+      group_barrier_rx(d.sync, target, d.err);
     }
     // x_n[j] -= A_n[0:k, j]^T x_c[0:k]: one wavefront per column, columns
     // interleaved over all waves of the problem (atomic: several clusters of
@@ -297,7 +302,7 @@ __global__ __launch_bounds__(kSThreads) void k_bwd(const BwdDesc<T>* __restrict_
       const R12Desc<T> q = r12[d.blk0 + b];
       for (int j = g * kSWaves + wv; j < q.s; j += nwg * kSWaves) {
         T acc = 0;
-        for (int r = lane; r < k; r += kWave) acc += q.A[r + (int64_t)j * q.ld] * xs[r];
+        for (int r = lane; r < k; r += kWave) acc += q.A[r + (int64_t)j * q.ld] * xget(r);
         acc = wave_sum(acc);
         if (lane == 0) atomicAdd(xbuf + q.xoff + j, -acc);
       }

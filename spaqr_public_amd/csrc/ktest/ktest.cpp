@@ -156,7 +156,7 @@ int ktest_qp3(int k, int N, double* Ct, double* V, double* tau, double tol, int 
   Qp3Desc<double>* dd = dup(&d, 1);
   int zero = 0;
   int* ws = dup(&zero, 1);
-  int rc = form >= 1 ? launch_qp3r<double>(0, dd, ws, 1, nwg, G, nt, nwg > 1 ? 1 : 0)
+  int rc = form >= 1 ? launch_qp3r<double>(0, dd, ws, 1, nwg, G, N, nt, nwg > 1 ? 1 : 0)
                      : launch_qp3<double>(0, dd, ws, 1, nwg, k, nwg > 1 ? 1 : 0);
   if (rc != 0) { g_err = hipGetErrorString((hipError_t)rc); return -1; }
   KT(hipDeviceSynchronize());
@@ -212,7 +212,7 @@ int ktest_qp3_bench(int k, int N, const double* Ct, double tol, int form, int nw
     KT(hipMemcpy(dC, d0, sizeof(double) * (size_t)N * k, hipMemcpyDeviceToDevice));
     KT(hipMemset(sync, 0, sizeof(unsigned int) * barrier_words(nwg)));
     KT(hipEventRecord(a, 0));
-    rc = form >= 1 ? launch_qp3r<double>(0, dd, ws, 1, nwg, G, nt, nwg > 1 ? 1 : 0)
+    rc = form >= 1 ? launch_qp3r<double>(0, dd, ws, 1, nwg, G, N, nt, nwg > 1 ? 1 : 0)
                    : launch_qp3<double>(0, dd, ws, 1, nwg, k, nwg > 1 ? 1 : 0);
     KT(hipEventRecord(b, 0));
     KT(hipEventSynchronize(b));
