@@ -62,14 +62,21 @@ const void* Stager::push_raw(const void* src, size_t bytes) {
   size_t need = (bytes + 255) / 256 * 256;
   if (need > cap_) throw Error(kErrOom, "descriptor table larger than the staging ring");
   if (off_ + need > cap_) {
+    // wrap: copy what is pending, let every reader of the ring finish
+    flush();
     SPQ_HIP(hipStreamSynchronize(stream_));
     off_ = 0;
+    pend_ = 0;
   }
   std::memcpy(host_ + off_, src, bytes);
-  SPQ_HIP(hipMemcpyAsync(dev_ + off_, host_ + off_, bytes, hipMemcpyHostToDevice, stream_));
   const void* d = dev_ + off_;
   off_ += need;
   return d;
+}
+
+void Stager::flush() {
+  if (off_ > pend_) SPQ_HIP(hipMemcpyAsync(dev_ + pend_, host_ + pend_, off_ - pend_, hipMemcpyHostToDevice, stream_));
+  pend_ = off_;
 }
 
 }  // namespace spq

@@ -102,7 +102,10 @@ class Stager {
     return (const X*)push_raw(v.data(), sizeof(X) * v.size());
   }
   const void* push_raw(const void* src, size_t bytes);
-  void reset() { off_ = 0; }
+  // Copies everything pushed since the last flush with one transfer; every
+  // launch that reads pushed tables flushes first.
+  void flush();
+  void reset() { off_ = 0; pend_ = 0; }
   // Persistent copy (for solve programs): allocated from `perm` and kept.
   template <typename X>
   X* upload(Arena& perm, const std::vector<X>& v) {
@@ -117,7 +120,7 @@ class Stager {
   hipStream_t stream_ = nullptr;
   char* host_ = nullptr;
   char* dev_ = nullptr;
-  size_t cap_ = 0, off_ = 0;
+  size_t cap_ = 0, off_ = 0, pend_ = 0;  // [pend_, off_) not yet copied
 };
 
 // Prefix sums of per-descriptor tile counts (tstart table of tiled kernels).

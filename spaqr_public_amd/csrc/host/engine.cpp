@@ -245,6 +245,7 @@ class Engine final : public EngineBase {
   // kernels run).
   template <typename F>
   void timed(const char* name, double flops, double bytes, F&& launch) {
+    stager_.flush();
     if (!kprof_) { launch(); return; }
     hipEvent_t a = pooled_event(), b = pooled_event();
     SPQ_HIP(hipEventRecord(a, stream_));
@@ -482,6 +483,7 @@ class Engine final : public EngineBase {
     const ReflDesc<T>* refl = nullptr;
     const BwdDesc<T>* bwd = nullptr;
     const MoveDesc* mv = nullptr;
+    const R12Chunk<T>* r12c = nullptr;
     const int* wstart = nullptr;
     T* work = nullptr;
   };
@@ -1323,6 +1325,7 @@ void Engine<T>::reassign_level(const std::vector<int>& cs) {
     const int nt = prefix_tiles(ncnt, st);
     const RowNormDesc<T>* dd = stager_.push(nd);
     const int* dst = stager_.push(st);
+    stager_.flush();
     launch_rownorm<T>(stream_, dd, dst, (int)nd.size(), nt);
     SPQ_HIP(hipMemcpyAsync(norms.data(), dn, sizeof(double) * total, hipMemcpyDeviceToHost, stream_));
     sync();
@@ -1562,15 +1565,16 @@ void Engine<T>::build_solve_program() {
   std::vector<R12Desc<T>> r12;
   // buf: the x buffer a stage works on (0 = x, 1 = xt); moves copy sbuf -> buf
   struct HStage {
-    int type;
+    int type;  // 0 reflectors, 1 triangular sweeps, 2 moves, 3 R12 products
     std::vector<ReflDesc<T>> refl;
     std::vector<BwdDesc<T>> bwd;
     std::vector<MoveDesc> mv;
+    std::vector<R12Chunk<T>> r12c;
     int mmax = 0, buf = 0, sbuf = 0;
   };
   std::vector<HStage> hf, hb, htf, htb, hpf;
   auto push = [](std::vector<HStage>& v, HStage&& st) {
-    if (!st.refl.empty() || !st.bwd.empty() || !st.mv.empty()) v.push_back(std::move(st));
+    if (!st.refl.empty() || !st.bwd.empty() || !st.mv.empty() || !st.r12c.empty()) v.push_back(std::move(st));
   };
   auto stage = [](int type, int buf, int sbuf = -1) {
     HStage st;
@@ -1585,13 +1589,19 @@ void Engine<T>::build_solve_program() {
     solve_bytes_ += sizeof(T) * ((double)o.m * o.k + (double)nb_ * o.k) + 2.0 * sizeof(T) * o.m;
     return d;
   };
-  auto bwd_of = [&](const Op& o, int trans) {
+  // triangular sweep of a QR op; its R12 products go to `rs` as column
+  // chunks (run before the sweep when trans = 0, after it when trans = 1)
+  auto bwd_of = [&](const Op& o, int trans, HStage& rs) {
     BwdDesc<T> d{};
     d.R = o.R; d.ldr = o.ldr; d.k = o.kr; d.xoff = cl_[o.c].xoff;
-    d.blk0 = (int)r12.size(); d.nblk = (int)o.r12.size(); d.trans = trans;
+    d.blk0 = 0; d.nblk = 0; d.trans = trans;
+    const int cc = r12_chunk_cols();
     for (auto& q : o.r12) {
-      r12.push_back(R12Desc<T>{q.first.p, q.first.ld, q.second.second, cl_[q.second.first].xoff});
-      solve_bytes_ += sizeof(T) * (double)o.kr * q.second.second;
+      const int sn = q.second.second;
+      for (int j0 = 0; j0 < sn; j0 += cc)
+        rs.r12c.push_back(R12Chunk<T>{q.first.p + (int64_t)j0 * q.first.ld, cl_[o.c].xoff,
+                                      cl_[q.second.first].xoff + j0, q.first.ld, o.kr, std::min(cc, sn - j0), trans});
+      solve_bytes_ += sizeof(T) * (double)o.kr * sn;
     }
     solve_bytes_ += sizeof(T) * 0.5 * o.kr * o.kr + 2.0 * sizeof(T) * o.kr;
     return d;
@@ -1722,8 +1732,9 @@ void Engine<T>::build_solve_program() {
     for (auto o : L.sd) wmax = std::max(wmax, o->wave);
     for (int w = L.fq.empty() ? 0 : wmax; w >= 0; --w) {
       const bool all = L.fq.empty();
-      HStage sfq = stage(1, 0);
-      for (auto o : L.fq) if (o->wave == w) sfq.bwd.push_back(bwd_of(*o, 0));
+      HStage sfr = stage(3, 0), sfq = stage(1, 0);
+      for (auto o : L.fq) if (o->wave == w) sfq.bwd.push_back(bwd_of(*o, 0, sfr));
+      push(hb, std::move(sfr));
       push(hb, std::move(sfq));
       HStage sa = stage(2, 0), sb = stage(2, 0);
       for (auto o : L.sd) if (all || o->wave == w) splitd_bwd(o, sa, sb);
@@ -1740,8 +1751,9 @@ void Engine<T>::build_solve_program() {
     for (auto o : L.sc) ssc.bwd.push_back(scaled_of(*o, 0));
     push(hb, std::move(ssc));
     // Reassign has no backward step on x (operations.cpp:70)
-    HStage sqr = stage(1, 0);
-    for (auto o : L.qr) sqr.bwd.push_back(bwd_of(*o, 0));
+    HStage sqrr = stage(3, 0), sqr = stage(1, 0);
+    for (auto o : L.qr) sqr.bwd.push_back(bwd_of(*o, 0, sqrr));
+    push(hb, std::move(sqrr));
     push(hb, std::move(sqr));
   }
   if (!square_) {
@@ -1750,9 +1762,10 @@ void Engine<T>::build_solve_program() {
     // forward (permutations only) on x, then the backward program above
     for (int lvl = 0; lvl < nlevels_; ++lvl) {
       LevelOps L = level_ops(lvl);
-      HStage sqr = stage(1, 1);  // R^{-T}, then x_n -= A_n^T x_c (toperations.cpp:12-24)
-      for (auto o : L.qr) sqr.bwd.push_back(bwd_of(*o, 1));
+      HStage sqr = stage(1, 1), sqrr = stage(3, 1);  // R^{-T}, then x_n -= A_n^T x_c (toperations.cpp:12-24)
+      for (auto o : L.qr) sqr.bwd.push_back(bwd_of(*o, 1, sqrr));
       push(htf, std::move(sqr));
+      push(htf, std::move(sqrr));
       HStage ssc = stage(1, 1);
       for (auto o : L.sc) ssc.bwd.push_back(scaled_of(*o, 1));
       push(htf, std::move(ssc));
@@ -1770,9 +1783,10 @@ void Engine<T>::build_solve_program() {
         for (auto o : L.sd) if (all || o->wave == w) splitd_fwd(o, sa, sb);
         push(htf, std::move(sa));
         push(htf, std::move(sb));
-        HStage sfq = stage(1, 1);
-        for (auto o : L.fq) if (o->wave == w) sfq.bwd.push_back(bwd_of(*o, 1));
+        HStage sfq = stage(1, 1), sfr = stage(3, 1);
+        for (auto o : L.fq) if (o->wave == w) sfq.bwd.push_back(bwd_of(*o, 1, sfr));
         push(htf, std::move(sfq));
+        push(htf, std::move(sfr));
       }
       HStage smg = stage(2, 1);
       for (auto o : L.mg) merge_up(o, smg, false);
@@ -1836,7 +1850,7 @@ void Engine<T>::build_solve_program() {
       std::vector<int> cnt, st;
       if (h.type == 0) {
         std::vector<int> want;
-        for (auto& d : h.refl) want.push_back(d.m >= 2048 ? std::min(32, d.m / 256) : 1);
+        for (auto& d : h.refl) want.push_back(d.m >= 768 ? std::min(32, iceil(d.m, 256)) : 1);
         plan_nwg(want);
         std::vector<int> order(h.refl.size());
         std::iota(order.begin(), order.end(), 0);
@@ -1879,10 +1893,12 @@ void Engine<T>::build_solve_program() {
         s.nwg_total = prefix_tiles(cnt, st);
         s.nd = (int)ds.size();
         h.bwd = ds;
+      } else if (h.type == 3) {
+        s.nd = (int)h.r12c.size();
       } else {
         s.nd = (int)h.mv.size();
       }
-      if (h.type != 2) s.wstart = stager_.upload(perm_, st);
+      if (h.type <= 1) s.wstart = stager_.upload(perm_, st);
       out.push_back(s);
     }
   };
@@ -1910,6 +1926,8 @@ void Engine<T>::build_solve_program() {
           d.err = solve_err_;
         }
         st[i].bwd = stager_.upload(perm_, h.bwd);
+      } else if (h.type == 3) {
+        st[i].r12c = stager_.upload(perm_, h.r12c);
       } else {
         st[i].mv = stager_.upload(perm_, h.mv);
       }
@@ -1976,6 +1994,7 @@ void Engine<T>::run_program(const double* b, double* x, bool prof, bool nrml) {
     T* xb = s.buf ? xtbuf_ : xbuf_;
     if (s.type == 0) launch_refl<T>(stream_, s.refl, s.wstart, s.nd, s.nwg_total, segs_dev_, xb, s.work, s.mmax);
     else if (s.type == 1) launch_bwd<T>(stream_, s.bwd, s.wstart, s.nd, s.nwg_total, r12_dev_, xb);
+    else if (s.type == 3) launch_r12<T>(stream_, s.r12c, s.nd, xb);
     else launch_move<T>(stream_, s.mv, s.nd, xb, s.sbuf ? xtbuf_ : xbuf_);
   };
   std::vector<const std::vector<Stage>*> progs;

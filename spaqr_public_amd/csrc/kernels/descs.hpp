@@ -168,6 +168,16 @@ struct RowGatherDesc {
   int pad_;
 };
 
+// One column chunk of an R12 product in the solve (x-buffer offsets):
+// trans = 0: x[xc + r] -= sum_{j<nc} A[r + j*ld] * x[xn + j], r < k
+// trans = 1: x[xn + j] -= sum_{r<k} A[r + j*ld] * x[xc + r],  j < nc
+template <typename T>
+struct R12Chunk {
+  const T* A;
+  int64_t xc, xn;
+  int ld, k, nc, trans;
+};
+
 // Vector segment move dst[0:len] = src[0:len] (x-buffer offsets).
 struct MoveDesc {
   int64_t src, dst;

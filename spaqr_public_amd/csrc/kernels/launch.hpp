@@ -65,6 +65,9 @@ void launch_refl(hipStream_t s, const ReflDesc<T>* d, const int* wstart, int nd,
 template <typename T>
 void launch_bwd(hipStream_t s, const BwdDesc<T>* d, const int* wstart, int nd, int nwg_total, const R12Desc<T>* r12,
                 T* xbuf);
+int r12_chunk_cols();
+template <typename T>
+void launch_r12(hipStream_t s, const R12Chunk<T>* d, int nd, T* xbuf);
 template <typename T>
 void launch_move(hipStream_t s, const MoveDesc* d, int nd, T* dst, const T* src);
 template <typename T>

@@ -310,6 +310,37 @@ __global__ __launch_bounds__(kSThreads) void k_bwd(const BwdDesc<T>* __restrict_
   }
 }
 
+// R12 products of a solve stage as one batched GEMV over column chunks
+// (one workgroup per chunk), accumulated with atomics: the products of a
+// stage are independent of its triangular sweeps and stream far more data.
+template <typename T>
+__global__ __launch_bounds__(kSThreads) void k_r12(const R12Chunk<T>* __restrict__ d, T* __restrict__ xbuf) {
+  const R12Chunk<T> q = d[blockIdx.x];
+  const int tid = threadIdx.x, lane = tid & (kWave - 1), wv = tid / kWave;
+  if (!q.trans) {
+    const T* __restrict__ xn = xbuf + q.xn;
+    for (int r = tid; r < q.k; r += kSThreads) {
+      T acc = 0;
+      for (int j = 0; j < q.nc; ++j) acc += q.A[r + (int64_t)j * q.ld] * xn[j];
+      atomicAdd(xbuf + q.xc + r, -acc);
+    }
+  } else {
+    const T* __restrict__ xc = xbuf + q.xc;
+    for (int j = wv; j < q.nc; j += kSWaves) {
+      T acc = 0;
+      for (int r = lane; r < q.k; r += kWave) acc += q.A[r + (int64_t)j * q.ld] * xc[r];
+      acc = wave_sum(acc);
+      if (lane == 0) atomicAdd(xbuf + q.xn + j, -acc);
+    }
+  }
+}
+int r12_chunk_cols() { return 64; }
+template <typename T>
+void launch_r12(hipStream_t s, const R12Chunk<T>* d, int nd, T* xbuf) {
+  if (nd <= 0) return;
+  hipLaunchKernelGGL((k_r12<T>), dim3(nd), dim3(kSThreads), 0, s, d, xbuf);
+}
+
 template <typename T>
 __global__ __launch_bounds__(kSThreads) void k_move(const MoveDesc* __restrict__ d, T* __restrict__ dst_base,
                                                     const T* __restrict__ src_base) {
@@ -368,6 +399,7 @@ void launch_scatter_out(hipStream_t s, const int64_t* src, const int* dst, int n
   template void launch_refl<T>(hipStream_t, const ReflDesc<T>*, const int*, int, int, const Seg*, T*, T*, int); \
   template void launch_bwd<T>(hipStream_t, const BwdDesc<T>*, const int*, int, int, const R12Desc<T>*, T*);     \
   template void launch_move<T>(hipStream_t, const MoveDesc*, int, T*, const T*);                               \
+  template void launch_r12<T>(hipStream_t, const R12Chunk<T>*, int, T*);                                        \
   template void launch_gather_in<T>(hipStream_t, const int64_t*, const int*, int, const double*, T*);          \
   template void launch_scatter_out<T>(hipStream_t, const int64_t*, const int*, int, const T*, double*);
 SPQ_INST(double)
