@@ -289,10 +289,36 @@ class Engine final : public EngineBase {
     const int* dst = stager_.push(st);
     timed("init", 0.0, bytes, [&] { launch_init_blocks(stream_, sizeof(T), dd, dst, (int)ds.size(), nt); });
   }
-  void gemm_batch(const std::vector<GemmDesc<T>>& ds) {
+  // Batched GEMM.  When the batch has few output tiles, descriptors with a
+  // long K are split along K (partials in ws_, then an ordered reduce) so the
+  // launch fills the device.
+  void gemm_batch(std::vector<GemmDesc<T>> ds) {
     if (ds.empty()) return;
+    int tiles = 0;
+    for (auto& d : ds) tiles += gemm_tiles(d.m, d.n);
+    const int want = 4 * num_cus_;
+    bool split = false;
+    if (tiles < want) {
+      // partials live in gws_ until this batch's reduce has run; the next
+      // batch is ordered after it on the stream, so the arena is reused.
+      gws_.rewind();
+      size_t room = kSplitWsBytes;  // bytes, allocations rounded to 256
+      for (auto& d : ds) {
+        if (d.m <= 0 || d.n <= 0) continue;
+        const size_t mn = (size_t)d.m * d.n;
+        int ks = std::min(std::max(1, want / std::max(tiles, 1)), std::max(1, d.k / 256));
+        ks = (int)std::min<size_t>(ks, room / (sizeof(T) * mn + 256));
+        if (ks > 1) {
+          const size_t bytes = (sizeof(T) * (size_t)ks * mn + 255) / 256 * 256;
+          d.ksplit = ks;
+          d.work = (T*)gws_.alloc(bytes);
+          room -= bytes;
+          split = true;
+        }
+      }
+    }
     std::vector<int> cnt(ds.size()), st;
-    for (size_t i = 0; i < ds.size(); ++i) cnt[i] = gemm_tiles(ds[i].m, ds[i].n);
+    for (size_t i = 0; i < ds.size(); ++i) cnt[i] = gemm_tiles(ds[i].m, ds[i].n) * ds[i].ksplit;
     const int nt = prefix_tiles(cnt, st);
     double fl = 0, by = 0;
     for (auto& d : ds) {
@@ -301,7 +327,18 @@ class Engine final : public EngineBase {
     }
     const GemmDesc<T>* dd = stager_.push(ds);
     const int* dst = stager_.push(st);
-    timed("gemm", fl, by, [&] { launch_gemm<T>(stream_, dd, dst, (int)ds.size(), nt); });
+    std::vector<GemmDesc<T>> rd;
+    std::vector<int> rcnt, rst;
+    if (split)
+      for (auto& d : ds)
+        if (d.ksplit > 1) { rd.push_back(d); rcnt.push_back(iceil(d.m * d.n, 256)); }
+    const int rnt = prefix_tiles(rcnt, rst);
+    const GemmDesc<T>* drd = rd.empty() ? nullptr : stager_.push(rd);
+    const int* drst = rd.empty() ? nullptr : stager_.push(rst);
+    timed("gemm", fl, by, [&] {
+      launch_gemm<T>(stream_, dd, dst, (int)ds.size(), nt);
+      if (!rd.empty()) launch_gemm_reduce<T>(stream_, drd, drst, (int)rd.size(), rnt);
+    });
   }
   static GemmDesc<T> gd(const T* A, int lda, const T* B, int ldb, T* C, int ldc, int m, int n, int k, int flags,
                         T alpha, T beta) {
@@ -454,7 +491,8 @@ class Engine final : public EngineBase {
   int64_t xsize_ = 0;
   T* xbuf_ = nullptr;
   hipStream_t stream_ = nullptr;
-  Arena perm_, scratch_, ws_;
+  Arena perm_, scratch_, ws_, gws_{kSplitWsBytes + 4096};
+  static constexpr size_t kSplitWsBytes = size_t(64) << 20;
   int* qp3_err_ = nullptr;
   int qp3_max_wg_ = 64;
   Stager stager_;
@@ -1825,9 +1863,10 @@ void Engine<T>::build_solve_program() {
   }
   segs_dev_ = stager_.upload(perm_, segs);
   r12_dev_ = stager_.upload(perm_, r12);
-  // Multi-workgroup split of large ops: row slices of >= 256 rows (reflector
-  // applies) or >= 4 blocks of 32 (triangular solves); the stage's
-  // multi-workgroup ops come first and use at most one workgroup per CU.
+  // Multi-workgroup split of large ops: row slices of <= 256 rows for
+  // reflector applies of >= 384 rows, or >= 4 blocks of 32 for triangular
+  // sweeps of >= 512; the stage's multi-workgroup ops come first and use at
+  // most one workgroup per CU.
   int nsync = 0;
   auto plan_nwg = [&](std::vector<int>& want) {
     int total = 0;
@@ -1850,7 +1889,7 @@ void Engine<T>::build_solve_program() {
       std::vector<int> cnt, st;
       if (h.type == 0) {
         std::vector<int> want;
-        for (auto& d : h.refl) want.push_back(d.m >= 768 ? std::min(32, iceil(d.m, 256)) : 1);
+        for (auto& d : h.refl) want.push_back(d.m >= 384 ? std::min(32, iceil(d.m, 256)) : 1);
         plan_nwg(want);
         std::vector<int> order(h.refl.size());
         std::iota(order.begin(), order.end(), 0);

@@ -33,6 +33,9 @@ enum : int { kStructNone = 0, kStructUnitLower = 1, kStructUpper = 2, kStructLow
 constexpr int gemm_flags(bool ta, bool tb, int sa = 0, int sb = 0) {
   return (ta ? kTransA : 0) | (tb ? kTransB : 0) | (sa << 4) | (sb << 6);
 }
+// Split-K (ksplit > 1): the K range is cut into ksplit chunks whose partial
+// products alpha * op(A) op(B) go to work (ksplit x m x n, ld m); a reduce
+// pass sums them in chunk order and applies beta (deterministic).
 template <typename T>
 struct GemmDesc {
   const T* A;
@@ -42,6 +45,8 @@ struct GemmDesc {
   int m, n, k;
   int flags;
   T alpha, beta;
+  int ksplit = 1;
+  T* work = nullptr;
 };
 
 // Householder panel block: factor columns [jb, jb+w) of P (m x k, ld), writing

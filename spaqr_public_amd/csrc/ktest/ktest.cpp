@@ -54,6 +54,24 @@ int ktest_gemm(int m, int n, int k, const double* A, int lda, int acols, const d
   return 0;
 }
 
+// Same product split along K into ksplit chunks (partials + ordered reduce).
+int ktest_gemm_split(int m, int n, int k, const double* A, int lda, int acols, const double* B, int ldb, int bcols,
+                     double* Cm, int ldc, int flags, double alpha, double beta, int ksplit) {
+  if (ksplit < 1 || m < 1 || n < 1) { g_err = "bad split"; return -1; }
+  double *dA = dup(A, (size_t)lda * acols), *dB = dup(B, (size_t)ldb * bcols), *dC = dup(Cm, (size_t)ldc * n);
+  double* dW = dup<double>(nullptr, (size_t)ksplit * m * n);
+  GemmDesc<double> g{dA, dB, dC, lda, ldb, ldc, m, n, k, flags, alpha, beta, ksplit, dW};
+  GemmDesc<double>* dg = dup(&g, 1);
+  int zero = 0;
+  int* ds = dup(&zero, 1);
+  launch_gemm<double>(0, dg, ds, 1, gemm_tiles(m, n) * ksplit);
+  launch_gemm_reduce<double>(0, dg, ds, 1, (m * n + 255) / 256);
+  KT(hipDeviceSynchronize());
+  back(Cm, dC, (size_t)ldc * n);
+  (void)hipFree(dA); (void)hipFree(dB); (void)hipFree(dC); (void)hipFree(dW); (void)hipFree(dg); (void)hipFree(ds);
+  return 0;
+}
+
 // Blocked Householder QR of one m x k panel (col-major, ld m): A <- V\R,
 // tau (min(m,k)), Tb (nb x roundup(k,nb)), and C (m x nc) <- Q^T C.
 int ktest_qr(int m, int k, double* A, double* tau, double* Tb, int nc, double* Cm) {

@@ -28,6 +28,7 @@ def rel(a, b):
 def K(ktest):
     ktest.ktest_gemm.argtypes = [C.c_int, C.c_int, C.c_int, dp, C.c_int, C.c_int, dp, C.c_int, C.c_int, dp, C.c_int,
                                  C.c_int, C.c_double, C.c_double]
+    ktest.ktest_gemm_split.argtypes = ktest.ktest_gemm.argtypes + [C.c_int]
     ktest.ktest_qr.argtypes = [C.c_int, C.c_int, dp, dp, dp, C.c_int, dp]
     ktest.ktest_larft.argtypes = [C.c_int, C.c_int, dp, dp, dp]
     ktest.ktest_qp3.argtypes = [C.c_int, C.c_int, dp, dp, dp, C.c_double, C.c_int, C.c_int]
@@ -99,6 +100,29 @@ def test_gemm(K, mnk, ta, tb, sa, sb):
     flags = ta | (tb << 1) | (sa << 4) | (sb << 6)
     assert K.ktest_gemm(m, n, k, d(A), lda, Ac, d(B), ldb, Bc, d(Cm), ldc, flags, alpha, beta) == 0
     assert rel(Cm[:m, :n], ref) < 1e-13
+
+
+# split-K: K cut into ksplit chunks (some empty when k < 16 * ksplit), partial
+# tiles summed in chunk order by the reduce pass, which applies beta
+@pytest.mark.parametrize("mnk,ks,ta,sa", [((32, 300, 5000), 9, 1, 1), ((32, 70, 1000), 4, 1, 0),
+                                          ((100, 33, 37), 5, 0, 0), ((64, 64, 20), 3, 1, 2), ((7, 5, 3), 2, 0, 3)])
+def test_gemm_split(K, mnk, ks, ta, sa):
+    m, n, k = mnk
+    rng = np.random.default_rng(m * 11 + n + ks)
+    Ar, Ac = (k, m) if ta else (m, k)
+    lda, ldb, ldc = Ar + 1, k + 2, m + 1
+    A = np.asfortranarray(rng.standard_normal((lda, Ac)))
+    B = np.asfortranarray(rng.standard_normal((ldb, n)))
+    Cm = np.asfortranarray(rng.standard_normal((ldc, n)))
+    alpha, beta = 0.75, -1.0
+    Am = struct_mask(A[:Ar, :Ac], sa)
+    opA = Am.T if ta else Am
+    ref = alpha * opA @ B[:k, :n] + beta * Cm[:m, :n]
+    flags = ta | (sa << 4)
+    before = Cm.copy()
+    assert K.ktest_gemm_split(m, n, k, d(A), lda, Ac, d(B), ldb, n, d(Cm), ldc, flags, alpha, beta, ks) == 0
+    assert rel(Cm[:m, :n], ref) < 1e-13
+    assert np.array_equal(Cm[m:, :], before[m:, :])  # padding rows untouched
 
 
 # panel steps run on ceil((m - jb) / 256) workgroups: (513, 65), (2000, 70)
