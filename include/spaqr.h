@@ -91,7 +91,14 @@ void spaqr_matrix_destroy(spaqr_matrix* A);
 /* gmres(mat, rhs, x, precond, iters, restart, tol, verb)   is.cpp:95-272
  * Left-preconditioned restarted GMRES with the tree as preconditioner,
  * FP64.  Host vectors; x holds the initial guess on entry.
- * iters_out: iterations; est_out: the stopping quantity |g_k|/||M^{-1}r_0||. */
+ * iters_out: iterations; est_out: the stopping quantity |g_k|/||M^{-1}r_0||.
+ * For a tree factored in FP32 (the mixed form) the call runs FP64 iterative
+ * refinement around this GMRES: true residual r = b - Ax by the FP64 SpMV,
+ * GMRES on A d = r to a relative estimate of max(1e-6, tol / (2 ||r||/||b||))
+ * with the FP32 factors as
+ * preconditioner, x += d, until ||b - Ax||/||b|| <= tol (or a step stops
+ * halving it, or `iters` inner iterations are spent); then iters_out is
+ * the total of inner iterations and est_out the final ||b - Ax||/||b||.   */
 int spaqr_gmres(spaqr_tree* t, const spaqr_matrix* A, const double* b, double* x, int iters, int restart,
                 double tol, int verbose, int* iters_out, double* est_out);
 /* Same on device vectors.                                                 */

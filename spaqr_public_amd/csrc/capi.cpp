@@ -196,7 +196,8 @@ int spaqr_gmres_device(spaqr_tree* t, const spaqr_matrix* A, const double* b, do
     EngineBase& e = eng(t);
     if (!A || !A->d) throw Error(kErrArg, "null matrix");
     if (A->d->m != A->d->n || A->d->m != e.rows()) throw Error(kErrArg, "gmres needs the square matrix of the tree");
-    KrylovResult r = gmres_dev(e, *A->d, b, x, iters, restart, tol, verbose);
+    KrylovResult r = e.precision_bytes() == 4 ? gmres_ir_dev(e, *A->d, b, x, iters, restart, tol, verbose)
+                                               : gmres_dev(e, *A->d, b, x, iters, restart, tol, verbose);
     e.check_solve();
     if (iters_out) *iters_out = r.iters;
     if (est_out) *est_out = r.est;
@@ -247,7 +248,8 @@ int spaqr_gmres(spaqr_tree* t, const spaqr_matrix* A, const double* b, double* x
     SPQ_HIP(hipMemcpy(db, b, sizeof(double) * m, hipMemcpyHostToDevice));
     SPQ_HIP(hipMemcpy(dx, x, sizeof(double) * n, hipMemcpyHostToDevice));
     if (A->d->m != A->d->n || A->d->m != e.rows()) throw Error(kErrArg, "gmres needs the square matrix of the tree");
-    KrylovResult r = gmres_dev(e, *A->d, db, dx, iters, restart, tol, verbose);
+    KrylovResult r = e.precision_bytes() == 4 ? gmres_ir_dev(e, *A->d, db, dx, iters, restart, tol, verbose)
+                                               : gmres_dev(e, *A->d, db, dx, iters, restart, tol, verbose);
     e.check_solve();
     SPQ_HIP(hipMemcpy(x, dx, sizeof(double) * n, hipMemcpyDeviceToHost));
     if (iters_out) *iters_out = r.iters;

@@ -138,9 +138,66 @@ KrylovResult gmres_dev(EngineBase& M, const DevMatrix& A, const double* b, doubl
   return res;
 }
 
+KrylovResult gmres_ir_dev(EngineBase& M, const DevMatrix& A, const double* b, double* x, int iters, int restart,
+                          double tol, int verbose) {
+  KrylovResult res;
+  hipStream_t s = M.stream();
+  const int n = A.m;
+  const double t0 = wall();
+  double *r = nullptr, *d = nullptr, *h = nullptr, *hh = nullptr;
+  SPQ_HIP(hipMalloc((void**)&r, sizeof(double) * std::max(n, 1)));
+  SPQ_HIP(hipMalloc((void**)&d, sizeof(double) * std::max(n, 1)));
+  SPQ_HIP(hipMalloc((void**)&h, sizeof(double) * 2));
+  SPQ_HIP(hipHostMalloc((void**)&hh, sizeof(double) * 2, hipHostMallocDefault));
+  struct Free {
+    std::vector<void*> dv;
+    void* hp;
+    ~Free() { for (void* q : dv) hipFree(q); hipHostFree(hp); }
+  } fr{{r, d, h}, hh};
+  auto norm = [&](const double* v) {
+    launch_dots(s, n, 1, v, n, v, h);
+    SPQ_HIP(hipMemcpyAsync(hh, h, sizeof(double), hipMemcpyDeviceToHost, s));
+    SPQ_HIP(hipStreamSynchronize(s));
+    return std::sqrt(hh[0]);
+  };
+  const double bn = norm(b);
+  if (bn <= std::numeric_limits<double>::min()) {
+    SPQ_HIP(hipMemsetAsync(x, 0, sizeof(double) * n, s));
+    SPQ_HIP(hipStreamSynchronize(s));
+    res.t_total = wall() - t0;
+    return res;
+  }
+  double prev = 0;
+  for (;;) {
+    SPQ_HIP(hipMemcpyAsync(r, b, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    launch_spmv(s, n, A.rp, A.ci, A.v, x, r, -1.0, 1.0);
+    const double rel = norm(r) / bn;
+    res.est = rel;
+    if (verbose) printf("refinement %d: |b - Ax|/|b| = %3.2e (%d inner iterations)\n", res.outer, rel, res.iters);
+    if (rel <= tol || res.iters >= iters || res.outer >= kIrMaxOuter) break;
+    if (res.outer > 0 && rel > 0.5 * prev) break;  // the FP32 factors gain no more
+    prev = rel;
+    SPQ_HIP(hipMemsetAsync(d, 0, sizeof(double) * n, s));
+    const int left = iters - res.iters;
+    // no tighter than this step needs to reach tol (with a factor 2 margin)
+    const double inner_tol = std::max(kIrInnerTol, 0.5 * tol / rel);
+    KrylovResult in = gmres_dev(M, A, r, d, left, std::min(restart, left), inner_tol, verbose > 1 ? verbose : 0);
+    res.iters += in.iters;
+    res.n_precond += in.n_precond;
+    res.t_precond += in.t_precond;
+    launch_axpby(s, n, 1.0, d, 1.0, x);
+    ++res.outer;
+  }
+  SPQ_HIP(hipStreamSynchronize(s));
+  res.t_total = wall() - t0;
+  return res;
+}
+
 // Preconditioned CGLS on the normal equations (is.cpp:275-346): the
 // preconditioner is the normal-equation apply of the tree, the stop test is
-// ||A^T r||^2 < tol^2 ||A^T b||^2.  FP64 throughout; b (m), x (n) on the device.
+// ||A^T r||^2 < tol^2 ||A^T b||^2.  FP64 vectors and recurrences; with FP32
+// factors the direction update uses the Polak-Ribiere beta.  b (m), x (n) on
+// the device.
 KrylovResult cgls_dev(EngineBase& M, const DevMatrix& A, const double* b, double* x, int iters, double tol,
                       int verbose) {
   KrylovResult res;
@@ -154,13 +211,16 @@ KrylovResult cgls_dev(EngineBase& M, const DevMatrix& A, const double* b, double
   SPQ_HIP(hipMalloc((void**)&nr, sizeof(double) * std::max(n, 1)));
   SPQ_HIP(hipMalloc((void**)&p, sizeof(double) * std::max(n, 1)));
   SPQ_HIP(hipMalloc((void**)&z, sizeof(double) * std::max(n, 1)));
+  const bool flexible = M.precision_bytes() == 4;
+  double* zold = nullptr;
+  SPQ_HIP(hipMalloc((void**)&zold, sizeof(double) * (flexible ? std::max(n, 1) : 1)));
   SPQ_HIP(hipMalloc((void**)&h, sizeof(double) * 4));
   SPQ_HIP(hipHostMalloc((void**)&hh, sizeof(double) * 4, hipHostMallocDefault));
   struct Free {
     std::vector<void*> d;
     void* hp;
     ~Free() { for (void* q : d) hipFree(q); hipHostFree(hp); }
-  } fr{{r, tmp, nr, p, z, h}, hh};
+  } fr{{r, tmp, nr, p, z, zold, h}, hh};
   auto dot = [&](int len, const double* u, const double* v) {
     launch_dots(s, len, 1, u, len, v, h);
     SPQ_HIP(hipMemcpyAsync(hh, h, sizeof(double), hipMemcpyDeviceToHost, s));
@@ -202,10 +262,13 @@ KrylovResult cgls_dev(EngineBase& M, const DevMatrix& A, const double* b, double
       rn2 = dot(n, nr, nr);
       if (verbose > 1) printf("%d %3.2e \n", i, std::sqrt(rn2 / rhsn2));
       if (rn2 < thr) break;
+      if (flexible) SPQ_HIP(hipMemcpyAsync(zold, z, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
       precond(nr, z);
       const double absOld = absNew;
       absNew = dot(n, nr, z);
-      const double beta = absNew / absOld;
+      // FP32 factors make the preconditioner slightly non-linear: the
+      // Polak-Ribiere form (nr . (z - z_old)) keeps the directions conjugate
+      const double beta = (flexible ? absNew - dot(n, nr, zold) : absNew) / absOld;
       launch_axpby(s, n, 1.0, z, beta, p);
       ++i;
     }

@@ -21,6 +21,7 @@ struct KrylovResult {
   double t_total = 0;   // seconds, device synchronised
   double t_precond = 0; // seconds spent in preconditioner applies (host clock)
   int n_precond = 0;
+  int outer = 0;        // refinement steps (gmres_ir_dev)
 };
 
 // Left-preconditioned restarted GMRES: r0 = M^{-1}(b - A x), stop when
@@ -28,6 +29,19 @@ struct KrylovResult {
 // with one reorthogonalisation, FP64 throughout.  b, x are device vectors.
 KrylovResult gmres_dev(EngineBase& M, const DevMatrix& A, const double* b, double* x, int iters, int restart,
                        double tol, int verbose);
+
+// Mixed form: FP64 iterative refinement around gmres_dev for a tree factored
+// in FP32.  Each step forms the true residual r = b - A x with the FP64 SpMV,
+// solves A d = r by gmres_dev (FP64 Arnoldi, FP32 factors as the
+// preconditioner) to a relative estimate of max(kIrInnerTol, tol / (2 rel)),
+// rel the current true relative residual, and adds d.
+// Stops when ||b - A x|| / ||b|| <= tol, when a step fails to halve the true
+// residual, or when `iters` inner iterations are spent.  est = final true
+// relative residual; iters = total inner iterations.
+constexpr double kIrInnerTol = 1e-6;
+constexpr int kIrMaxOuter = 20;
+KrylovResult gmres_ir_dev(EngineBase& M, const DevMatrix& A, const double* b, double* x, int iters, int restart,
+                          double tol, int verbose);
 
 // Preconditioned CGLS on the normal equations (is.cpp:275-346): stop when
 // ||A^T r||^2 < tol^2 ||A^T b||^2; est = ||A^T r|| / ||A^T b||.  b has length

@@ -138,3 +138,49 @@ def test_least_squares(oracle, gpu_lib, n, tol):
     assert sg.shape == so.shape
     if len(so):
         assert np.mean(sg[:, 4] == so[:, 4]) >= 0.95
+
+
+@pytest.mark.parametrize("name,n,dim,tol", [
+    ("advdiff2d_128", 128, 2, 1e-3),
+    ("advdiff3d_16", 16, 3, 1e-2),
+    ("hicontrast3d_16", 16, 3, 1e-2),
+    ("hicontrast3d_32", 32, 3, 1e-2),
+])
+def test_mixed_precision(oracle, gpu_lib, name, n, dim, tol):
+    """Mixed form: FP32 factorization, FP64 GMRES as iterative refinement.
+    Same tree as the FP64 oracle; the one-shot solve stays within 2x of the
+    oracle's residual and close to its solution; refinement reaches 1e-12."""
+    import spaqr_public_amd as sq
+    from spaqr_public_amd import gen
+    A = gen.config_matrix(name)
+    oracle.set_backend("own")
+    ref = oracle.run_reference_flow(A, n, dim, tol)
+    gpu = sq.run_flow(A, n, dim, tol, precision=sq.FP32)
+    _check_tree(gpu, ref)
+    x32 = gpu["tree"].solve(gpu["b"])
+    x64 = ref["tree"].solve(ref["b"])
+    err = np.linalg.norm(x32 - x64) / np.linalg.norm(x64)
+    print(f"{name}: one-shot {gpu['oneshot_res']:.3e} (oracle {ref['oneshot_res']:.3e}), "
+          f"|x32 - x64|/|x64| = {err:.2e}, refinement its {gpu['iters']} (oracle GMRES {ref['iters']}), "
+          f"final {gpu['final_res']:.2e}")
+    assert gpu["oneshot_res"] <= 2.0 * ref["oneshot_res"]
+    assert err < 1e-3
+    assert gpu["final_res"] <= 1e-12
+    assert gpu["iters"] <= 3 * ref["iters"] + 6
+
+
+@pytest.mark.parametrize("n,tol", [(32, 1e-2), (64, 1e-2)])
+def test_least_squares_mixed(oracle, gpu_lib, n, tol):
+    """Least squares with FP32 factors and FP64 CGLS (configs[3] form)."""
+    import spaqr_public_amd as sq
+    from spaqr_public_amd import gen
+    A = gen.config_matrix(f"ls_lapgrad_{n}")
+    oracle.set_backend("own")
+    ref = oracle.run_reference_flow(A, n, 2, tol, res=1e-10)
+    gpu = sq.run_flow(A, n, 2, tol, res=1e-10, precision=sq.FP32)
+    _check_tree(gpu, ref)
+    print(f"ls {n}: one-shot {gpu['oneshot_res']:.3e} (oracle {ref['oneshot_res']:.3e}), "
+          f"CGLS its {gpu['iters']} (oracle {ref['iters']}), final {gpu['final_res']:.2e}")
+    assert gpu["oneshot_res"] <= 2.0 * ref["oneshot_res"] + 1e-6
+    assert gpu["final_res"] <= 1e-10
+    assert gpu["iters"] <= 2 * ref["iters"] + 4
