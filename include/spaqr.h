@@ -105,6 +105,26 @@ int spaqr_gmres(spaqr_tree* t, const spaqr_matrix* A, const double* b, double* x
 int spaqr_gmres_device(spaqr_tree* t, const spaqr_matrix* A, const double* b, double* x, int iters, int restart,
                        double tol, int verbose, int* iters_out, double* est_out);
 
+/* Adjoint of x = A^{-1} b (differentiable solve).  No counterpart in the
+ * reference, whose tOperations (toperations.h:14-20) transpose only the
+ * R-only least-squares factors: these transpose the full square factors.
+ * x = M^{-T} b: the op tape's backward steps transposed in reverse, then its
+ * forward steps transposed in reverse (Householder products as Q instead of
+ * Q^T, triangular sweeps with R^T, R12 products as x_n -= R12^T x_c,
+ * permutations inverted).  Square trees; host vectors of length rows.     */
+int spaqr_solve_transpose(spaqr_tree* t, const double* b, double* x);
+/* Same on device vectors (b may alias x).                                 */
+int spaqr_solve_transpose_device(spaqr_tree* t, const double* b, double* x);
+/* GMRES on A^T x = b left-preconditioned with M^{-T}, otherwise exactly
+ * spaqr_gmres_device (same stopping rule; iterative refinement for FP32
+ * trees).  A must be the tree's square matrix.                             */
+int spaqr_gmres_transpose_device(spaqr_tree* t, const spaqr_matrix* A, const double* b, double* x, int iters,
+                                 int restart, double tol, int verbose, int* iters_out, double* est_out);
+/* out[k] = alpha * u[i_k] * v[j_k] for the nonzeros (i_k, j_k) of A in CSC
+ * order; device vectors.  With u = lambda (A^T lambda = dL/dx), v = x and
+ * alpha = -1 it is dL/dA on A's sparsity pattern.                          */
+int spaqr_csc_outer_device(const spaqr_matrix* A, double alpha, const double* u, const double* v, double* out);
+
 /* cgls(mat, rhs, x, precond, iters, tol, verb)            is.cpp:275-346
  * CGLS on A^T A x = A^T b preconditioned by the tree's normal-equation
  * apply; stops when ||A^T r||^2 < tol^2 ||A^T b||^2.  A is m x n (uploaded

@@ -39,6 +39,8 @@ SYMBOLS = [
     "spaqr_num_sparsified", "spaqr_sparsified", "spaqr_num_clusters", "spaqr_cluster_table",
     "spaqr_set_phase_sync", "spaqr_kernel_timing", "spaqr_kernel_stat_count", "spaqr_kernel_stat",
     "spaqr_kernel_stats_reset", "spaqr_solve_nrml_device", "spaqr_cgls", "spaqr_cgls_device",
+    "spaqr_solve_transpose", "spaqr_solve_transpose_device", "spaqr_gmres_transpose_device",
+    "spaqr_csc_outer_device",
 ]
 
 
@@ -78,6 +80,10 @@ def load_library(path: str = LIB_PATH):
     L.spaqr_gmres.argtypes = [C.c_void_p, C.c_void_p, _dp, _dp, C.c_int, C.c_int, C.c_double, C.c_int, _ip, _dp]
     L.spaqr_gmres_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int,
                                      C.c_double, C.c_int, _ip, _dp]
+    L.spaqr_solve_transpose.argtypes = [C.c_void_p, _dp, _dp]
+    L.spaqr_solve_transpose_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+    L.spaqr_gmres_transpose_device.argtypes = L.spaqr_gmres_device.argtypes
+    L.spaqr_csc_outer_device.argtypes = [C.c_void_p, C.c_double, C.c_void_p, C.c_void_p, C.c_void_p]
     L.spaqr_colscale.argtypes = [C.c_int, C.c_int, _ip, _ip, _dp, _dp]
     L.spaqr_random.argtypes = [C.c_int, C.c_int, _dp]
     L.spaqr_perms.argtypes = [C.c_void_p, _ip, _ip]
@@ -202,6 +208,17 @@ class Tree:
         """Device-pointer apply (e.g. torch tensors' data_ptr())."""
         _check(load_library().spaqr_solve_device(self.h, C.c_void_p(b_ptr), C.c_void_p(x_ptr)), "solve_device")
 
+    def solve_transpose(self, b):
+        """x = M^{-T} b (square trees): the adjoint of solve()."""
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        x = np.zeros(self.m)
+        _check(load_library().spaqr_solve_transpose(self.h, _d(b), _d(x)), "solve_transpose")
+        return x
+
+    def solve_transpose_device(self, b_ptr, x_ptr):
+        _check(load_library().spaqr_solve_transpose_device(self.h, C.c_void_p(b_ptr), C.c_void_p(x_ptr)),
+               "solve_transpose_device")
+
     def solve_nrml(self, b):
         """tree.cpp:1471-1517: x = (W^T W)^{-1} b (least-squares tree, b of length cols)."""
         b = np.ascontiguousarray(b, dtype=np.float64)
@@ -275,7 +292,7 @@ class Matrix:
 
     def __init__(self, tree: Tree, A):
         m, n, cp, ri, v = _csc_arrays(A)
-        self.m, self.n = m, n
+        self.m, self.n, self.nnz = m, n, int(cp[-1])
         self.h = load_library().spaqr_matrix_create(tree.h, m, n, _i(cp), _i(ri), _d(v))
         if not self.h:
             raise SpaqrError("matrix upload failed: " + load_library().spaqr_last_error().decode())
@@ -304,6 +321,22 @@ def gmres_device(tree: Tree, M: Matrix, b_ptr, x_ptr, iters=300, restart=None, t
                                              restart or iters, tol, verbose, C.byref(it), C.byref(est)),
            "gmres_device")
     return it.value, est.value
+
+
+def gmres_transpose_device(tree: Tree, M: Matrix, b_ptr, x_ptr, iters=300, restart=None, tol=1e-12, verbose=0):
+    """GMRES on A^T x = b preconditioned with M^{-T} (the adjoint solve)."""
+    it = C.c_int(0)
+    est = C.c_double(0)
+    _check(load_library().spaqr_gmres_transpose_device(tree.h, M.h, C.c_void_p(b_ptr), C.c_void_p(x_ptr), iters,
+                                                       restart or iters, tol, verbose, C.byref(it), C.byref(est)),
+           "gmres_transpose_device")
+    return it.value, est.value
+
+
+def csc_outer_device(M: Matrix, alpha, u_ptr, v_ptr, out_ptr):
+    """out[k] = alpha * u[i_k] * v[j_k] over M's nonzeros in CSC order."""
+    _check(load_library().spaqr_csc_outer_device(M.h, alpha, C.c_void_p(u_ptr), C.c_void_p(v_ptr),
+                                                 C.c_void_p(out_ptr)), "csc_outer_device")
 
 
 def cgls(tree: Tree, M: Matrix, b, x0=None, iters=300, tol=1e-10, verbose=0):

@@ -8,6 +8,7 @@
 #include "../../include/spaqr.h"
 #include "host/engine.hpp"
 #include "host/krylov.hpp"
+#include "kernels/launch.hpp"
 
 using namespace spq;
 
@@ -183,7 +184,7 @@ spaqr_matrix* spaqr_matrix_create(spaqr_tree* t, int m, int n, const int* colptr
     eng(t);
     SPQ_HIP(hipSetDevice(t->device));
     M = new spaqr_matrix;
-    M->d.reset(upload_matrix(make_csc(m, n, colptr, rowind, val), m != n));
+    M->d.reset(upload_matrix(make_csc(m, n, colptr, rowind, val), true));
   });
   if (rc != SPAQR_OK) { delete M; return nullptr; }
   return M;
@@ -201,6 +202,53 @@ int spaqr_gmres_device(spaqr_tree* t, const spaqr_matrix* A, const double* b, do
     e.check_solve();
     if (iters_out) *iters_out = r.iters;
     if (est_out) *est_out = r.est;
+  });
+}
+int spaqr_gmres_transpose_device(spaqr_tree* t, const spaqr_matrix* A, const double* b, double* x, int iters,
+                                 int restart, double tol, int verbose, int* iters_out, double* est_out) {
+  return guard([&] {
+    EngineBase& e = eng(t);
+    if (!A || !A->d) throw Error(kErrArg, "null matrix");
+    if (A->d->m != A->d->n || A->d->m != e.rows()) throw Error(kErrArg, "gmres needs the square matrix of the tree");
+    KrylovResult r = e.precision_bytes() == 4 ? gmres_ir_dev(e, *A->d, b, x, iters, restart, tol, verbose, true)
+                                               : gmres_dev(e, *A->d, b, x, iters, restart, tol, verbose, true);
+    e.check_solve();
+    if (iters_out) *iters_out = r.iters;
+    if (est_out) *est_out = r.est;
+  });
+}
+int spaqr_solve_transpose_device(spaqr_tree* t, const double* b, double* x) {
+  return guard([&] {
+    EngineBase& e = eng(t);
+    if (!b || !x) throw Error(kErrArg, "null vector");
+    e.solve_t_dev(b, x);
+    SPQ_HIP(hipStreamSynchronize(e.stream()));
+    e.check_solve();
+  });
+}
+int spaqr_solve_transpose(spaqr_tree* t, const double* b, double* x) {
+  return guard([&] {
+    EngineBase& e = eng(t);
+    if (!b || !x) throw Error(kErrArg, "null vector");
+    const int n = e.rows();
+    double* d = nullptr;
+    SPQ_HIP(hipMalloc((void**)&d, sizeof(double) * std::max(n, 1)));
+    struct F { double* p; ~F() { hipFree(p); } } fd{d};
+    SPQ_HIP(hipMemcpy(d, b, sizeof(double) * n, hipMemcpyHostToDevice));
+    e.solve_t_dev(d, d);
+    SPQ_HIP(hipStreamSynchronize(e.stream()));
+    e.check_solve();
+    SPQ_HIP(hipMemcpy(x, d, sizeof(double) * n, hipMemcpyDeviceToHost));
+  });
+}
+int spaqr_csc_outer_device(const spaqr_matrix* A, double alpha, const double* u, const double* v, double* out) {
+  return guard([&] {
+    if (!A || !A->d) throw Error(kErrArg, "null matrix");
+    if (!u || !v || !out) throw Error(kErrArg, "null vector");
+    const DevMatrix& d = *A->d;
+    launch_csc_outer(nullptr, d.n, d.nnz, d.rpt, d.cit, alpha, u, v, out);
+    SPQ_HIP(hipGetLastError());
+    SPQ_HIP(hipStreamSynchronize(nullptr));
   });
 }
 int spaqr_cgls_device(spaqr_tree* t, const spaqr_matrix* A, const double* b, double* x, int iters, double tol,

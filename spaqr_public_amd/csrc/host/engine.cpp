@@ -45,6 +45,7 @@ class Engine final : public EngineBase {
     for (auto e : ev_pool_) (void)hipEventDestroy(e);
     if (graph_exec_fwd_) hipGraphExecDestroy(graph_exec_fwd_);
     if (graph_exec_nrml_) hipGraphExecDestroy(graph_exec_nrml_);
+    if (graph_exec_t_) hipGraphExecDestroy(graph_exec_t_);
     if (xtbuf_) hipFree(xtbuf_);
     if (rank_host_) hipHostFree(rank_host_);
     if (xbuf_) hipFree(xbuf_);
@@ -108,6 +109,7 @@ class Engine final : public EngineBase {
   int factorize() override;
   void solve_dev(const double* b, double* x) override;
   void solve_nrml_dev(const double* b, double* x) override;
+  void solve_t_dev(const double* b, double* x) override;
 
   std::vector<std::array<int, 12>> cluster_table() const override {
     std::vector<std::array<int, 12>> t;
@@ -540,7 +542,11 @@ class Engine final : public EngineBase {
   hipGraphExec_t graph_exec_fwd_ = nullptr;
   double* bin_ = nullptr;
   double* xout_ = nullptr;
-  void run_program(const double* b, double* x, bool prof = false, bool nrml = false);
+  // mode 0: M^{-1}; 1: normal equations (W^T W)^{-1}; 2: M^{-T}
+  void run_program(const double* b, double* x, bool prof = false, int mode = 0);
+  // M^{-T} (square trees): transposes of the backward, then forward programs
+  std::vector<Stage> afwd_, abwd_;
+  hipGraphExec_t graph_exec_t_ = nullptr;
   // normal-equation programs (least squares) and their entry map
   std::vector<Stage> tfwd_, tbwd_, pfwd_;
   T* xtbuf_ = nullptr;
@@ -1861,6 +1867,30 @@ void Engine<T>::build_solve_program() {
       push(hpf, std::move(smg));
     }
   }
+  // ---- x <- M^{-T} x (square): M^{-1} = S B F G with F, B the programs
+  // above and G, S the entry and exit maps, so M^{-T} = G^T F^T B^T S^T.  A
+  // program's transpose runs its stages in reverse, each transposed:
+  // reflector and triangular stages flip `trans`, R12 products become
+  // x_n -= R12^T x_c, and moves run dst -> src.  Every cluster owns its own
+  // x segment, so a move's source is dead after it and its destination dead
+  // before it, which makes the reversed move the exact transpose on the
+  // values that are read.
+  std::vector<HStage> haf, hab;
+  if (square_) {
+    auto transpose = [](const std::vector<HStage>& src, std::vector<HStage>& dst) {
+      for (auto it = src.rbegin(); it != src.rend(); ++it) {
+        HStage st = *it;
+        for (auto& d : st.refl) d.trans ^= 1;
+        for (auto& d : st.bwd) d.trans ^= 1;
+        for (auto& d : st.r12c) d.trans ^= 1;
+        for (auto& d : st.mv) std::swap(d.src, d.dst);
+        std::swap(st.buf, st.sbuf);
+        dst.push_back(std::move(st));
+      }
+    };
+    transpose(hb, haf);
+    transpose(hf, hab);
+  }
   segs_dev_ = stager_.upload(perm_, segs);
   r12_dev_ = stager_.upload(perm_, r12);
   // Multi-workgroup split of large ops: row slices of <= 256 rows for
@@ -1946,6 +1976,8 @@ void Engine<T>::build_solve_program() {
   finish(htf, tfwd_);
   finish(htb, tbwd_);
   finish(hpf, pfwd_);
+  finish(haf, afwd_);
+  finish(hab, abwd_);
   n_solve_sync_ = nsync;
   solve_sync_ = (unsigned int*)perm_.alloc(sizeof(unsigned int) * std::max(nsync, 1));
   solve_err_ = (int*)perm_.alloc(256);
@@ -1977,6 +2009,8 @@ void Engine<T>::build_solve_program() {
   upload_stage(htf, tfwd_);
   upload_stage(htb, tbwd_);
   upload_stage(hpf, pfwd_);
+  upload_stage(haf, afwd_);
+  upload_stage(hab, abwd_);
   if (verbose_ > 1) {
     auto dump = [&](const char* dir, std::vector<HStage>& hs) {
       for (auto& h : hs) {
@@ -2020,7 +2054,7 @@ void Engine<T>::build_solve_program() {
 }
 
 template <typename T>
-void Engine<T>::run_program(const double* b, double* x, bool prof, bool nrml) {
+void Engine<T>::run_program(const double* b, double* x, bool prof, int mode) {
   launch_zero(stream_, sizeof(T), xbuf_, xsize_);
   if (n_solve_sync_ > 0) launch_zero(stream_, 4, solve_sync_, n_solve_sync_);
   std::vector<hipEvent_t> ev;
@@ -2037,10 +2071,14 @@ void Engine<T>::run_program(const double* b, double* x, bool prof, bool nrml) {
     else launch_move<T>(stream_, s.mv, s.nd, xb, s.sbuf ? xtbuf_ : xbuf_);
   };
   std::vector<const std::vector<Stage>*> progs;
-  if (nrml) {
+  if (mode == 1) {
     launch_zero(stream_, sizeof(T), xtbuf_, xsize_);
     launch_gather_in<T>(stream_, tin_dst_, tin_src_, n_tin_, b, xtbuf_);
     progs = {&tfwd_, &tbwd_, &pfwd_, &bwd_};
+  } else if (mode == 2) {
+    // S^T: b enters through the exit map
+    launch_gather_in<T>(stream_, out_src_, out_dst_, n_out_, b, xbuf_);
+    progs = {&afwd_, &abwd_};
   } else {
     launch_gather_in<T>(stream_, in_dst_, in_src_, n_in_, b, xbuf_);
     progs = {&fwd_, &bwd_};
@@ -2066,7 +2104,8 @@ void Engine<T>::run_program(const double* b, double* x, bool prof, bool nrml) {
     fprintf(stderr, "stages total ms %.3f\n", tot);
     for (auto e : ev) (void)hipEventDestroy(e);
   }
-  launch_scatter_out<T>(stream_, out_src_, out_dst_, n_out_, xbuf_, x);
+  if (mode == 2) launch_scatter_out<T>(stream_, in_dst_, in_src_, n_in_, xbuf_, x);  // G^T
+  else launch_scatter_out<T>(stream_, out_src_, out_dst_, n_out_, xbuf_, x);
 }
 
 template <typename T>
@@ -2101,12 +2140,12 @@ void Engine<T>::solve_nrml_dev(const double* b, double* x) {
   if (square_) throw Error(kErrState, "solve_nrml: the tree was factored as a square system (set_square(0) first)");
   SPQ_HIP(hipMemcpyAsync(bin_, b, sizeof(double) * ncols_, hipMemcpyDeviceToDevice, stream_));
   if (stage_prof_ || !solve_graph_) {
-    timed("apply", 0.0, solve_bytes_, [&] { run_program(bin_, xout_, stage_prof_, true); });
+    timed("apply", 0.0, solve_bytes_, [&] { run_program(bin_, xout_, stage_prof_, 1); });
   } else {
     if (!graph_exec_nrml_) {
       hipGraph_t g = nullptr;
       SPQ_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
-      run_program(bin_, xout_, false, true);
+      run_program(bin_, xout_, false, 1);
       SPQ_HIP(hipStreamEndCapture(stream_, &g));
       SPQ_HIP(hipGraphInstantiate(&graph_exec_nrml_, g, nullptr, nullptr, 0));
       hipGraphDestroy(g);
@@ -2114,6 +2153,29 @@ void Engine<T>::solve_nrml_dev(const double* b, double* x) {
     timed("apply", 0.0, solve_bytes_, [&] { SPQ_HIP(hipGraphLaunch(graph_exec_nrml_, stream_)); });
   }
   SPQ_HIP(hipMemcpyAsync(x, xout_, sizeof(double) * ncols_, hipMemcpyDeviceToDevice, stream_));
+}
+
+// x = M^{-T} b (square trees; the preconditioner of the adjoint solve with
+// A^T); b may alias x.
+template <typename T>
+void Engine<T>::solve_t_dev(const double* b, double* x) {
+  if (!factored_) throw Error(kErrState, "solve_transpose before factorize");
+  if (!square_) throw Error(kErrState, "solve_transpose: rectangular (least-squares) tree");
+  SPQ_HIP(hipMemcpyAsync(bin_, b, sizeof(double) * ncols_, hipMemcpyDeviceToDevice, stream_));
+  if (stage_prof_ || !solve_graph_) {
+    timed("apply", 0.0, solve_bytes_, [&] { run_program(bin_, xout_, stage_prof_, 2); });
+  } else {
+    if (!graph_exec_t_) {
+      hipGraph_t g = nullptr;
+      SPQ_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+      run_program(bin_, xout_, false, 2);
+      SPQ_HIP(hipStreamEndCapture(stream_, &g));
+      SPQ_HIP(hipGraphInstantiate(&graph_exec_t_, g, nullptr, nullptr, 0));
+      hipGraphDestroy(g);
+    }
+    timed("apply", 0.0, solve_bytes_, [&] { SPQ_HIP(hipGraphLaunch(graph_exec_t_, stream_)); });
+  }
+  SPQ_HIP(hipMemcpyAsync(x, xout_, sizeof(double) * nrows_, hipMemcpyDeviceToDevice, stream_));
 }
 
 EngineBase* make_engine(int nlevels, int skip, int precision_bytes, int device) {

@@ -54,6 +54,8 @@ class EngineBase {
   // x = M^{-1} b with device FP64 vectors (tree.cpp:1440-1466)
   virtual void solve_dev(const double* b, double* x) = 0;
   virtual void solve_nrml_dev(const double* b, double* x) = 0;
+  // x = M^{-T} b (square trees): the transposed ops, backward program first
+  virtual void solve_t_dev(const double* b, double* x) = 0;
   virtual hipStream_t stream() const = 0;
   // inspection
   virtual const std::vector<int>& cperm() const = 0;

@@ -37,10 +37,14 @@ DevMatrix* upload_matrix(const CscMat& A, bool with_transpose) {
 }
 
 KrylovResult gmres_dev(EngineBase& M, const DevMatrix& A, const double* b, double* x, int iters, int restart,
-                       double tol, int verbose) {
+                       double tol, int verbose, bool trans) {
   KrylovResult res;
   hipStream_t s = M.stream();
   const int n = A.m;
+  if (trans && !A.rpt) throw Error(kErrArg, "gmres: the matrix was uploaded without its transpose");
+  const int* rp = trans ? A.rpt : A.rp;
+  const int* ci = trans ? A.cit : A.ci;
+  const double* av = trans ? A.vt : A.v;
   restart = std::max(1, std::min(restart, n));
   const double t0 = wall();
   double *V = nullptr, *w = nullptr, *p0 = nullptr, *h = nullptr, *hh = nullptr;
@@ -58,7 +62,8 @@ KrylovResult gmres_dev(EngineBase& M, const DevMatrix& A, const double* b, doubl
   };
   auto precond = [&](const double* in, double* out) {
     const double a = wall();
-    M.solve_dev(in, out);
+    if (trans) M.solve_t_dev(in, out);
+    else M.solve_dev(in, out);
     if (verbose) { SPQ_HIP(hipStreamSynchronize(s)); res.t_precond += wall() - a; }
     res.n_precond++;
   };
@@ -73,7 +78,7 @@ KrylovResult gmres_dev(EngineBase& M, const DevMatrix& A, const double* b, doubl
   for (;;) {
     // r0 = M^{-1} (b - A x)
     SPQ_HIP(hipMemcpyAsync(p0, b, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
-    launch_spmv(s, n, A.rp, A.ci, A.v, x, p0, -1.0, 1.0);
+    launch_spmv(s, n, rp, ci, av, x, p0, -1.0, 1.0);
     precond(p0, V);
     const double beta = norm_dev(V);
     if (beta == 0.0) break;
@@ -87,7 +92,7 @@ KrylovResult gmres_dev(EngineBase& M, const DevMatrix& A, const double* b, doubl
       ++k;
       double* vk = V + (size_t)n * (k - 1);
       double* vn = V + (size_t)n * k;
-      launch_spmv(s, n, A.rp, A.ci, A.v, vk, w, 1.0, 0.0);
+      launch_spmv(s, n, rp, ci, av, vk, w, 1.0, 0.0);
       precond(w, vn);
       // CGS2 against V(:, 0:k)
       launch_dots(s, n, k, V, n, vn, h);
@@ -139,8 +144,9 @@ KrylovResult gmres_dev(EngineBase& M, const DevMatrix& A, const double* b, doubl
 }
 
 KrylovResult gmres_ir_dev(EngineBase& M, const DevMatrix& A, const double* b, double* x, int iters, int restart,
-                          double tol, int verbose) {
+                          double tol, int verbose, bool trans) {
   KrylovResult res;
+  if (trans && !A.rpt) throw Error(kErrArg, "gmres: the matrix was uploaded without its transpose");
   hipStream_t s = M.stream();
   const int n = A.m;
   const double t0 = wall();
@@ -170,7 +176,8 @@ KrylovResult gmres_ir_dev(EngineBase& M, const DevMatrix& A, const double* b, do
   double prev = 0;
   for (;;) {
     SPQ_HIP(hipMemcpyAsync(r, b, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
-    launch_spmv(s, n, A.rp, A.ci, A.v, x, r, -1.0, 1.0);
+    if (trans) launch_spmv(s, n, A.rpt, A.cit, A.vt, x, r, -1.0, 1.0);
+    else launch_spmv(s, n, A.rp, A.ci, A.v, x, r, -1.0, 1.0);
     const double rel = norm(r) / bn;
     res.est = rel;
     if (verbose) printf("refinement %d: |b - Ax|/|b| = %3.2e (%d inner iterations)\n", res.outer, rel, res.iters);
@@ -181,7 +188,7 @@ KrylovResult gmres_ir_dev(EngineBase& M, const DevMatrix& A, const double* b, do
     const int left = iters - res.iters;
     // no tighter than this step needs to reach tol (with a factor 2 margin)
     const double inner_tol = std::max(kIrInnerTol, 0.5 * tol / rel);
-    KrylovResult in = gmres_dev(M, A, r, d, left, std::min(restart, left), inner_tol, verbose > 1 ? verbose : 0);
+    KrylovResult in = gmres_dev(M, A, r, d, left, std::min(restart, left), inner_tol, verbose > 1 ? verbose : 0, trans);
     res.iters += in.iters;
     res.n_precond += in.n_precond;
     res.t_precond += in.t_precond;

@@ -27,8 +27,10 @@ struct KrylovResult {
 // Left-preconditioned restarted GMRES: r0 = M^{-1}(b - A x), stop when
 // |g_k| / ||r0|| < tol (is.cpp:217-218), Arnoldi by classical Gram-Schmidt
 // with one reorthogonalisation, FP64 throughout.  b, x are device vectors.
+// trans: solve A^T x = b instead, with A^T from the uploaded transpose and
+// M^{-T} as the preconditioner (the adjoint solve).
 KrylovResult gmres_dev(EngineBase& M, const DevMatrix& A, const double* b, double* x, int iters, int restart,
-                       double tol, int verbose);
+                       double tol, int verbose, bool trans = false);
 
 // Mixed form: FP64 iterative refinement around gmres_dev for a tree factored
 // in FP32.  Each step forms the true residual r = b - A x with the FP64 SpMV,
@@ -41,7 +43,7 @@ KrylovResult gmres_dev(EngineBase& M, const DevMatrix& A, const double* b, doubl
 constexpr double kIrInnerTol = 1e-6;
 constexpr int kIrMaxOuter = 20;
 KrylovResult gmres_ir_dev(EngineBase& M, const DevMatrix& A, const double* b, double* x, int iters, int restart,
-                          double tol, int verbose);
+                          double tol, int verbose, bool trans = false);
 
 // Preconditioned CGLS on the normal equations (is.cpp:275-346): stop when
 // ||A^T r||^2 < tol^2 ||A^T b||^2; est = ||A^T r|| / ||A^T b||.  b has length

@@ -58,6 +58,29 @@ __global__ __launch_bounds__(256) void k_gemv(int n, int ncol, const double* __r
   y[i] = s;
 }
 
+// out[k] = alpha * u[rowind[k]] * v[j] for the nonzeros k of column j of a
+// CSC pattern (one thread per nonzero; j by binary search in colptr).  With
+// u = lambda, v = x and alpha = -1 this is dL/dA on A's pattern for
+// x = A^{-1} b and A^T lambda = dL/dx.
+__global__ __launch_bounds__(256) void k_csc_outer(int n, int nnz, const int* __restrict__ colptr,
+                                                   const int* __restrict__ rowind, double alpha,
+                                                   const double* __restrict__ u, const double* __restrict__ v,
+                                                   double* __restrict__ out) {
+  const int k = blockIdx.x * 256 + threadIdx.x;
+  if (k >= nnz) return;
+  int lo = 0, hi = n - 1;  // last column with colptr[j] <= k
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (colptr[mid] <= k) lo = mid; else hi = mid - 1;
+  }
+  out[k] = alpha * u[rowind[k]] * v[lo];
+}
+void launch_csc_outer(hipStream_t s, int n, int nnz, const int* colptr, const int* rowind, double alpha,
+                      const double* u, const double* v, double* out) {
+  if (n <= 0 || nnz <= 0) return;
+  hipLaunchKernelGGL(k_csc_outer, dim3((nnz + 255) / 256), dim3(256), 0, s, n, nnz, colptr, rowind, alpha, u, v, out);
+}
+
 void launch_spmv(hipStream_t s, int n, const int* rp, const int* ci, const double* v, const double* x, double* y,
                  double alpha, double beta) {
   if (n <= 0) return;

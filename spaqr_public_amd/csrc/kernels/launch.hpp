@@ -85,5 +85,7 @@ void launch_dots(hipStream_t s, int n, int ncol, const double* V, int64_t ld, co
 void launch_gemv_sub(hipStream_t s, int n, int ncol, const double* V, int64_t ld, const double* h, double* w);
 void launch_gemv(hipStream_t s, int n, int ncol, const double* V, int64_t ld, const double* h, double* y);
 void launch_axpby(hipStream_t s, int n, double a, const double* x, double b, double* y);
+void launch_csc_outer(hipStream_t s, int n, int nnz, const int* colptr, const int* rowind, double alpha,
+                      const double* u, const double* v, double* out);
 
 }  // namespace spq
