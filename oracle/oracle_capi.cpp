@@ -52,6 +52,11 @@ void orc_geqrf(int m, int n, double* A, double* tau) {
   std::memcpy(A, M.data(), sizeof(double) * (size_t)m * n);
   std::memcpy(tau, t.data(), sizeof(double) * t.size());
 }
+// singular values of an m x n matrix, descending (min(m, n) of them)
+void orc_singular_values(int m, int n, const double* A, double* S) {
+  std::vector<double> s = singular_values(make_mat(m, n, A));
+  std::memcpy(S, s.data(), sizeof(double) * s.size());
+}
 void orc_larft(int m, int k, const double* V, const double* tau, double* T) {
   Mat Vm = make_mat(m, k, V), Tm;
   std::vector<double> t(tau, tau + k);

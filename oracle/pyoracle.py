@@ -109,6 +109,15 @@ def geqrf(A):
     return A, tau
 
 
+def singular_values(A):
+    """util.cpp:767-782 (gesvd_values) as restated: descending."""
+    A = np.array(A, dtype=np.float64, order="F")
+    m, n = A.shape
+    S = np.zeros(min(m, n))
+    lib().orc_singular_values(m, n, _d(A), _d(S))
+    return S
+
+
 def larft(V, tau):
     V = np.array(V, dtype=np.float64, order="F")
     m, k = V.shape

@@ -174,6 +174,8 @@ class Engine final : public EngineBase {
     // SplitD
     int crows = 0, ccols = 0, rank = 0;
     int wave = 0;  // sparsify wave (OrthoD, SplitD, FineQR)
+    bool nofwd = false;     // OrthoD without a forward step (Orthogonal, operations.h:113-129)
+    bool fwd_refl = false;  // FineQR with reflectors (the --scale 0 fine elimination)
     // Reassign: x_f[nstart + t] = x_c[idx[t]]
     std::vector<int> idx;
     int nstart = 0;
@@ -207,18 +209,23 @@ class Engine final : public EngineBase {
     b.p = alloc_perm((size_t)b.ld * std::max(cols, 1));
     return b;
   }
-  int add_edge(int c1, int c2, std::vector<Block>* zero_list) {  // tree.cpp:251-261
+  // link_nbr / link_in = false: the edge joins c1's out list only (the fine
+  // cluster edges of split_edges, tree.cpp:480-530, which touch neither the
+  // column neighbours of c1 nor the in list of c2 unless stated)
+  int add_edge(int c1, int c2, std::vector<Block>* zero_list, bool link_nbr = true, bool link_in = true) {  // tree.cpp:251-261
     Edge e;
     e.n1 = c1; e.n2 = c2;
     e.a = new_block(cl_[c2].rsize, cl_[c1].csize);
     if (zero_list) zero_list->push_back(e.a);
     ed_.push_back(e);
     const int id = (int)ed_.size() - 1;
-    auto& cn = cl_[c1].cnbrs;
-    auto it = std::lower_bound(cn.begin(), cn.end(), c2);
-    if (it == cn.end() || *it != c2) cn.insert(it, c2);
+    if (link_nbr) {
+      auto& cn = cl_[c1].cnbrs;
+      auto it = std::lower_bound(cn.begin(), cn.end(), c2);
+      if (it == cn.end() || *it != c2) cn.insert(it, c2);
+    }
     cl_[c1].out.push_back(id);
-    if (c1 != c2) cl_[c2].in.push_back(id);
+    if (c1 != c2 && link_in) cl_[c2].in.push_back(id);
     return id;
   }
   int find_out(int c, int n2) const {
@@ -456,7 +463,10 @@ class Engine final : public EngineBase {
 
   // ------------------------------------------------------------ phases
   void get_sparsity(int c);
-  void eliminate_level(const std::vector<int>& cs);
+  // fine_wave >= 0: the inline elimination of --scale 0 fine clusters of
+  // that sparsify wave (ops recorded as FineQR with reflectors)
+  void eliminate_level(const std::vector<int>& cs, int fine_wave = -1);
+  void sparsify0_level(const std::vector<int>& cs);
   void scale_level(const std::vector<int>& cs);
   void sparsify_level(const std::vector<int>& cs);
   void merge_all();
@@ -727,7 +737,7 @@ void Engine<T>::get_sparsity(int c) {  // tree.cpp:834-870 (+ check_if_valid 168
 // tree.cpp:780-832 (eliminate_cluster, householder, update_cluster) for every
 // cluster of the level at once.
 template <typename T>
-void Engine<T>::eliminate_level(const std::vector<int>& cs) {
+void Engine<T>::eliminate_level(const std::vector<int>& cs, int fine_wave) {
   struct Plan {
     int c, m, k, Nw;
     std::vector<int> pe;                  // panel edges
@@ -784,8 +794,10 @@ void Engine<T>::eliminate_level(const std::vector<int>& cs) {
       nw = std::max(nw, w + 1);
     }
   }
-  info_[ilvl_].elim_waves = nw;
-  info_[ilvl_].n_elim = (int)plans.size();
+  if (fine_wave < 0) {
+    info_[ilvl_].elim_waves = nw;
+    info_[ilvl_].n_elim = (int)plans.size();
+  }
   for (auto& p : plans) {
     p.P = alloc_perm((size_t)p.m * p.k);
     p.tau = alloc_perm(p.k);
@@ -826,7 +838,8 @@ void Engine<T>::eliminate_level(const std::vector<int>& cs) {
   // ops and bookkeeping (tree.cpp:826-829)
   for (auto& p : plans) {
     Op o;
-    o.kind = OpQR; o.level = ilvl_; o.c = p.c;
+    o.kind = fine_wave < 0 ? OpQR : OpFineQR; o.level = ilvl_; o.c = p.c;
+    if (fine_wave >= 0) { o.wave = fine_wave; o.fwd_refl = true; }
     o.V = p.P; o.ldv = p.m; o.m = p.m; o.k = p.k; o.Tb = p.Tb;
     for (int e : p.pe) o.segs.push_back({ed_[e].n2, cl_[ed_[e].n2].rsize});
     o.R = p.P; o.ldr = p.m; o.kr = p.k;
@@ -1328,6 +1341,260 @@ void Engine<T>::sparsify_level(const std::vector<int>& cs) {
   sprs_.insert(sprs_.end(), recs.begin(), recs.end());
 }
 
+// tree.cpp:887-989 (sparsify, --scale 0) for every interface of the level,
+// wave by wave as in sparsify_level: per interface
+//   s = sigma_min([App; Ampc])             (QR, then k_smin on R)
+//   C = [Ampc^T, (1/s) App^T Apmc]         (gathers, GEMMs, k_scale_inv)
+//   geqp3 + choose_rank on C               (rrqr_batch)
+//   App <- App Q                           (larft, three GEMMs per block)
+//   Amp blocks <- triu(R[0:rank]) P^T      (from the RRQR output)
+// then the fine cluster (the last k - rank columns and rows), split_edges
+// (tree.cpp:480-530) and the inline eliminate_cluster(cf) (tree.cpp:988),
+// batched over the wave.  Square trees only (least squares forces scale 1).
+template <typename T>
+void Engine<T>::sparsify0_level(const std::vector<int>& cs) {
+  if (!square_) throw Error(kErrState, "--scale 0 sparsification needs a square tree");
+  std::vector<int> S;
+  for (int c : cs)
+    if (cl_[c].csize > 0) S.push_back(c);
+  std::vector<int> slot(cl_.size(), -1);
+  for (size_t i = 0; i < S.size(); ++i) slot[S[i]] = (int)i;
+  std::vector<int> wave(S.size(), 0);
+  int nw = S.empty() ? 0 : 1;
+  for (size_t i = 0; i < S.size(); ++i) {
+    int w = 0;
+    auto see = [&](int n) {
+      const int jj = n >= 0 && n < (int)slot.size() ? slot[n] : -1;
+      if (jj >= 0 && jj < (int)i) w = std::max(w, wave[jj] + 1);
+    };
+    for (int e : cl_[S[i]].out) if (ed_[e].n2 != S[i]) see(ed_[e].n2);
+    for (int e : cl_[S[i]].in) if (ed_[e].n1 != S[i]) see(ed_[e].n1);
+    wave[i] = w;
+    nw = std::max(nw, w + 1);
+  }
+  info_[ilvl_].sprs_waves = nw;
+  info_[ilvl_].n_sprs = (int)S.size();
+  std::vector<SparsifyInfo> recs(S.size());
+  for (int w = 0; w < nw; ++w) {
+    scratch_.rewind();
+    struct Job {
+      int c, k, rows, N, smp, spm, es, map;
+      std::vector<int> amp, apm;
+      T *Ap, *Ct;
+    };
+    std::vector<Job> jobs;
+    std::vector<CopyDesc> gather;
+    std::vector<QrJob> qr;
+    for (size_t i = 0; i < S.size(); ++i) {
+      if (wave[i] != w) continue;
+      const int c = S[i];
+      Job jb;
+      jb.c = c; jb.k = cl_[c].csize; jb.rows = cl_[c].rsize; jb.smp = 0; jb.spm = 0;
+      jb.es = find_out(c, c);
+      if (jb.es < 0) throw Error(kErrState, "sparsify: interface without a self block");
+      if (jb.k > kSminMaxK) throw Error(kErrArg, "--scale 0: interface wider than the sigma_min kernel limit");
+      for (int e : cl_[c].in)
+        if (!cl_[ed_[e].n1].elim && ed_[e].n1 != c) { jb.apm.push_back(e); jb.spm += ed_[e].a.cols; }
+      for (int e : cl_[c].out)
+        if (!cl_[ed_[e].n2].elim && ed_[e].n2 != c) { jb.amp.push_back(e); jb.smp += ed_[e].a.rows; }
+      jb.N = jb.smp + jb.spm;
+      jb.map = jb.rows + jb.smp;
+      // Ap = [App; Ampc] (tree.cpp:921-923), factorised for its R
+      jb.Ap = alloc_scr((size_t)jb.map * jb.k);
+      const Block& self = ed_[jb.es].a;
+      gather.push_back(CopyDesc{self.p, jb.Ap, self.ld, jb.map, jb.rows, jb.k, 0, 0});
+      // Ct = C^T: rows [0, smp) = Ampc, rows [smp, N) = (1/s) Apmc^T App
+      const int N = std::max(jb.N, 1);
+      jb.Ct = alloc_scr((size_t)N * jb.k);
+      int off = 0;
+      for (int e : jb.amp) {
+        const Block& b = ed_[e].a;
+        gather.push_back(CopyDesc{b.p, jb.Ap + jb.rows + off, b.ld, jb.map, b.rows, jb.k, 0, 0});
+        gather.push_back(CopyDesc{b.p, jb.Ct + off, b.ld, N, b.rows, jb.k, 0, 0});
+        off += b.rows;
+      }
+      QrJob q;
+      q.P = jb.Ap; q.ld = jb.map; q.m = jb.map; q.k = jb.k;
+      q.tau = alloc_scr(jb.k);
+      q.Tb = alloc_scr((size_t)nb_ * iceil(jb.k, nb_) * nb_);
+      qr.push_back(std::move(q));
+      jobs.push_back(std::move(jb));
+    }
+    copy_blocks(gather);
+    // P blocks: Ct[smp + q0 : ..., :] = A(c, n)^T App, before App changes
+    std::vector<GemmDesc<T>> gp;
+    for (auto& jb : jobs) {
+      const Block& self = ed_[jb.es].a;
+      const int N = std::max(jb.N, 1);
+      int off = jb.smp;
+      for (int e : jb.apm) {
+        const Block& b = ed_[e].a;
+        if (b.cols > 0 && jb.rows > 0)
+          gp.push_back(gd(b.p, b.ld, self.p, self.ld, jb.Ct + off, N, b.cols, jb.k, jb.rows,
+                          gemm_flags(true, false, kStructNone, kStructNone), T(1), T(0)));
+        off += b.cols;
+      }
+    }
+    gemm_batch(gp);
+    blocked_qr(qr);
+    // sigma_min of each R, then the (1/s) scaling of the P rows
+    {
+      std::vector<SminDesc<T>> sd;
+      int kmax = 0;
+      if (jobs.size() > 16000) throw Error(kErrArg, "--scale 0: too many interfaces in one sparsification wave");
+      double* sdev = (double*)scratch_.alloc(sizeof(double) * std::max<size_t>(jobs.size(), 1));
+      for (size_t q = 0; q < jobs.size(); ++q) {
+        SminDesc<T> d;
+        d.R = jobs[q].Ap; d.ld = jobs[q].map; d.k = jobs[q].k;
+        d.work = (double*)scratch_.alloc(sizeof(double) * smin_work_elems(jobs[q].k));
+        d.out = sdev + q;
+        sd.push_back(d);
+        kmax = std::max(kmax, jobs[q].k);
+      }
+      const SminDesc<T>* dsd = stager_.push(sd);
+      timed("smin", 0.0, 0.0, [&] { launch_smin<T>(stream_, dsd, (int)sd.size(), kmax); });
+      std::vector<ScaleInvDesc<T>> si;
+      std::vector<int> cnt, st;
+      for (size_t q = 0; q < jobs.size(); ++q) {
+        const Job& jb = jobs[q];
+        if (jb.spm == 0) continue;
+        si.push_back(ScaleInvDesc<T>{jb.Ct, sdev + q, std::max(jb.N, 1), jb.smp, jb.N, jb.k});
+        cnt.push_back(iceil(jb.spm * jb.k, 256));
+      }
+      if (!si.empty()) {
+        const int nt = prefix_tiles(cnt, st);
+        const ScaleInvDesc<T>* dsi = stager_.push(si);
+        const int* dst = stager_.push(st);
+        timed("copy", 0.0, 0.0, [&] { launch_scale_inv<T>(stream_, dsi, dst, (int)si.size(), nt); });
+      }
+      // an exactly singular Ap gives s = 0 (the reference then divides by 0)
+      SPQ_HIP(hipMemcpyAsync(rank_host_ + 32768, sdev, sizeof(double) * jobs.size(), hipMemcpyDeviceToHost, stream_));
+    }
+    std::vector<RrqrJob> rj;
+    for (auto& jb : jobs) rj.push_back(RrqrJob{jb.Ct, jb.k, jb.N});
+    rrqr_batch(rj);  // synchronises (rank read-back)
+    for (size_t q = 0; q < jobs.size(); ++q) {
+      double sv;
+      std::memcpy(&sv, (const char*)(rank_host_ + 32768) + sizeof(double) * q, sizeof(double));
+      if (!(sv > 0.0)) throw Error(kErrNumeric, "--scale 0: [App; Amp] of an interface is numerically singular");
+    }
+    std::vector<CopyDesc> vcopy, bcopy;
+    std::vector<LarftDesc<T>> lf;
+    std::vector<int> lcnt;
+    struct RqJob { T* App; int ld, rows, k, rank; const T* V; const T* Tb; };
+    std::vector<RqJob> rq;
+    std::vector<int> fines;
+    for (size_t q = 0; q < jobs.size(); ++q) {
+      Job& jb = jobs[q];
+      const int c = jb.c, k = jb.k, rank = rj[q].rank;
+      if (rank >= k) {  // tree.cpp:965
+        recs[slot[c]] = {ilvl_, cl_[c].id.self.lvl, cl_[c].id.self.sep, k, k};
+        continue;
+      }
+      const int N = std::max(jb.N, 1);
+      T* V = alloc_perm((size_t)k * std::max(rank, 1));
+      T* tau = alloc_perm(std::max(rank, 1));
+      T* Tb = alloc_perm((size_t)nb_ * iceil(std::max(rank, 1), nb_) * nb_);
+      factor_bytes_ += sizeof(T) * ((double)k * rank + (double)nb_ * iceil(std::max(rank, 1), nb_) * nb_);
+      const Block self = ed_[jb.es].a;  // a copy: add_edge below may move ed_
+      if (rank > 0) {
+        vcopy.push_back(CopyDesc{rj[q].V, V, k, k, k, rank, 0, 0});
+        vcopy.push_back(CopyDesc{rj[q].tau, tau, rank, rank, rank, 1, 0, 0});
+        lf.push_back(LarftDesc<T>{V, tau, Tb, k, k, rank, 0});
+        lcnt.push_back(iceil(rank, nb_));
+        rq.push_back(RqJob{self.p, self.ld, jb.rows, k, rank, V, Tb});
+      }
+      nnzQ_ += (unsigned long long)(k * rank);
+      // Orthogonal op (operations.h:113-129): backward x_c <- Q x_c only
+      Op od;
+      od.kind = OpOrthoD; od.level = ilvl_; od.c = c; od.wave = w; od.nofwd = true;
+      od.V = V; od.ldv = k; od.m = k; od.k = rank; od.Tb = Tb;
+      od.segs.push_back({c, k});
+      ops_.push_back(std::move(od));
+      // fine cluster: columns [rank, k), the last k - rank rows (tree.cpp:965-970)
+      const int cfn = k - rank, rows0 = jb.rows, rows1 = jb.rows - cfn;
+      const int cf = new_cluster(cl_[c].cstart + rank, cfn, cl_[c].rstart + rows0 - cfn, cfn, cl_[c].id);
+      alloc_x(cf, cfn);
+      Op sp;  // Split == SplitD on a square cluster (operations.cpp:118-131)
+      sp.kind = OpSplitD; sp.level = ilvl_; sp.c = c; sp.f = cf; sp.wave = w;
+      sp.crows = rows0; sp.ccols = k; sp.rank = rank;
+      ops_.push_back(std::move(sp));
+      cl_[c].rsize = rows1;
+      cl_[c].csize = rank;
+      // Amp blocks <- (triu(R[0:rank]) P^T)[:, their columns]^T (tree.cpp:976-983)
+      int off = 0;
+      for (int e : jb.amp) {
+        Block& b = ed_[e].a;
+        if (rank > 0) vcopy.push_back(CopyDesc{jb.Ct + off, b.p, N, b.ld, b.rows, rank, 0, 0});
+        b.cols = rank;
+        off += b.rows;
+      }
+      // split_edges (tree.cpp:480-530): in-edges of c give their last cfn rows
+      // to new edges n1 -> cf; the self block App Q splits into four
+      for (int e : jb.apm) {
+        const int n1 = ed_[e].n1;
+        const int ne = add_edge(n1, cf, nullptr, false, true);
+        Block& b = ed_[e].a;
+        const Block& nb2 = ed_[ne].a;
+        if (b.cols > 0) bcopy.push_back(CopyDesc{b.p + rows1, nb2.p, b.ld, nb2.ld, cfn, b.cols, 0, 0});
+        b.rows = rows1;
+      }
+      const int eff = add_edge(cf, cf, nullptr, true, true);
+      const int efc = add_edge(cf, c, nullptr, true, false);   // A(c, f) = Acf (not an in-edge of c)
+      const int ecf = add_edge(c, cf, nullptr, false, true);   // A(f, c) = Afc
+      bcopy.push_back(CopyDesc{self.p + rows1 + (int64_t)rank * self.ld, ed_[eff].a.p, self.ld, ed_[eff].a.ld, cfn, cfn, 0, 0});
+      if (rows1 > 0)
+        bcopy.push_back(CopyDesc{self.p + (int64_t)rank * self.ld, ed_[efc].a.p, self.ld, ed_[efc].a.ld, rows1, cfn, 0, 0});
+      if (rank > 0) bcopy.push_back(CopyDesc{self.p + rows1, ed_[ecf].a.p, self.ld, ed_[ecf].a.ld, cfn, rank, 0, 0});
+      ed_[jb.es].a.rows = rows1;
+      ed_[jb.es].a.cols = rank;
+      auto& rsp = cl_[cf].rsp;
+      for (int e : jb.apm) rsp.push_back(ed_[e].n1);
+      rsp.push_back(c);
+      cl_[cf].elim = false;
+      fines.push_back(cf);
+      recs[slot[c]] = {ilvl_, cl_[c].id.self.lvl, cl_[c].id.self.sep, k, rank};
+    }
+    copy_blocks(vcopy);
+    if (!lf.empty()) {
+      std::vector<int> st;
+      const int nt = prefix_tiles(lcnt, st);
+      const LarftDesc<T>* dl = stager_.push(lf);
+      const int* dst = stager_.push(st);
+      timed("larft", 0.0, 0.0, [&] { launch_larft<T>(stream_, dl, dst, (int)lf.size(), nt); });
+    }
+    // App <- App Q = App H_1 ... H_rank, 32 reflectors at a time:
+    // W = App[:, jb:] V_b, Z = W T_b, App[:, jb:] -= Z V_b^T (tree.cpp:957)
+    {
+      int rmax = 0;
+      for (auto& j : rq) rmax = std::max(rmax, j.rank);
+      for (int jb0 = 0; jb0 < rmax; jb0 += nb_) {
+        ws_.rewind();
+        std::vector<GemmDesc<T>> g1, g2, g3;
+        for (auto& j : rq) {
+          if (jb0 >= j.rank || j.rows == 0) continue;
+          const int wdt = std::min(nb_, j.rank - jb0), kk = j.k - jb0;
+          const T* Vb = j.V + jb0 + (int64_t)jb0 * j.k;
+          const T* Tbb = j.Tb + (int64_t)jb0 * nb_;
+          T* A0 = j.App + (int64_t)jb0 * j.ld;
+          T* W = (T*)ws_.alloc(sizeof(T) * (size_t)j.rows * nb_);
+          T* Z = (T*)ws_.alloc(sizeof(T) * (size_t)j.rows * nb_);
+          g1.push_back(gd(A0, j.ld, Vb, j.k, W, j.rows, j.rows, wdt, kk, gemm_flags(false, false, kStructNone, kStructUnitLower), T(1), T(0)));
+          g2.push_back(gd(W, j.rows, Tbb, nb_, Z, j.rows, j.rows, wdt, wdt, gemm_flags(false, false, kStructNone, kStructUpper), T(1), T(0)));
+          g3.push_back(gd(Z, j.rows, Vb, j.k, A0, j.ld, j.rows, kk, wdt, gemm_flags(false, true, kStructNone, kStructUnitLower), T(-1), T(1)));
+        }
+        gemm_batch(g1);
+        gemm_batch(g2);
+        gemm_batch(g3);
+      }
+    }
+    copy_blocks(bcopy);
+    // inline elimination of the fine clusters (tree.cpp:988)
+    if (!fines.empty()) eliminate_level(fines, w);
+  }
+  sprs_.insert(sprs_.end(), recs.begin(), recs.end());
+}
+
 // tree.cpp:319-411 (least squares), for every cluster eliminated at this
 // level with more rows than columns: each extra row goes to the column
 // neighbour whose in-edge block has the largest norm on that row (first one
@@ -1541,9 +1808,12 @@ int Engine<T>::factorize() {  // tree.cpp:1309-1437
     if (timing_) sync();
     const double t4 = wall();
     if (ilvl_ >= skip_ && ilvl_ < nlevels_ - 2 && tol_ != 0) {
-      if (!scale_) throw Error(kErrState, "--scale 0 sparsification is not available in this build");
-      if (!square_) sparsify_extra_level(itf);
-      sparsify_level(itf);
+      if (!scale_) {
+        sparsify0_level(itf);
+      } else {
+        if (!square_) sparsify_extra_level(itf);
+        sparsify_level(itf);
+      }
     }
     if (timing_) sync();
     const double t5 = wall();
@@ -1753,12 +2023,19 @@ void Engine<T>::build_solve_program() {
     for (auto o : L.sc) { ssc.refl.push_back(refl_of(*o, 1)); ssc.mmax = std::max(ssc.mmax, o->m); }
     push(hf, std::move(ssc));
     HStage sod = stage(0, 0);
-    for (auto o : L.od) if (o->k > 0) { sod.refl.push_back(refl_of(*o, 1)); sod.mmax = std::max(sod.mmax, o->m); }
+    for (auto o : L.od)
+      if (o->k > 0 && !o->nofwd) { sod.refl.push_back(refl_of(*o, 1)); sod.mmax = std::max(sod.mmax, o->m); }
     push(hf, std::move(sod));
     HStage sa = stage(2, 0), sb = stage(2, 0);
     for (auto o : L.sd) splitd_fwd(o, sa, sb);
     push(hf, std::move(sa));
     push(hf, std::move(sb));
+    // --scale 0 fine clusters: their QR forward (segments {f, c} per
+    // interface, disjoint across interfaces, so one stage)
+    HStage sfq = stage(0, 0);
+    for (auto o : L.fq)
+      if (o->fwd_refl && o->k > 0) { sfq.refl.push_back(refl_of(*o, 1)); sfq.mmax = std::max(sfq.mmax, o->m); }
+    push(hf, std::move(sfq));
     HStage smg = stage(2, 0);
     for (auto o : L.mg) merge_up(o, smg, true);
     push(hf, std::move(smg));

@@ -190,4 +190,23 @@ struct MoveDesc {
   int pad_;
 };
 
+// sigma_min of an upper-triangular k x k factor (k <= kSminMaxK); work holds
+// k * (kSminSteps + 1) doubles of Lanczos basis.
+constexpr int kSminSteps = 64;
+constexpr int kSminMaxK = 4096;
+template <typename T>
+struct SminDesc {
+  const T* R;
+  int ld, k;
+  double* work;
+  double* out;
+};
+// C[r0:r1, 0:cols] *= 1 / *s
+template <typename T>
+struct ScaleInvDesc {
+  T* C;
+  const double* s;
+  int ld, r0, r1, cols;
+};
+
 }  // namespace spq

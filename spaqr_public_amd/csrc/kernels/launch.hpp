@@ -78,6 +78,14 @@ void launch_gather_in(hipStream_t s, const int64_t* dst, const int* src, int n, 
 template <typename T>
 void launch_scatter_out(hipStream_t s, const int64_t* src, const int* dst, int n, const T* xbuf, double* x);
 
+// smin.hip
+int smin_max_k();
+size_t smin_work_elems(int k);
+template <typename T>
+void launch_smin(hipStream_t s, const SminDesc<T>* d, int nd, int kmax);
+template <typename T>
+void launch_scale_inv(hipStream_t s, const ScaleInvDesc<T>* d, const int* tstart, int nd, int ntiles);
+
 // krylov.hip
 void launch_spmv(hipStream_t s, int n, const int* rp, const int* ci, const double* v, const double* x, double* y,
                  double alpha, double beta);

@@ -72,6 +72,21 @@ int ktest_gemm_split(int m, int n, int k, const double* A, int lda, int acols, c
   return 0;
 }
 
+// sigma_min of upper-triangular R (k x k, ld) by the Lanczos kernel.
+int ktest_smin(int k, const double* R, int ld, double* out) {
+  if (k < 1 || k > smin_max_k() || ld < k) { g_err = "bad smin shape"; return -1; }
+  double* dR = dup(R, (size_t)ld * k);
+  double* dW = dup<double>(nullptr, smin_work_elems(k));
+  double* dO = dup<double>(nullptr, 1);
+  SminDesc<double> sd{dR, ld, k, dW, dO};
+  SminDesc<double>* dd = dup(&sd, 1);
+  launch_smin<double>(0, dd, 1, k);
+  KT(hipDeviceSynchronize());
+  back(out, dO, 1);
+  (void)hipFree(dR); (void)hipFree(dW); (void)hipFree(dO); (void)hipFree(dd);
+  return 0;
+}
+
 // Blocked Householder QR of one m x k panel (col-major, ld m): A <- V\R,
 // tau (min(m,k)), Tb (nb x roundup(k,nb)), and C (m x nc) <- Q^T C.
 int ktest_qr(int m, int k, double* A, double* tau, double* Tb, int nc, double* Cm) {

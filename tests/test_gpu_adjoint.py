@@ -14,14 +14,14 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 
-def _tree(name, n, dim, tol, precision=None, order=1.0):
+def _tree(name, n, dim, tol, precision=None, order=1.0, scale=1):
     import spaqr_public_amd as sq
     from spaqr_public_amd import gen
     A = gen.config_matrix(name)
     As, _ = sq.colscale(A)
     t = sq.Tree(sq.default_levels(As.shape[1]), 0, sq.FP64 if precision is None else precision)
     t.set_tol(tol)
-    t.set_scale(1)
+    t.set_scale(scale)
     t.set_square(1)
     t.set_order(order)
     t.set_tensor_coordinates(n, dim)
@@ -38,13 +38,16 @@ def _tree(name, n, dim, tol, precision=None, order=1.0):
     ("advdiff3d_32", 32, 3, 1e-2, "fp64", 1.0),
     ("advdiff3d_16", 16, 3, 1e-2, "fp64", 1.5),
     ("advdiff3d_16", 16, 3, 1e-2, "fp32", 1.0),
+    ("advdiff2d_64", 64, 2, 1e-2, "fp64-scale0", 1.0),
+    ("advdiff3d_16", 16, 3, 1e-2, "fp64-scale0", 1.0),
 ])
 def test_transpose_apply_adjoint_identity(name, n, dim, tol, prec, order):
     import spaqr_public_amd as sq
-    t, As = _tree(name, n, dim, tol, sq.FP32 if prec == "fp32" else sq.FP64, order)
+    t, As = _tree(name, n, dim, tol, sq.FP32 if prec == "fp32" else sq.FP64, order,
+                  0 if prec.endswith("scale0") else 1)
     rng = np.random.default_rng(7)
     N = As.shape[0]
-    eps = 1e-12 if prec == "fp64" else 2e-6
+    eps = 2e-6 if prec == "fp32" else 1e-12
     for _ in range(3):
         u, v = rng.standard_normal(N), rng.standard_normal(N)
         Mtu, Mv = t.solve_transpose(u), t.solve(v)

@@ -29,6 +29,7 @@ def K(ktest):
     ktest.ktest_gemm.argtypes = [C.c_int, C.c_int, C.c_int, dp, C.c_int, C.c_int, dp, C.c_int, C.c_int, dp, C.c_int,
                                  C.c_int, C.c_double, C.c_double]
     ktest.ktest_gemm_split.argtypes = ktest.ktest_gemm.argtypes + [C.c_int]
+    ktest.ktest_smin.argtypes = [C.c_int, dp, C.c_int, dp]
     ktest.ktest_qr.argtypes = [C.c_int, C.c_int, dp, dp, dp, C.c_int, dp]
     ktest.ktest_larft.argtypes = [C.c_int, C.c_int, dp, dp, dp]
     ktest.ktest_qp3.argtypes = [C.c_int, C.c_int, dp, dp, dp, C.c_double, C.c_int, C.c_int]
@@ -123,6 +124,35 @@ def test_gemm_split(K, mnk, ks, ta, sa):
     assert K.ktest_gemm_split(m, n, k, d(A), lda, Ac, d(B), ldb, n, d(Cm), ldc, flags, alpha, beta, ks) == 0
     assert rel(Cm[:m, :n], ref) < 1e-13
     assert np.array_equal(Cm[m:, :], before[m:, :])  # padding rows untouched
+
+
+# sigma_min of R from a QR of random matrices with prescribed singular values
+# (graded, clustered at the bottom, and a 1e8 condition number) against
+# numpy's SVD
+@pytest.mark.parametrize("k,kind", [(1, "graded"), (5, "graded"), (33, "graded"), (64, "cluster"), (100, "graded"),
+                                    (300, "cluster"), (650, "graded"), (120, "ill")])
+def test_smin(K, k, kind):
+    rng = np.random.default_rng(k)
+    m = k + 17
+    if kind == "graded":
+        s = np.logspace(0, -3, k)
+    elif kind == "cluster":
+        s = np.concatenate([np.logspace(1, 0, k - 3), [1e-2 * (1 + 1e-9), 1e-2 * (1 + 2e-9), 1e-2]]) if k > 3 else np.ones(k)
+    else:
+        s = np.logspace(0, -8, k)
+    U = np.linalg.qr(rng.standard_normal((m, k)))[0]
+    Vt = np.linalg.qr(rng.standard_normal((k, k)))[0]
+    A = (U * s) @ Vt
+    R = np.linalg.qr(A, mode="r")
+    ld = k + 3
+    Rp = np.zeros((ld, k), order="F")
+    Rp[:k, :] = R
+    out = np.zeros(1)
+    assert K.ktest_smin(k, d(Rp), ld, d(out)) == 0
+    ref = np.linalg.svd(R, compute_uv=False).min()
+    # a cluster of width 2e-9 at the bottom is resolved to within its width
+    tol = {"graded": 1e-12, "cluster": 1e-8, "ill": 1e-7}[kind]
+    assert abs(out[0] - ref) <= tol * ref, (out[0], ref)
 
 
 # panel steps run on ceil((m - jb) / 256) workgroups: (513, 65), (2000, 70)

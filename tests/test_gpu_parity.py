@@ -184,3 +184,56 @@ def test_least_squares_mixed(oracle, gpu_lib, n, tol):
     assert gpu["oneshot_res"] <= 2.0 * ref["oneshot_res"] + 1e-6
     assert gpu["final_res"] <= 1e-10
     assert gpu["iters"] <= 2 * ref["iters"] + 4
+
+
+@pytest.mark.parametrize("name,n,dim,tol", [
+    ("advdiff2d_64", 64, 2, 1e-2),
+    ("advdiff2d_128", 128, 2, 1e-3),
+    ("advdiff3d_16", 16, 3, 1e-2),
+    ("advdiff3d_32", 32, 3, 1e-2),
+])
+def test_scale_0(oracle, gpu_lib, name, n, dim, tol):
+    """--scale 0 (sparsify, tree.cpp:887-989): sigma_min normalisation,
+    App Q, split_edges and the inline fine-cluster elimination on the GPU.
+
+    This branch is rounding-sensitive in the reference itself: the oracle run
+    with its own BLAS and with OpenBLAS already disagrees on 7 of 463
+    (advdiff2d_128) and 4 of 117 (advdiff3d_16) ranks, where scale 1 agrees
+    exactly.  So the tree is compared structurally (permutations, cluster
+    ids, offsets, creation order; sizes follow the ranks) and the ranks to
+    90 %, next to the north-star bar on residual and iterations."""
+    from spaqr_public_amd import gen
+    A = gen.config_matrix(name)
+    gpu, ref = _run_both(oracle, A, n, dim, tol, scale=0)
+    cg, rg = gpu["tree"].perms()
+    co, ro = ref["tree"].perms()
+    assert np.array_equal(cg, co) and np.array_equal(rg, ro)
+    tg, to = gpu["tree"].cluster_table(), ref["tree"].cluster_table()
+    cols = [0, 1, 2, 3, 4, 5, 6, 7, 9, 11]
+    assert np.array_equal(tg[:, cols], to[:, cols])
+    print(f"{name}: one-shot {gpu['oneshot_res']:.3e} (oracle {ref['oneshot_res']:.3e}), its {gpu['iters']} "
+          f"(oracle {ref['iters']}), final {gpu['final_res']:.2e}")
+    assert gpu["oneshot_res"] <= 2.0 * ref["oneshot_res"]
+    assert abs(gpu["iters"] - ref["iters"]) <= 1
+    assert gpu["final_res"] <= 1e-12
+    sg, so = gpu["tree"].sparsified(), ref["tree"].sparsified()
+    assert sg.shape == so.shape
+    assert np.mean(sg[:, 4] == so[:, 4]) >= 0.90
+    if name == "advdiff2d_64":
+        assert np.array_equal(sg, so)
+
+
+def test_scale_0_consistent():
+    """GPU --scale 0 at tol 1e-6: M^{-1} A - I is of the order of tol while
+    interfaces are compressed (the oracle test of the same name pins the
+    restatement)."""
+    import spaqr_public_amd as sq
+    from spaqr_public_amd import gen
+    A = gen.config_matrix("advdiff2d_32")
+    r = sq.run_flow(A, 32, 2, 1e-6, scale=0)
+    sp = r["tree"].sparsified()
+    assert (sp[:, 4] < sp[:, 3]).sum() >= 3
+    N = A.shape[0]
+    E = np.eye(N)
+    Minv = np.stack([r["tree"].solve(E[:, j]) for j in range(N)], axis=1)
+    assert np.linalg.norm(Minv @ r["A"].toarray() - E) <= 1e-4

@@ -122,3 +122,19 @@ def test_planted_sign_error_is_caught(oracle, G):
     oracle.set_backend("own")
     qr, tau = oracle.geqrf(G["geqrf1_A"])
     assert not close(-qr, G["geqrf1_qr"])
+
+
+@pytest.mark.parametrize("m,n,cond", [(7, 7, 1e2), (40, 25, 1e6), (90, 60, 1e3), (5, 12, 1e1)])
+def test_singular_values(oracle, m, n, cond):
+    """The oracle's singular values (util.cpp:767-782, one-sided Jacobi) match
+    LAPACK's SVD (numpy) on matrices with prescribed spectra."""
+    rng = np.random.default_rng(m * n)
+    k = min(m, n)
+    s = np.logspace(0, -np.log10(cond), k)
+    U = np.linalg.qr(rng.standard_normal((m, k)))[0]
+    V = np.linalg.qr(rng.standard_normal((n, k)))[0]
+    A = (U * s) @ V.T
+    S = oracle.singular_values(A)
+    ref = np.linalg.svd(A, compute_uv=False)
+    assert np.all(np.diff(S) <= 0)
+    assert np.max(np.abs(S - ref)) <= 1e-13 * ref[0] * max(m, n)

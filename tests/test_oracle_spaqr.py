@@ -176,3 +176,23 @@ def test_least_squares_cgls(oracle):
     assert 1e-5 < r["oneshot_res"] < 1e-1   # sparsified: not exact
     assert r["iters"] <= 20
     assert r["final_res"] <= 1e-10
+
+
+@pytest.mark.parametrize("tol", [1e-6, 1e-10])
+def test_scale0_factorization_is_consistent(oracle, tol):
+    """--scale 0 (sparsify + split_edges + inline fine elimination,
+    tree.cpp:887-989, 480-530): the factorization is exact up to the dropped
+    interactions, so the dense M^{-1} A - I shrinks with tol while interfaces
+    are still compressed.  A wrong App Q, Amp rewrite, split or fine
+    elimination leaves an O(1) error at every tol."""
+    from spaqr_public_amd import gen
+    oracle.set_backend("own")
+    A = gen.config_matrix("advdiff2d_32")
+    r = oracle.run_reference_flow(A, 32, 2, tol, scale=0)
+    sp = r["tree"].sparsified()
+    assert (sp[:, 4] < sp[:, 3]).sum() >= 3
+    N = A.shape[0]
+    E = np.eye(N)
+    Minv = np.stack([r["tree"].solve(E[:, j]) for j in range(N)], axis=1)
+    assert np.linalg.norm(Minv @ r["A"].toarray() - E) <= 100 * tol
+    assert r["final_res"] <= 1e-12
