@@ -4,6 +4,9 @@
 
 namespace spq {
 
+long Arena::n_chunks = 0;
+double Arena::t_chunks = 0;
+
 void* Arena::alloc(size_t bytes, size_t align) {
   if (bytes == 0) bytes = align;
   while (cur_ < chunks_.size()) {
@@ -18,7 +21,10 @@ void* Arena::alloc(size_t bytes, size_t align) {
   }
   size_t sz = bytes > chunk_ ? (bytes + align) : chunk_;
   void* p = nullptr;
+  const double t0 = wall();
   hipError_t e = hipMalloc(&p, sz);
+  t_chunks += wall() - t0;
+  ++n_chunks;
   if (e != hipSuccess)
     throw Error(kErrOom, "device allocation of " + std::to_string(sz) + " bytes failed: " + hipGetErrorString(e));
   chunks_.push_back({(char*)p, sz, 0});
@@ -46,7 +52,29 @@ void Arena::release() {
 
 Stager::~Stager() {
   if (host_) hipHostFree(host_);
+  if (up_host_) hipHostFree(up_host_);
   if (dev_) hipFree(dev_);
+}
+
+void Stager::upload_raw(void* dst, const void* src, size_t bytes) {
+  constexpr size_t kUpCap = size_t(32) << 20;
+  if (!up_host_) {
+    SPQ_HIP(hipHostMalloc((void**)&up_host_, kUpCap, hipHostMallocDefault));
+    up_cap_ = kUpCap;
+    up_off_ = 0;
+  }
+  if (bytes > up_cap_) {  // larger than the buffer: a synchronous copy
+    SPQ_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream_));
+    SPQ_HIP(hipStreamSynchronize(stream_));
+    return;
+  }
+  if (up_off_ + bytes > up_cap_) {  // every earlier upload must have landed
+    SPQ_HIP(hipStreamSynchronize(stream_));
+    up_off_ = 0;
+  }
+  std::memcpy(up_host_ + up_off_, src, bytes);
+  SPQ_HIP(hipMemcpyAsync(dst, up_host_ + up_off_, bytes, hipMemcpyHostToDevice, stream_));
+  up_off_ += (bytes + 255) / 256 * 256;
 }
 
 void Stager::init(hipStream_t s, size_t bytes) {

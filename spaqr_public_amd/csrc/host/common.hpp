@@ -79,6 +79,9 @@ class Arena {
   void rewind();  // keep chunks, reset offsets
   void release();
   size_t used() const { return used_; }
+  // chunk allocations of all arenas: count and host seconds (diagnostics)
+  static long n_chunks;
+  static double t_chunks;
   size_t reserved() const { return reserved_; }
 
  private:
@@ -105,22 +108,28 @@ class Stager {
   // Copies everything pushed since the last flush with one transfer; every
   // launch that reads pushed tables flushes first.
   void flush();
-  void reset() { off_ = 0; pend_ = 0; }
+  // only after the stream has been synchronised
+  void reset() { off_ = 0; pend_ = 0; up_off_ = 0; }
   // Persistent copy (for solve programs): allocated from `perm` and kept.
+  // The table goes through a pinned buffer with an asynchronous copy; the
+  // buffer is reused after the next synchronisation (or a forced one when
+  // it is full).
   template <typename X>
   X* upload(Arena& perm, const std::vector<X>& v) {
     if (v.empty()) return nullptr;
     X* d = (X*)perm.alloc(sizeof(X) * v.size());
-    SPQ_HIP(hipMemcpyAsync(d, v.data(), sizeof(X) * v.size(), hipMemcpyHostToDevice, stream_));
-    SPQ_HIP(hipStreamSynchronize(stream_));
+    upload_raw(d, v.data(), sizeof(X) * v.size());
     return d;
   }
+  void upload_raw(void* dst, const void* src, size_t bytes);
 
  private:
   hipStream_t stream_ = nullptr;
   char* host_ = nullptr;
   char* dev_ = nullptr;
   size_t cap_ = 0, off_ = 0, pend_ = 0;  // [pend_, off_) not yet copied
+  char* up_host_ = nullptr;               // pinned source of persistent uploads
+  size_t up_cap_ = 0, up_off_ = 0;
 };
 
 // Prefix sums of per-descriptor tile counts (tstart table of tiled kernels).

@@ -39,6 +39,10 @@ class Engine final : public EngineBase {
     SPQ_HIP(hipDeviceGetAttribute(&num_cus_, hipDeviceAttributeMultiprocessorCount, device));
     if (const char* e = std::getenv("SPAQR_SOLVE_STAGES")) stage_prof_ = std::atoi(e) != 0;
     if (const char* e = std::getenv("SPAQR_SOLVE_GRAPH")) solve_graph_ = std::atoi(e) != 0;
+    if (const char* e = std::getenv("SPAQR_QP3_RC")) {
+      const int r = std::atoi(e);
+      if (r >= 16 && r <= 48 && r % 4 == 0 && r != 36 && r != 44) qp3_rc_ = r;
+    }
   }
   ~Engine() override {
     (void)hipStreamSynchronize(stream_);
@@ -247,19 +251,40 @@ class Engine final : public EngineBase {
     const Clu& s = cl_[c];
     return s.level() > ilvl_ && s.id.l.lvl <= ilvl_ && s.id.r.lvl <= ilvl_;
   }
-  void sync() { SPQ_HIP(hipStreamSynchronize(stream_)); stager_.reset(); }
+  void sync() {
+    const double a = wall();
+    SPQ_HIP(hipStreamSynchronize(stream_));
+    t_wait_ += wall() - a;
+    stager_.reset();
+  }
+  double t_wait_ = 0;  // host time blocked in stream synchronisations (diagnostics)
+  double t_flush_ = 0, t_launch_ = 0;  // host time in descriptor uploads / launch calls (verbose > 1)
+  long n_launch_ = 0;
 
   // Records a HIP event pair around one launch of a kernel family when kernel
   // timing is on (events are recorded on the engine's stream, where the
   // kernels run).
   template <typename F>
   void timed(const char* name, double flops, double bytes, F&& launch) {
+    const double ta = verbose_ > 1 ? wall() : 0.0;
     stager_.flush();
-    if (!kprof_) { launch(); return; }
+    if (verbose_ > 1) {
+      const double tb = wall();
+      t_flush_ += tb - ta;
+      ++n_launch_;
+    }
+    if (!kprof_) {
+      const double tb = verbose_ > 1 ? wall() : 0.0;
+      launch();
+      if (verbose_ > 1) t_launch_ += wall() - tb;
+      return;
+    }
+    const double tb = verbose_ > 1 ? wall() : 0.0;
     hipEvent_t a = pooled_event(), b = pooled_event();
     SPQ_HIP(hipEventRecord(a, stream_));
     launch();
     SPQ_HIP(hipEventRecord(b, stream_));
+    if (verbose_ > 1) t_launch_ += wall() - tb;
     auto& k = kev_[name];
     k.ev.push_back({a, b});
     k.flops += flops;
@@ -572,12 +597,13 @@ class Engine final : public EngineBase {
     if (panel_cap_ < 0) panel_cap_ = panel_r_capacity<T>();
     return panel_cap_;
   }
-  int qp3r_cap_[2] = {-1, -1};
+  int qp3r_cap_[2][9] = {{-1, -1, -1, -1, -1, -1, -1, -1, -1}, {-1, -1, -1, -1, -1, -1, -1, -1, -1}};
+  int qp3_rc_ = 0;  // SPAQR_QP3_RC: force the register form's rows per lane
   // resident workgroups of the register-form QR with nt threads per
   // workgroup (largest LDS footprint, G = 64)
-  int qp3r_cap(int nt) {
-    int& c = qp3r_cap_[nt == 1024 ? 1 : 0];
-    if (c < 0) c = qp3r_capacity<T>(64, nt);
+  int qp3r_cap(int nt, int rc) {  // rc in {16, 20, ..., 48} (40 and 48: 256 threads only)
+    int& c = qp3r_cap_[nt == 1024 ? 1 : 0][(rc - 16) / 4];
+    if (c < 0) c = qp3r_capacity<T>(64, nt, rc);
     return c;
   }
 };
@@ -944,18 +970,38 @@ void Engine<T>::rrqr_batch(std::vector<RrqrJob>& jobs) {
   std::vector<Qp3Desc<T>> qd;
   std::vector<int> sync_off, nts;
   int sync_words = 0;
+  // rows per lane of the register form, one for the whole batch (one launch
+  // per thread count): the rc whose lanes per column, over the resident
+  // 256-thread workgroups at its register budget, cover the batch with the
+  // fewest rounds (ties keep the earlier candidate)
+  int rc = 32;
+  if (qp3_rc_ > 0) {
+    rc = qp3_rc_;
+  } else {
+    double best = -1;
+    for (int r : {32, 28, 24, 20, 16, 40, 48}) {
+      const int cap = qp3r_cap(256, r);
+      if (cap <= 0) continue;
+      double cost = 0;
+      for (auto& j : jobs) {
+        const int g = qp3r_lanes(j.k, r);
+        cost += g ? (double)g * std::max(j.N, 1) / cap : 1e30;
+      }
+      if (best < 0 || cost < best) { best = cost; rc = r; }
+    }
+  }
   for (size_t q = 0; q < jobs.size(); ++q) {
     RrqrJob& jb = jobs[q];
     const int N = std::max(jb.N, 1);
     int G = 0, nwg = 1, nt = 256;
-    if (qp3_reg_ && jb.k <= 64 * qp3r_rows()) {
-      G = 1;
-      while (G * qp3r_rows() < jb.k) G <<= 1;
+    if (qp3_reg_ && (G = qp3r_lanes(jb.k, rc)) > 0) {
       // very large problems: 1024-thread workgroups (one per CU, fewer to
       // meet); below ~256 workgroups the 256-thread form is faster
-      nt = iceil(N, 256 / G) > 256 ? 1024 : 256;
+      nt = iceil(N, 256 / G) > 256 && rc <= 32 ? 1024 : 256;
       nwg = iceil(N, nt / G);
-      if (nwg > qp3r_cap(nt)) G = 0;
+      if (nwg > qp3r_cap(nt, rc)) G = 0;
+    } else {
+      G = 0;
     }
     if (G == 0) nwg = lds_ok ? std::max(1, std::min(qp3_max_wg_, jb.N / 256)) : 1;
     jb.nwg = nwg;
@@ -973,7 +1019,7 @@ void Engine<T>::rrqr_batch(std::vector<RrqrJob>& jobs) {
     nts.push_back(G ? nt : 0);
     d.err = qp3_err_;
     d.rank_out = ranks_dev + q;
-    d.ldc = N; d.k = jb.k; d.N = jb.N; d.ldv = jb.k; d.tol = tol_; d.nwg = nwg; d.G = G;
+    d.ldc = N; d.k = jb.k; d.N = jb.N; d.ldv = jb.k; d.tol = tol_; d.nwg = nwg; d.G = G; d.rc = G ? rc : 0;
     d.part_col = G ? alloc_scr((size_t)2 * nwg * jb.k) : nullptr;
     qd.push_back(d);
     const int kk = std::min(jb.k, jb.N);
@@ -987,15 +1033,23 @@ void Engine<T>::rrqr_batch(std::vector<RrqrJob>& jobs) {
   unsigned int* syncs = (unsigned int*)scratch_.alloc(sizeof(unsigned int) * std::max(sync_words, 1));
   for (size_t q = 0; q < qd.size(); ++q) qd[q].sync = syncs + sync_off[q];
   launch_zero(stream_, 4, syncs, sync_words);
-  struct Group { int nt, coop, gmax; std::vector<Qp3Desc<T>> d; int nmax = 0; };
+  struct Group { int nt, rc, coop, gmax; std::vector<Qp3Desc<T>> d; int nmax = 0; };
   std::vector<Group> groups;
-  for (int nt : {0, 256, 1024}) {  // 0: streaming form
-    const int cap = nt ? qp3r_cap(nt) : qp3_capacity<T>(kmax);
-    Group single{nt, 0, 1, {}};
-    Group cur{nt, 1, 1, {}};
+  std::vector<std::pair<int, int>> forms = {{0, 0}};  // (threads, rows per lane); 0: streaming form
+  for (int nt : {256, 1024})
+    for (int r : {16, 20, 24, 28, 32, 40, 48})
+      if (nt == 256 || r <= 32) forms.push_back({nt, r});
+  for (auto form : forms) {
+    const int nt = form.first, rcf = form.second;
+    bool any = false;
+    for (size_t q = 0; q < qd.size() && !any; ++q) any = nts[q] == nt && qd[q].rc == rcf;
+    if (!any) continue;
+    const int cap = nt ? qp3r_cap(nt, rcf) : qp3_capacity<T>(kmax);
+    Group single{nt, rcf, 0, 1, {}};
+    Group cur{nt, rcf, 1, 1, {}};
     int used = 0;
     for (size_t q = 0; q < qd.size(); ++q) {
-      if (nts[q] != nt) continue;
+      if (nts[q] != nt || qd[q].rc != rcf) continue;
       Qp3Desc<T> d = qd[q];
       if (d.nwg == 1) {
         single.d.push_back(d);
@@ -1003,7 +1057,7 @@ void Engine<T>::rrqr_batch(std::vector<RrqrJob>& jobs) {
         continue;
       }
       if (!nt && d.nwg > cap) d.nwg = std::max(1, cap);
-      if (used + d.nwg > cap && !cur.d.empty()) { groups.push_back(cur); cur = Group{nt, 1, 1, {}}; used = 0; }
+      if (used + d.nwg > cap && !cur.d.empty()) { groups.push_back(cur); cur = Group{nt, rcf, 1, 1, {}}; used = 0; }
       cur.d.push_back(d);
       cur.gmax = std::max(cur.gmax, d.G);
       used += d.nwg;
@@ -1024,12 +1078,16 @@ void Engine<T>::rrqr_batch(std::vector<RrqrJob>& jobs) {
     int rc = 0;
     // one timing family per kernel: streaming, register 256 / 1024 threads;
     // flops are added once the ranks are known
-    const char* fam = gq.nt == 0 ? "qp3s" : (gq.nt == 1024 ? "qp3r1024" : "qp3r256");
+    static const char* const kFam[2][9] = {
+        {"qp3r256_16", "qp3r256_20", "qp3r256_24", "qp3r256_28", "qp3r256_32", "qp3r256_36", "qp3r256_40",
+         "qp3r256_44", "qp3r256_48"},
+        {"qp3r1024_16", "qp3r1024_20", "qp3r1024_24", "qp3r1024_28", "qp3r1024_32", "", "", "", ""}};
+    const char* fam = gq.nt == 0 ? "qp3s" : kFam[gq.nt == 1024 ? 1 : 0][(gq.rc - 16) / 4];
     for (auto& d : gq.d) qp3_family_[d.rank_out] = fam;
     timed(fam, 0.0, qby, [&] {
       int nmax = 0;
       for (auto& d : gq.d) nmax = std::max(nmax, d.N);
-      rc = gq.nt ? launch_qp3r<T>(stream_, dqd, dst, (int)gq.d.size(), nt, gq.gmax, nmax, gq.nt, gq.coop)
+      rc = gq.nt ? launch_qp3r<T>(stream_, dqd, dst, (int)gq.d.size(), nt, gq.gmax, nmax, gq.nt, gq.rc, gq.coop)
                  : launch_qp3<T>(stream_, dqd, dst, (int)gq.d.size(), nt, kmax, gq.coop);
     });
     if (rc != 0) throw Error(kErrHip, std::string("qp3 launch failed: ") + hipGetErrorString((hipError_t)rc));
@@ -1789,7 +1847,8 @@ int Engine<T>::factorize() {  // tree.cpp:1309-1437
   const double fstart = wall();
   for (ilvl_ = 0; ilvl_ < nlevels_; ++ilvl_) {
     LevelInfo& li = info_[ilvl_];
-    const double t0 = wall();
+    const double t0 = wall(), w0 = t_wait_, f0 = t_flush_, l0 = t_launch_, a0 = Arena::t_chunks;
+    const long n0 = n_launch_, c0 = Arena::n_chunks;
     std::vector<int> lev;
     for (int s : bottoms_[cur_])
       if (cl_[s].level() == ilvl_) lev.push_back(s);
@@ -1832,8 +1891,12 @@ int Engine<T>::factorize() {  // tree.cpp:1309-1437
         line += buf;
         prev[k.name] = k.ms;
       }
-      fprintf(stderr, "lvl %d kernel ms:%s | wall %.1f | elim %d in %d waves, sprs %d in %d waves\n", ilvl_,
-              line.c_str(), 1e3 * (t6 - t0), li.n_elim, li.elim_waves, li.n_sprs, li.sprs_waves);
+      fprintf(stderr,
+              "lvl %d kernel ms:%s | wall %.1f (host busy %.1f: uploads %.1f, launches %.1f, %ld launches, %ld chunk "
+              "allocations %.1f) | elim %d in %d waves, sprs %d in %d waves\n",
+              ilvl_, line.c_str(), 1e3 * (t6 - t0), 1e3 * ((t6 - t0) - (t_wait_ - w0)), 1e3 * (t_flush_ - f0),
+              1e3 * (t_launch_ - l0), n_launch_ - n0, Arena::n_chunks - c0, 1e3 * (Arena::t_chunks - a0), li.n_elim,
+              li.elim_waves, li.n_sprs, li.sprs_waves);
     }
     auto ts = topsize();
     li.top_rows = ts.first; li.top_cols = ts.second;
@@ -1843,7 +1906,9 @@ int Engine<T>::factorize() {  // tree.cpp:1309-1437
              ilvl_, li.t_fill, li.t_elmn, li.t_shift, li.t_scale, li.t_sprsfy, li.t_merge, ts.first, ts.second,
              (double)ts.first / (double)ts.second);
   }
+  const double tbp = wall();
   build_solve_program();
+  if (verbose_ > 1) fprintf(stderr, "solve program built in %.1f ms\n", 1e3 * (wall() - tbp));
   // a multi-workgroup kernel whose barrier wait gave up leaves the flag set
   SPQ_HIP(hipMemcpyAsync(rank_host_ + 65535, qp3_err_, sizeof(int), hipMemcpyDeviceToHost, stream_));
   sync();
@@ -1976,10 +2041,12 @@ void Engine<T>::build_solve_program() {
     }
   };
   struct LevelOps { std::vector<const Op*> qr, sc, od, sd, mg, fq, ra; };
+  std::vector<std::vector<const Op*>> by_level(nlevels_);
+  for (auto& o : ops_) by_level[o.level].push_back(&o);
   auto level_ops = [&](int lvl) {
     LevelOps L;
-    for (auto& o : ops_) {
-      if (o.level != lvl) continue;
+    for (const Op* po : by_level[lvl]) {
+      const Op& o = *po;
       switch (o.kind) {
         case OpQR: L.qr.push_back(&o); break;
         case OpScaleD: L.sc.push_back(&o); break;
@@ -1993,23 +2060,23 @@ void Engine<T>::build_solve_program() {
     return L;
   };
   // ---- x <- M^{-1} x: ops forward, then backward (tree.cpp:1450-1457)
+  std::vector<int> seg_wave(cl_.size(), -1);
   for (int lvl = 0; lvl < nlevels_; ++lvl) {
     LevelOps L = level_ops(lvl);
-    // QR forward waves: ops sharing an x segment keep their order
+    // QR forward waves: ops sharing an x segment keep their order (wave =
+    // 1 + the latest wave of an earlier op on one of its segments, tracked
+    // per segment)
     std::vector<int> wv(L.qr.size(), 0);
     int nw = 0;
     for (size_t i = 0; i < L.qr.size(); ++i) {
-      for (size_t j = 0; j < i; ++j) {
-        bool hit = false;
-        for (auto& a2 : L.qr[i]->segs) {
-          for (auto& b2 : L.qr[j]->segs)
-            if (a2.first == b2.first) { hit = true; break; }
-          if (hit) break;
-        }
-        if (hit) wv[i] = std::max(wv[i], wv[j] + 1);
-      }
-      nw = std::max(nw, wv[i] + 1);
+      int w = 0;
+      for (auto& a2 : L.qr[i]->segs) w = std::max(w, seg_wave[a2.first] + 1);
+      wv[i] = w;
+      for (auto& a2 : L.qr[i]->segs) seg_wave[a2.first] = std::max(seg_wave[a2.first], w);
+      nw = std::max(nw, w + 1);
     }
+    for (size_t i = 0; i < L.qr.size(); ++i)
+      for (auto& a2 : L.qr[i]->segs) seg_wave[a2.first] = -1;
     for (int w = 0; w < nw; ++w) {
       HStage st = stage(0, 0);
       for (size_t i = 0; i < L.qr.size(); ++i)

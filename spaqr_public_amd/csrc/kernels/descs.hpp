@@ -111,6 +111,7 @@ struct Qp3Desc {
   int ldc, k, N, ldv;
   double tol;
   int nwg, G;
+  int rc = 0;         // register form: rows per lane (the kernel's RC)
 };
 
 // Solve-side vector segment (cluster x buffer offset, length).

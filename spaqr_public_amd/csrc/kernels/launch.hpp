@@ -45,12 +45,14 @@ void launch_trsm_block(hipStream_t s, const TrsmDesc<T>* d, const int* tstart, i
 // Register form of the truncated pivoted QR: rows held per lane, resident
 // workgroups for lane-group size G and nt threads per workgroup (256 or
 // 1024), launch (gmax = largest G, nmax = largest N of the launch).
-int qp3r_rows();
+// register form: lanes per column for k rows at rc rows per lane (rc in
+// {16, 20, 24, 28, 32}; 0 if k is beyond 64 lanes), capacity and launch
+int qp3r_lanes(int k, int rc);
 template <typename T>
-int qp3r_capacity(int G, int nt);
+int qp3r_capacity(int G, int nt, int rc);
 template <typename T>
 int launch_qp3r(hipStream_t s, const Qp3Desc<T>* d, const int* wstart, int nd, int nwg_total, int gmax, int nmax,
-                int nt, int coop);
+                int nt, int rc, int coop);
 int qp3_threads();
 template <typename T>
 int qp3_capacity(int kmax);

@@ -203,8 +203,9 @@ __global__ __launch_bounds__(kQThreads) void k_qp3(const Qp3Desc<T>* __restrict_
 
 // ---------------------------------------------------------------------------
 // Register form.  Column j of C lives in the registers of an aligned group of
-// G lanes (G a power of two): lane l of the group holds the kRC contiguous
-// rows [l*kRC, (l+1)*kRC) (rows >= k zero), so C is read once and written
+// G lanes (G a power of two): lane l of the group holds the RC contiguous
+// rows [l*RC, (l+1)*RC) (rows >= k zero; RC in {16, 20, 24, 28, 32} chosen
+// per problem to pad k the least), so C is read once and written
 // once and a pivot step moves only O(k) values per workgroup.  Per step every
 // workgroup publishes its local-best column next to its (norm, position,
 // column) partial; after one inter-workgroup barrier all workgroups pick the
@@ -213,14 +214,13 @@ __global__ __launch_bounds__(kQThreads) void k_qp3(const Qp3Desc<T>* __restrict_
 // written by workgroup 0 only (entry p is read at step p, after the barrier
 // that follows its last write).  Arithmetic, pivot rule, downdate and stop
 // test are those of the streaming form.
-constexpr int kRC = 32;
-constexpr int kRP = kRC + 1;  // LDS stride per lane's rows (one pad element)
 constexpr int kPosLds = 4096; // largest N whose position table lives in LDS
+// LDS bytes of the reflector column: G lanes x (RC + 1) (one pad element per lane)
 template <typename T>
-__host__ __device__ constexpr int vsh_bytes(int G) { return (int)(((sizeof(T) * G * kRP) + 15) / 16 * 16); }
+__host__ __device__ constexpr int vsh_bytes(int G, int RC) { return (int)(((sizeof(T) * G * (RC + 1)) + 15) / 16 * 16); }
 template <typename T>
-constexpr size_t qp3r_lds(int gmax, int nmax) {
-  return (size_t)vsh_bytes<T>(gmax) + (nmax <= kPosLds ? sizeof(int) * (size_t)nmax : 0);
+constexpr size_t qp3r_lds(int gmax, int rc, int nmax) {
+  return (size_t)vsh_bytes<T>(gmax, rc) + (nmax <= kPosLds ? sizeof(int) * (size_t)nmax : 0);
 }
 
 __device__ __forceinline__ void argmax4(double& v, int& p, int& c, int& w, double ov, int op, int oc, int ow) {
@@ -233,10 +233,14 @@ __device__ __forceinline__ double group_sum(double v, int G) {
   return v;
 }
 // LDS slot of row r in the padded reflector layout
-__device__ __forceinline__ int vslot(int r) { return r + r / kRC; }
+template <int RC>
+__device__ __forceinline__ int vslot(int r) { return r + r / RC; }
 
-template <typename T, int NT>
-__global__ __launch_bounds__(NT, 1024 / NT) void k_qp3r(const Qp3Desc<T>* __restrict__ descs,
+// resident workgroups per CU the register budget is sized for: 4 waves per
+// SIMD up to 32 rows per lane, 3 beyond (256-thread workgroups only)
+constexpr int qp3r_min_blocks(int nt, int rc) { return rc <= 32 ? 1024 / nt : 768 / nt; }
+template <typename T, int NT, int RC>
+__global__ __launch_bounds__(NT, qp3r_min_blocks(NT, RC)) void k_qp3r(const Qp3Desc<T>* __restrict__ descs,
                                                      const int* __restrict__ wstart, int nd) {
   extern __shared__ unsigned char smem_raw[];
   constexpr int NW = NT / kWave;
@@ -250,28 +254,28 @@ __global__ __launch_bounds__(NT, 1024 / NT) void k_qp3r(const Qp3Desc<T>* __rest
   const Qp3Desc<T> d = descs[p];
   const int g = blockIdx.x - wstart[p], nwg = d.nwg;
   const int k = d.k, N = d.N, tid = threadIdx.x, lane = tid & (kWave - 1);
-  const int G = d.G, gl = tid & (G - 1), rbase = gl * kRC;
+  const int G = d.G, gl = tid & (G - 1), rbase = gl * RC;
   const int j = g * (NT / G) + tid / G;  // this group's column
   const bool has = j < N;
-  T* vsh = reinterpret_cast<T*>(smem_raw);  // G*kRP entries: pivot column, then reflector
-  const T* vmy = vsh + gl * kRP;           // this lane's rows
+  T* vsh = reinterpret_cast<T*>(smem_raw);  // G*(RC+1) entries: pivot column, then reflector
+  const T* vmy = vsh + gl * (RC + 1);           // this lane's rows
   // position -> column table: in LDS (every workgroup applies the same swaps)
   // when N <= kPosLds, else in memory (written by workgroup 0)
   const bool lpos = N <= kPosLds;
-  int* ptab = reinterpret_cast<int*>(smem_raw + vsh_bytes<T>(G));
+  int* ptab = reinterpret_cast<int*>(smem_raw + vsh_bytes<T>(G, RC));
   const int kmin = min(k, N);
   const double tol3z = sqrt(ldexp(1.0, -53));
 
-  T c[kRC];
+  T c[RC];
   {
     const T* src = d.Ct + j + (int64_t)rbase * d.ldc;
 #pragma unroll
-    for (int q = 0; q < kRC; ++q) c[q] = (has && rbase + q < k) ? src[(int64_t)q * d.ldc] : T(0);
+    for (int q = 0; q < RC; ++q) c[q] = (has && rbase + q < k) ? src[(int64_t)q * d.ldc] : T(0);
   }
-  for (int r = k + tid; r < G * kRC; r += NT) vsh[vslot(r)] = T(0);
+  for (int r = k + tid; r < G * RC; r += NT) vsh[vslot<RC>(r)] = T(0);
   double s = 0;
 #pragma unroll
-  for (int q = 0; q < kRC; ++q) s += (double)c[q] * (double)c[q];
+  for (int q = 0; q < RC; ++q) s += (double)c[q] * (double)c[q];
   s = group_sum(s, G);
   double vn1 = sqrt(s), vn2 = vn1, beta_own = 0;
   int pos = j;
@@ -311,8 +315,8 @@ __global__ __launch_bounds__(NT, 1024 / NT) void k_qp3r(const Qp3Desc<T>* __rest
       // one workgroup: the winner is local and its column goes straight to LDS
       if (has && j == wbest) {
 #pragma unroll
-        for (int q = 0; q < kRC; ++q)
-          if (rbase + q < k) vsh[vslot(rbase + q)] = c[q];
+        for (int q = 0; q < RC; ++q)
+          if (rbase + q < k) vsh[vslot<RC>(rbase + q)] = c[q];
       }
       if (tid == 0) {
         s_piv = wbest; s_pos = s_bp; s_gw = 0;
@@ -325,7 +329,7 @@ __global__ __launch_bounds__(NT, 1024 / NT) void k_qp3r(const Qp3Desc<T>* __rest
       if (has && j == wbest) {
         T* __restrict__ pc = d.part_col + ((int64_t)buf * nwg + g) * k + rbase;
 #pragma unroll
-        for (int q = 0; q < kRC; ++q)
+        for (int q = 0; q < RC; ++q)
           if (rbase + q < k) agent_store(pc + q, c[q]);
       }
       if (tid == 0) {
@@ -361,15 +365,15 @@ __global__ __launch_bounds__(NT, 1024 / NT) void k_qp3r(const Qp3Desc<T>* __rest
         __syncthreads();
       }
       const T* __restrict__ pcol = d.part_col + ((int64_t)buf * nwg + s_gw) * k;
-      for (int r = tid; r < k; r += NT) vsh[vslot(r)] = agent_load(pcol + r);
+      for (int r = tid; r < k; r += NT) vsh[vslot<RC>(r)] = agent_load(pcol + r);
       __syncthreads();
     }
     const int piv = s_piv, ppos = s_pos, qcol = s_q;
     // ---- Householder vector of the pivot column (identical in every workgroup)
     double ss = 0;
-    for (int r = i + 1 + tid; r < k; r += NT) { const double v = (double)vsh[vslot(r)]; ss += v * v; }
+    for (int r = i + 1 + tid; r < k; r += NT) { const double v = (double)vsh[vslot<RC>(r)]; ss += v * v; }
     ss = block_sum(ss, sred);
-    const double alpha = (double)vsh[vslot(i)];
+    const double alpha = (double)vsh[vslot<RC>(i)];
     double beta = alpha, tau = 0, scal = 0;
     if (ss > 0) {
       beta = -copysign(hypot(alpha, sqrt(ss)), alpha);
@@ -380,8 +384,8 @@ __global__ __launch_bounds__(NT, 1024 / NT) void k_qp3r(const Qp3Desc<T>* __rest
     if (i >= 1 && !(fabs(beta) / r00 >= d.tol)) { rank = i; break; }
     __syncthreads();  // alpha read everywhere before vsh is overwritten
     for (int r = tid; r < k; r += NT) {
-      const T v = (r < i) ? T(0) : (r == i ? T(1) : (T)((double)vsh[vslot(r)] * scal));
-      vsh[vslot(r)] = v;
+      const T v = (r < i) ? T(0) : (r == i ? T(1) : (T)((double)vsh[vslot<RC>(r)] * scal));
+      vsh[vslot<RC>(r)] = v;
       if (g == 0) d.V[r + (int64_t)i * d.ldv] = v;
     }
     if (tid == 0) {
@@ -401,29 +405,29 @@ __global__ __launch_bounds__(NT, 1024 / NT) void k_qp3r(const Qp3Desc<T>* __rest
     bc = -1;
     if (has && pos > i) {  // uniform over the group
       const int li = i - rbase;  // row i relative to this lane's first row
-      // dot product in chunks of 8 LDS reads (bounds the live registers)
+      // dot product in chunks of 4 LDS reads (bounds the live registers)
       double w = 0;
 #pragma unroll
-      for (int q0 = 0; q0 < kRC; q0 += 8) {
+      for (int q0 = 0; q0 < RC; q0 += 4) {
 #pragma unroll
-        for (int q = q0; q < q0 + 8; ++q) w += (double)vmy[q] * (double)c[q];
+        for (int q = q0; q < q0 + 4; ++q) w += (double)vmy[q] * (double)c[q];
         asm volatile("" ::: "memory");
       }
       w = group_sum(w, G) * tau;
       if (w != 0.0) {
 #pragma unroll
-        for (int q0 = 0; q0 < kRC; q0 += 8) {
+        for (int q0 = 0; q0 < RC; q0 += 4) {
 #pragma unroll
-          for (int q = q0; q < q0 + 8; ++q)
+          for (int q = q0; q < q0 + 4; ++q)
             if (q >= li) c[q] = (T)((double)c[q] - w * (double)vmy[q]);
           asm volatile("" ::: "memory");
         }
       }
       T mine = T(0);
 #pragma unroll
-      for (int q = 0; q < kRC; ++q)
+      for (int q = 0; q < RC; ++q)
         if (q == li) mine = c[q];
-      const double cii = (double)__shfl(mine, (lane & ~(G - 1)) | (i / kRC), kWave);
+      const double cii = (double)__shfl(mine, (lane & ~(G - 1)) | (i / RC), kWave);
       double a = vn1;
       if (a != 0.0) {
         double tt = fabs(cii) / a;
@@ -435,7 +439,7 @@ __global__ __launch_bounds__(NT, 1024 / NT) void k_qp3r(const Qp3Desc<T>* __rest
           if (i < k - 1) {
             double s2 = 0;
 #pragma unroll
-            for (int q = 0; q < kRC; ++q)
+            for (int q = 0; q < RC; ++q)
               if (q > li) s2 += (double)c[q] * (double)c[q];
             a = sqrt(group_sum(s2, G));
             vn2 = a;
@@ -457,9 +461,9 @@ __global__ __launch_bounds__(NT, 1024 / NT) void k_qp3r(const Qp3Desc<T>* __rest
     int64_t ldc = d.ldc;
     asm volatile("" : "+s"(ldc));  // recompute the addresses here instead of keeping them live
     T* dst = d.Ct + j + (int64_t)rbase * ldc;
-    const int lp = (pos < rank) ? pos - rbase : kRC;  // first row (relative) replaced by beta / zero
+    const int lp = (pos < rank) ? pos - rbase   : RC;  // first row (relative) replaced by beta / zero
 #pragma unroll
-    for (int q = 0; q < kRC; ++q) {
+    for (int q = 0; q < RC; ++q) {
       if (rbase + q < k) {
         T v = c[q];
         if (q >= lp) v = (q == lp) ? (T)beta_own : T(0);
@@ -470,38 +474,61 @@ __global__ __launch_bounds__(NT, 1024 / NT) void k_qp3r(const Qp3Desc<T>* __rest
   if (g == 0 && tid == 0) *d.rank_out = rank;
 }
 
-int qp3r_rows() { return kRC; }
+// Register-form lanes per column for k rows at rc rows per lane: the least
+// power of two G with G * rc >= k, or 0 beyond 64 lanes.
+int qp3r_lanes(int k, int rc) {
+  int g = 1;
+  while (g * rc < k) g <<= 1;
+  return g <= 64 ? g : 0;
+}
+
+namespace {
+template <typename T, int NT>
+const void* qp3r_fn(int rc) {
+  switch (rc) {
+    case 16: return (const void*)k_qp3r<T, NT, 16>;
+    case 20: return (const void*)k_qp3r<T, NT, 20>;
+    case 24: return (const void*)k_qp3r<T, NT, 24>;
+    case 28: return (const void*)k_qp3r<T, NT, 28>;
+    case 32: return (const void*)k_qp3r<T, NT, 32>;
+  }
+  return nullptr;
+}
+template <typename T>
+const void* qp3r_fn(int nt, int rc) {
+  if (nt == 256 && rc == 40) return (const void*)k_qp3r<T, 256, 40>;
+  if (nt == 256 && rc == 48) return (const void*)k_qp3r<T, 256, 48>;
+  return nt == 1024 ? qp3r_fn<T, 1024>(rc) : (nt == 256 ? qp3r_fn<T, 256>(rc) : nullptr);
+}
+}  // namespace
 
 template <typename T>
-int qp3r_capacity(int G, int nt) {
+int qp3r_capacity(int G, int nt, int rc) {
   int per_cu = 0, dev = 0, cus = 0;
+  const void* f = qp3r_fn<T>(nt, rc);
+  if (!f) return 0;
   if (hipGetDevice(&dev) != hipSuccess) return 0;
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
-  const void* f = nt == 1024 ? (const void*)k_qp3r<T, 1024> : (const void*)k_qp3r<T, 256>;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, nt, qp3r_lds<T>(G, kPosLds)) != hipSuccess) return 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, nt, qp3r_lds<T>(G, rc, kPosLds)) != hipSuccess) return 0;
   return per_cu * cus;
 }
 
 template <typename T>
 int launch_qp3r(hipStream_t s, const Qp3Desc<T>* d, const int* wstart, int nd, int nwg_total, int gmax, int nmax,
-                int nt, int coop) {
+                int nt, int rc, int coop) {
   if (nd <= 0) return 0;
-  if (nt != 256 && nt != 1024) return (int)hipErrorInvalidValue;
+  const void* f = qp3r_fn<T>(nt, rc);
+  if (!f) return (int)hipErrorInvalidValue;
   // the LDS table is sized for the largest N that uses it
-  const size_t shm = qp3r_lds<T>(gmax, std::min(nmax, kPosLds));
-  const void* f = nt == 1024 ? (const void*)k_qp3r<T, 1024> : (const void*)k_qp3r<T, 256>;
-  if (!coop) {
-    if (nt == 1024) hipLaunchKernelGGL((k_qp3r<T, 1024>), dim3(nwg_total), dim3(1024), shm, s, d, wstart, nd);
-    else hipLaunchKernelGGL((k_qp3r<T, 256>), dim3(nwg_total), dim3(256), shm, s, d, wstart, nd);
-    return (int)hipGetLastError();
-  }
+  const size_t shm = qp3r_lds<T>(gmax, rc, std::min(nmax, kPosLds));
   void* args[] = {(void*)&d, (void*)&wstart, (void*)&nd};
+  if (!coop) return (int)hipLaunchKernel(f, dim3(nwg_total), dim3(nt), args, shm, s);
   return (int)hipLaunchCooperativeKernel(f, dim3(nwg_total), dim3(nt), args, shm, s);
 }
-template int qp3r_capacity<double>(int, int);
-template int qp3r_capacity<float>(int, int);
-template int launch_qp3r<double>(hipStream_t, const Qp3Desc<double>*, const int*, int, int, int, int, int, int);
-template int launch_qp3r<float>(hipStream_t, const Qp3Desc<float>*, const int*, int, int, int, int, int, int);
+template int qp3r_capacity<double>(int, int, int);
+template int qp3r_capacity<float>(int, int, int);
+template int launch_qp3r<double>(hipStream_t, const Qp3Desc<double>*, const int*, int, int, int, int, int, int, int);
+template int launch_qp3r<float>(hipStream_t, const Qp3Desc<float>*, const int*, int, int, int, int, int, int, int);
 
 int qp3_threads() { return kQThreads; }
 

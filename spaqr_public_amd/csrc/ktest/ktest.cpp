@@ -158,19 +158,20 @@ int ktest_larft(int m, int r, const double* V, const double* tau, double* Tb) {
 
 // Truncated QP3 of C (k x N): Ct is N x k (ld N) in/out; V k x k, tau k.
 // form 0: streaming kernel split over nwg workgroups; form 1 / 2: register
-// kernel with 256 / 1024-thread workgroups (group size and workgroup count
-// follow from k and N).  Multi-
+// kernel with 256 / 1024-thread workgroups and 32 rows per lane; form
+// 10 * rc + 1 / 2: the same with rc rows per lane (group size and workgroup
+// count follow from k and N).  Multi-
 // workgroup problems are launched cooperatively.  Returns the rank, or -1 on
 // a launch/run error.
 int ktest_qp3(int k, int N, double* Ct, double* V, double* tau, double tol, int nwg, int form) {
   int G = 0, nt = 256;
+  const int rcv = form >= 10 ? form / 10 : 32;  // register form rows per lane
   if (form >= 1) {
-    if (k > 64 * qp3r_rows()) { g_err = "k too large for the register form"; return -1; }
-    G = 1;
-    while (G * qp3r_rows() < k) G <<= 1;
-    nt = form == 2 ? 1024 : 256;
+    G = qp3r_lanes(k, rcv);
+    if (G == 0) { g_err = "k too large for the register form"; return -1; }
+    nt = (form % 10) == 2 ? 1024 : 256;
     nwg = (N + nt / G - 1) / (nt / G);
-    if (nwg > qp3r_capacity<double>(G, nt)) { g_err = "nwg exceeds resident capacity"; return -1; }
+    if (nwg > qp3r_capacity<double>(G, nt, rcv)) { g_err = "nwg exceeds resident capacity"; return -1; }
   } else if (nwg > qp3_capacity<double>(k)) {
     g_err = "nwg exceeds resident capacity";
     return -1;
@@ -185,11 +186,11 @@ int ktest_qp3(int k, int N, double* Ct, double* V, double* tau, double tol, int 
   int* err = dup<int>(nullptr, 1);
   int* rk = dup<int>(nullptr, 1);
   double* pcol = dup<double>(nullptr, 2 * (size_t)nwg * k);
-  Qp3Desc<double> d{dC, dV, dt, vn, pos, pv, pp, beta, sync, err, rk, pcol, N, k, N, k, tol, nwg, G};
+  Qp3Desc<double> d{dC, dV, dt, vn, pos, pv, pp, beta, sync, err, rk, pcol, N, k, N, k, tol, nwg, G, rcv};
   Qp3Desc<double>* dd = dup(&d, 1);
   int zero = 0;
   int* ws = dup(&zero, 1);
-  int rc = form >= 1 ? launch_qp3r<double>(0, dd, ws, 1, nwg, G, N, nt, nwg > 1 ? 1 : 0)
+  int rc = form >= 1 ? launch_qp3r<double>(0, dd, ws, 1, nwg, G, N, nt, rcv, nwg > 1 ? 1 : 0)
                      : launch_qp3<double>(0, dd, ws, 1, nwg, k, nwg > 1 ? 1 : 0);
   if (rc != 0) { g_err = hipGetErrorString((hipError_t)rc); return -1; }
   KT(hipDeviceSynchronize());
@@ -210,13 +211,13 @@ int ktest_qp3(int k, int N, double* Ct, double* V, double* tau, double tol, int 
 // fresh copy of C each, the average kernel time in *ms.  Returns the rank.
 int ktest_qp3_bench(int k, int N, const double* Ct, double tol, int form, int nwg, int reps, double* ms) {
   int G = 0, nt = 256;
+  const int rcv = form >= 10 ? form / 10 : 32;  // register form rows per lane
   if (form >= 1) {
-    if (k > 64 * qp3r_rows()) { g_err = "k too large for the register form"; return -1; }
-    G = 1;
-    while (G * qp3r_rows() < k) G <<= 1;
-    nt = form == 2 ? 1024 : 256;
+    G = qp3r_lanes(k, rcv);
+    if (G == 0) { g_err = "k too large for the register form"; return -1; }
+    nt = (form % 10) == 2 ? 1024 : 256;
     nwg = (N + nt / G - 1) / (nt / G);
-    if (nwg > qp3r_capacity<double>(G, nt)) { g_err = "nwg exceeds resident capacity"; return -1; }
+    if (nwg > qp3r_capacity<double>(G, nt, rcv)) { g_err = "nwg exceeds resident capacity"; return -1; }
   } else if (nwg > qp3_capacity<double>(k)) {
     g_err = "nwg exceeds resident capacity";
     return -1;
@@ -232,7 +233,7 @@ int ktest_qp3_bench(int k, int N, const double* Ct, double tol, int form, int nw
   int* err = dup<int>(nullptr, 1);
   int* rk = dup<int>(nullptr, 1);
   double* pcol = dup<double>(nullptr, 2 * (size_t)nwg * k);
-  Qp3Desc<double> d{dC, dV, dt, vn, pos, pv, pp, beta, sync, err, rk, pcol, N, k, N, k, tol, nwg, G};
+  Qp3Desc<double> d{dC, dV, dt, vn, pos, pv, pp, beta, sync, err, rk, pcol, N, k, N, k, tol, nwg, G, rcv};
   Qp3Desc<double>* dd = dup(&d, 1);
   int zero = 0;
   int* ws = dup(&zero, 1);
@@ -245,7 +246,7 @@ int ktest_qp3_bench(int k, int N, const double* Ct, double tol, int form, int nw
     KT(hipMemcpy(dC, d0, sizeof(double) * (size_t)N * k, hipMemcpyDeviceToDevice));
     KT(hipMemset(sync, 0, sizeof(unsigned int) * barrier_words(nwg)));
     KT(hipEventRecord(a, 0));
-    rc = form >= 1 ? launch_qp3r<double>(0, dd, ws, 1, nwg, G, N, nt, nwg > 1 ? 1 : 0)
+    rc = form >= 1 ? launch_qp3r<double>(0, dd, ws, 1, nwg, G, N, nt, rcv, nwg > 1 ? 1 : 0)
                    : launch_qp3<double>(0, dd, ws, 1, nwg, k, nwg > 1 ? 1 : 0);
     KT(hipEventRecord(b, 0));
     KT(hipEventSynchronize(b));

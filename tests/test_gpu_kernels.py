@@ -208,11 +208,15 @@ def test_larft(K, oracle, mr):
 
 
 # form 0 = streaming kernel over nwg workgroups; form 1 / 2 = register kernel
-# with 256 / 1024-thread workgroups (lane group size and workgroup count follow
-# from k and N: with 256 threads (120, 2000) runs on 32 workgroups with 4 lanes
-# per column and (300, 1500) on 94 with 16, both through the two-level
-# barrier; with 1024 threads on 8 and 24)
-@pytest.mark.parametrize("form,nwg", [(0, 1), (0, 3), (1, 0), (2, 0)])
+# with 256 / 1024-thread workgroups and 32 rows per lane (lane group size and
+# workgroup count follow from k and N: with 256 threads (120, 2000) runs on 32
+# workgroups with 4 lanes per column and (300, 1500) on 94 with 16, both
+# through the two-level barrier; with 1024 threads on 8 and 24); form
+# 10 * rc + 1 / 2 = the same with rc rows per lane (16 to 28: other lane
+# groups and padding, e.g. (300, 1500) at rc = 20 on 16 lanes, 320 rows; 40
+# and 48, 256 threads only: fewer lanes, e.g. (300, 1500) on 8)
+@pytest.mark.parametrize("form,nwg", [(0, 1), (0, 3), (1, 0), (2, 0), (161, 0), (201, 0), (241, 0), (282, 0), (401, 0),
+                                      (481, 0)])
 @pytest.mark.parametrize("kN,true_rank,tol", [((20, 100), 6, 1e-3), ((64, 300), 20, 1e-6), ((50, 40), 50, 1e-10),
                                               ((30, 30), 10, 1e-2), ((1, 5), 1, 1e-3), ((120, 2000), 90, 1e-8),
                                               ((300, 1500), 150, 1e-7)])
