@@ -25,11 +25,22 @@ PEAK = {"fp64_mfma_tflops": 78.6, "fp32_mfma_tflops": 157.3, "hbm_gbs": 8000.0}
 
 # kernel-timing family -> kernel symbol (as rocprofv3 names it), {T} = double/float
 FAMILY_KERNEL = {
-    "qp3r1024": "spq::k_qp3r<{T}, 1024>", "qp3r256": "spq::k_qp3r<{T}, 256>", "qp3s": "spq::k_qp3<{T}>",
+    "qp3s": "spq::k_qp3<{T}>",
     "panel": "spq::k_panel_r<{T}>", "gemm": "spq::k_gemm<{T}>", "larft": "spq::k_larft<{T}>",
     "trsm": "spq::k_trsm_block<{T}>", "copy": "spq::k_copy", "init": "spq::k_init_blocks",
     "apply": "solve program (hipGraph of k_refl/k_bwd/k_move)",
 }
+
+
+def family_kernel(fam):
+    """Kernel symbol (as rocprofv3 prints it) of a timing family; the register
+    RRQR families are qp3r<threads>_<rows per lane>."""
+    if fam.startswith("qp3r") and "_" in fam:
+        nt, rc = fam[4:].split("_")
+        return "spq::k_qp3r<{T}, " + nt + ", " + rc + ">"
+    if fam.startswith("gemm"):
+        return FAMILY_KERNEL["gemm"]
+    return FAMILY_KERNEL.get(fam, fam)
 
 
 def traffic_for(kernel):
@@ -279,7 +290,7 @@ def main():
     roofline = None
     if dom:
         k, v = dom
-        kernel = FAMILY_KERNEL.get(k, k).format(T=tname)
+        kernel = family_kernel(k).format(T=tname)
         avg_ms = v["ms"] / max(v["launches"], 1)
         if v["flops"] > 0:
             ach = v["flops"] / (v["ms"] * 1e-3) / 1e12

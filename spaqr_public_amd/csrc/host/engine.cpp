@@ -257,6 +257,7 @@ class Engine final : public EngineBase {
     t_wait_ += wall() - a;
     stager_.reset();
   }
+  std::vector<GemmDesc<T>> qr_g1_, qr_g2_, qr_g3_;  // blocked_qr's per-step GEMM lists (reused)
   double t_wait_ = 0;  // host time blocked in stream synchronisations (diagnostics)
   double t_flush_ = 0, t_launch_ = 0;  // host time in descriptor uploads / launch calls (verbose > 1)
   long n_launch_ = 0;
@@ -326,10 +327,12 @@ class Engine final : public EngineBase {
   // Batched GEMM.  When the batch has few output tiles, descriptors with a
   // long K are split along K (partials in ws_, then an ordered reduce) so the
   // launch fills the device.
-  void gemm_batch(std::vector<GemmDesc<T>> ds) {
+  // fam: timing family (every family runs k_gemm, plus k_gemm_reduce)
+  void gemm_batch(std::vector<GemmDesc<T>>& ds, const char* fam = "gemm") {  // ds may get split-K fields
     if (ds.empty()) return;
+    auto tiles_of = [&](const GemmDesc<T>& d) { return gemm_tiles(d.m, d.n); };
     int tiles = 0;
-    for (auto& d : ds) tiles += gemm_tiles(d.m, d.n);
+    for (auto& d : ds) tiles += tiles_of(d);
     const int want = 4 * num_cus_;
     bool split = false;
     if (tiles < want) {
@@ -352,7 +355,7 @@ class Engine final : public EngineBase {
       }
     }
     std::vector<int> cnt(ds.size()), st;
-    for (size_t i = 0; i < ds.size(); ++i) cnt[i] = gemm_tiles(ds[i].m, ds[i].n) * ds[i].ksplit;
+    for (size_t i = 0; i < ds.size(); ++i) cnt[i] = tiles_of(ds[i]) * ds[i].ksplit;
     const int nt = prefix_tiles(cnt, st);
     double fl = 0, by = 0;
     for (auto& d : ds) {
@@ -369,11 +372,12 @@ class Engine final : public EngineBase {
     const int rnt = prefix_tiles(rcnt, rst);
     const GemmDesc<T>* drd = rd.empty() ? nullptr : stager_.push(rd);
     const int* drst = rd.empty() ? nullptr : stager_.push(rst);
-    timed("gemm", fl, by, [&] {
+    timed(fam, fl, by, [&] {
       launch_gemm<T>(stream_, dd, dst, (int)ds.size(), nt);
       if (!rd.empty()) launch_gemm_reduce<T>(stream_, drd, drst, (int)rd.size(), rnt);
     });
   }
+
   static GemmDesc<T> gd(const T* A, int lda, const T* B, int ldb, T* C, int ldc, int m, int n, int k, int flags,
                         T alpha, T beta) {
     GemmDesc<T> d;
@@ -459,15 +463,19 @@ class Engine final : public EngineBase {
         if (rc != 0) throw Error(kErrHip, std::string("panel launch failed: ") + hipGetErrorString((hipError_t)rc));
       }
       // trailing matrices (Y, Z temporaries live in ws_, rewound per step)
-      std::vector<GemmDesc<T>> g1, g2, g3;
+      std::vector<GemmDesc<T>>& g1 = qr_g1_;
+      std::vector<GemmDesc<T>>& g2 = qr_g2_;
+      std::vector<GemmDesc<T>>& g3 = qr_g3_;
+      g1.clear(); g2.clear(); g3.clear();
+      struct Tgt { T* C; int ld, cols; };
+      std::vector<Tgt> tg;
       for (int q : live) {
         auto& j = jobs[q];
         const int w = std::min(nb_, std::min(j.m, j.k) - jb);
         const int rows = j.m - jb;
         const T* Vb = j.P + jb + (int64_t)jb * j.ld;
         const T* Tb = j.Tb + (int64_t)jb * nb_;
-        struct Tgt { T* C; int ld, cols; };
-        std::vector<Tgt> tg;
+        tg.clear();
         const int c0 = jb + std::min(nb_, j.k - jb);
         if (j.k > c0) tg.push_back({j.P + jb + (int64_t)c0 * j.ld, j.ld, j.k - c0});
         for (auto& b : j.extra)
@@ -480,9 +488,9 @@ class Engine final : public EngineBase {
           g3.push_back(gd(Vb, j.ld, Z, nb_, t.C, t.ld, rows, t.cols, w, gemm_flags(false, false, kStructUnitLower), T(-1), T(1)));
         }
       }
-      gemm_batch(g1);
-      gemm_batch(g2);
-      gemm_batch(g3);
+      gemm_batch(g1, "gemm_y");
+      gemm_batch(g2, "gemm_z");
+      gemm_batch(g3, "gemm_c");
     }
   }
 
@@ -730,15 +738,16 @@ void Engine<T>::get_sparsity(int c) {  // tree.cpp:834-870 (+ check_if_valid 168
   std::sort(colsp.begin(), colsp.end());
   auto has = [&colsp](const SepID& s) { return std::binary_search(colsp.begin(), colsp.end(), s); };
   const SepID cs = cl_[c].id.self;
+  // integer division by 2^d (the reference's (int)pow(2, d), exact here)
   auto valid = [&](const CID& nid) {
     const SepID ns = nid.self;
     if (cs.lvl > ns.lvl) return false;
     const int d = ns.lvl - cs.lvl;
-    if (cs.sep / (int)std::pow(2, d) != ns.sep) return false;
+    if (cs.sep / (1 << d) != ns.sep) return false;
     auto side_ok = [&](const SepID& s) {
       const int dd = s.lvl - cs.lvl;
-      if (s.lvl >= cs.lvl) return cs.sep / (int)std::pow(2, dd) == s.sep;
-      return s.sep / (int)std::pow(2, -dd) == cs.sep;
+      if (s.lvl >= cs.lvl) return cs.sep / (1 << dd) == s.sep;
+      return s.sep / (1 << -dd) == cs.sep;
     };
     if (!side_ok(nid.l) && !side_ok(nid.r)) return false;
     if (!has(ns)) return false;
@@ -940,7 +949,7 @@ void Engine<T>::scale_level(const std::vector<int>& cs) {
         cnt.push_back(iceil(b.rows, trsm_rows_per_tile()));
       }
     }
-    gemm_batch(g);
+    gemm_batch(g, "gemm_trsm");
     if (!td.empty()) {
       std::vector<int> st;
       const int nt = prefix_tiles(cnt, st);

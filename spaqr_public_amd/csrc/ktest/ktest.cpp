@@ -65,7 +65,7 @@ int ktest_gemm_split(int m, int n, int k, const double* A, int lda, int acols, c
   int zero = 0;
   int* ds = dup(&zero, 1);
   launch_gemm<double>(0, dg, ds, 1, gemm_tiles(m, n) * ksplit);
-  launch_gemm_reduce<double>(0, dg, ds, 1, (m * n + 255) / 256);
+  if (ksplit > 1) launch_gemm_reduce<double>(0, dg, ds, 1, (m * n + 255) / 256);
   KT(hipDeviceSynchronize());
   back(Cm, dC, (size_t)ldc * n);
   (void)hipFree(dA); (void)hipFree(dB); (void)hipFree(dC); (void)hipFree(dW); (void)hipFree(dg); (void)hipFree(ds);
