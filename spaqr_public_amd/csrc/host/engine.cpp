@@ -51,6 +51,7 @@ class Engine final : public EngineBase {
     if (graph_exec_nrml_) hipGraphExecDestroy(graph_exec_nrml_);
     if (graph_exec_t_) hipGraphExecDestroy(graph_exec_t_);
     if (xtbuf_) hipFree(xtbuf_);
+    if (x64_) hipFree(x64_);
     if (rank_host_) hipHostFree(rank_host_);
     if (xbuf_) hipFree(xbuf_);
     if (bin_) hipFree(bin_);
@@ -132,10 +133,12 @@ class Engine final : public EngineBase {
   }
 
  private:
-  struct Block {
-    T* p = nullptr;
+  template <typename U>
+  struct BlockT {
+    U* p = nullptr;
     int rows = 0, cols = 0, ld = 0;
   };
+  using Block = BlockT<T>;
   struct Edge {
     int n1, n2;
     Block a;
@@ -179,6 +182,8 @@ class Engine final : public EngineBase {
     int crows = 0, ccols = 0, rank = 0;
     int wave = 0;  // sparsify wave (OrthoD, SplitD, FineQR)
     bool nofwd = false;     // OrthoD without a forward step (Orthogonal, operations.h:113-129)
+    const double* V64 = nullptr;   // FP64 factors of the top separator of an FP32 tree (V and R
+    const double* Tb64 = nullptr;  // share the panel, as V / R do)
     bool fwd_refl = false;  // FineQR with reflectors (the --scale 0 fine elimination)
     // Reassign: x_f[nstart + t] = x_c[idx[t]]
     std::vector<int> idx;
@@ -257,7 +262,6 @@ class Engine final : public EngineBase {
     t_wait_ += wall() - a;
     stager_.reset();
   }
-  std::vector<GemmDesc<T>> qr_g1_, qr_g2_, qr_g3_;  // blocked_qr's per-step GEMM lists (reused)
   double t_wait_ = 0;  // host time blocked in stream synchronisations (diagnostics)
   double t_flush_ = 0, t_launch_ = 0;  // host time in descriptor uploads / launch calls (verbose > 1)
   long n_launch_ = 0;
@@ -293,16 +297,18 @@ class Engine final : public EngineBase {
   }
 
   // --------------------------------------------------- batched launch helpers
-  void copy_blocks(const std::vector<CopyDesc>& ds) {
+  // esrc / edst: element bytes of source and destination (a conversion when
+  // they differ)
+  void copy_blocks(const std::vector<CopyDesc>& ds, int esrc = (int)sizeof(T), int edst = (int)sizeof(T)) {
     if (ds.empty()) return;
     std::vector<int> cnt(ds.size()), st;
     for (size_t i = 0; i < ds.size(); ++i) cnt[i] = iceil(std::max(ds[i].rows, 1), 32) * iceil(std::max(ds[i].cols, 1), 32);
     const int nt = prefix_tiles(cnt, st);
     double bytes = 0;
-    for (auto& d : ds) bytes += 2.0 * sizeof(T) * d.rows * d.cols;
+    for (auto& d : ds) bytes += (double)(esrc + edst) * d.rows * d.cols;
     const CopyDesc* dd = stager_.push(ds);
     const int* dst = stager_.push(st);
-    timed("copy", 0.0, bytes, [&] { launch_copy(stream_, sizeof(T), sizeof(T), dd, dst, (int)ds.size(), nt); });
+    timed("copy", 0.0, bytes, [&] { launch_copy(stream_, esrc, edst, dd, dst, (int)ds.size(), nt); });
   }
   void init_blocks(const std::vector<Block>& bs, int identity) {
     if (bs.empty()) return;
@@ -328,9 +334,10 @@ class Engine final : public EngineBase {
   // long K are split along K (partials in ws_, then an ordered reduce) so the
   // launch fills the device.
   // fam: timing family (every family runs k_gemm, plus k_gemm_reduce)
-  void gemm_batch(std::vector<GemmDesc<T>>& ds, const char* fam = "gemm") {  // ds may get split-K fields
+  template <typename U>
+  void gemm_batch(std::vector<GemmDesc<U>>& ds, const char* fam = "gemm") {  // ds may get split-K fields
     if (ds.empty()) return;
-    auto tiles_of = [&](const GemmDesc<T>& d) { return gemm_tiles(d.m, d.n); };
+    auto tiles_of = [&](const GemmDesc<U>& d) { return gemm_tiles(d.m, d.n); };
     int tiles = 0;
     for (auto& d : ds) tiles += tiles_of(d);
     const int want = 4 * num_cus_;
@@ -344,11 +351,11 @@ class Engine final : public EngineBase {
         if (d.m <= 0 || d.n <= 0) continue;
         const size_t mn = (size_t)d.m * d.n;
         int ks = std::min(std::max(1, want / std::max(tiles, 1)), std::max(1, d.k / 256));
-        ks = (int)std::min<size_t>(ks, room / (sizeof(T) * mn + 256));
+        ks = (int)std::min<size_t>(ks, room / (sizeof(U) * mn + 256));
         if (ks > 1) {
-          const size_t bytes = (sizeof(T) * (size_t)ks * mn + 255) / 256 * 256;
+          const size_t bytes = (sizeof(U) * (size_t)ks * mn + 255) / 256 * 256;
           d.ksplit = ks;
-          d.work = (T*)gws_.alloc(bytes);
+          d.work = (U*)gws_.alloc(bytes);
           room -= bytes;
           split = true;
         }
@@ -360,27 +367,28 @@ class Engine final : public EngineBase {
     double fl = 0, by = 0;
     for (auto& d : ds) {
       fl += 2.0 * d.m * d.n * d.k;
-      by += sizeof(T) * ((double)d.m * d.k + (double)d.k * d.n + (d.beta == T(0) ? 1.0 : 2.0) * d.m * d.n);
+      by += sizeof(U) * ((double)d.m * d.k + (double)d.k * d.n + (d.beta == U(0) ? 1.0 : 2.0) * d.m * d.n);
     }
-    const GemmDesc<T>* dd = stager_.push(ds);
+    const GemmDesc<U>* dd = stager_.push(ds);
     const int* dst = stager_.push(st);
-    std::vector<GemmDesc<T>> rd;
+    std::vector<GemmDesc<U>> rd;
     std::vector<int> rcnt, rst;
     if (split)
       for (auto& d : ds)
         if (d.ksplit > 1) { rd.push_back(d); rcnt.push_back(iceil(d.m * d.n, 256)); }
     const int rnt = prefix_tiles(rcnt, rst);
-    const GemmDesc<T>* drd = rd.empty() ? nullptr : stager_.push(rd);
+    const GemmDesc<U>* drd = rd.empty() ? nullptr : stager_.push(rd);
     const int* drst = rd.empty() ? nullptr : stager_.push(rst);
     timed(fam, fl, by, [&] {
-      launch_gemm<T>(stream_, dd, dst, (int)ds.size(), nt);
-      if (!rd.empty()) launch_gemm_reduce<T>(stream_, drd, drst, (int)rd.size(), rnt);
+      launch_gemm<U>(stream_, dd, dst, (int)ds.size(), nt);
+      if (!rd.empty()) launch_gemm_reduce<U>(stream_, drd, drst, (int)rd.size(), rnt);
     });
   }
 
-  static GemmDesc<T> gd(const T* A, int lda, const T* B, int ldb, T* C, int ldc, int m, int n, int k, int flags,
-                        T alpha, T beta) {
-    GemmDesc<T> d;
+  template <typename U>
+  static GemmDesc<U> gd(const U* A, int lda, const U* B, int ldb, U* C, int ldc, int m, int n, int k, int flags,
+                        U alpha, U beta) {
+    GemmDesc<U> d;
     d.A = A; d.B = B; d.C = C; d.lda = lda; d.ldb = ldb; d.ldc = ldc; d.m = m; d.n = n; d.k = k;
     d.flags = flags; d.alpha = alpha; d.beta = beta;
     return d;
@@ -391,12 +399,17 @@ class Engine final : public EngineBase {
   // panel's trailing columns and to every extra matrix `extra[p]` (same row
   // space as the panel, ld given) with three batched GEMMs:
   //   Y = V_b^T C ; Z = T_b^T Y ; C -= V_b Z.
-  struct QrJob {
-    T* P; int ld, m, k;
-    T* tau; T* Tb;
-    std::vector<Block> extra;  // matrices receiving H^T (rows = m)
+  template <typename U>
+  struct QrJobT {
+    U* P; int ld, m, k;
+    U* tau; U* Tb;
+    std::vector<BlockT<U>> extra;  // matrices receiving H^T (rows = m)
   };
-  void blocked_qr(std::vector<QrJob>& jobs) {
+  using QrJob = QrJobT<T>;
+  // U is the element type of the batch (T, or double for the FP64 top
+  // separator of an FP32 tree)
+  template <typename U>
+  void blocked_qr(std::vector<QrJobT<U>>& jobs) {
     int kmax = 0;
     for (auto& j : jobs) kmax = std::max(kmax, std::min(j.m, j.k));
     // panel steps run in the register form: rows [jb, m) over nwg
@@ -410,21 +423,21 @@ class Engine final : public EngineBase {
     unsigned int* pool = words ? (unsigned int*)scratch_.alloc(sizeof(unsigned int) * words) : nullptr;
     if (words) launch_zero(stream_, 4, pool, words);
     size_t pool_used = 0;
-    const int cap = panel_r_cap();
+    const int cap = std::is_same<U, T>::value ? panel_r_cap() : panel_r_capacity<U>();
     for (int jb = 0; jb < kmax; jb += nb_) {
       ws_.rewind();
-      std::vector<PanelDesc<T>> single, multi;
+      std::vector<PanelDesc<U>> single, multi;
       std::vector<int> live;
       double pfl = 0, pby = 0;
       for (int q = 0; q < (int)jobs.size(); ++q) {
         auto& j = jobs[q];
         if (jb >= std::min(j.m, j.k)) continue;
-        PanelDesc<T> d{};
+        PanelDesc<U> d{};
         d.P = j.P; d.tau = j.tau; d.Tb = j.Tb; d.ld = j.ld; d.m = j.m; d.k = j.k;
         d.nwg = iceil(j.m - jb, rpw);
         d.err = qp3_err_;
         if (d.nwg > 1) {
-          d.part = (T*)ws_.alloc(sizeof(T) * 2 * (size_t)d.nwg * 2 * nb_);
+          d.part = (U*)ws_.alloc(sizeof(U) * 2 * (size_t)d.nwg * 2 * nb_);
           d.sync = pool + pool_used;
           pool_used += barrier_words(d.nwg);
           multi.push_back(d);
@@ -434,14 +447,14 @@ class Engine final : public EngineBase {
         live.push_back(q);
         const int w = std::min(nb_, std::min(j.m, j.k) - jb), r = j.m - jb, c = std::min(nb_, j.k - jb);
         pfl += 4.0 * r * w * c + 2.0 * r * w * w;  // column steps: dots + updates, then T
-        pby += 2.0 * sizeof(T) * (double)r * c;
+        pby += 2.0 * sizeof(U) * (double)r * c;
       }
       if (live.empty()) break;
       // single-workgroup panels in one plain launch; the others cooperatively,
       // packed within the resident capacity
-      std::vector<std::pair<std::vector<PanelDesc<T>>, int>> launches;
+      std::vector<std::pair<std::vector<PanelDesc<U>>, int>> launches;
       if (!single.empty()) launches.push_back({single, 0});
-      std::vector<PanelDesc<T>> cur;
+      std::vector<PanelDesc<U>> cur;
       int used = 0;
       for (auto& d : multi) {
         if (d.nwg > cap) throw Error(kErrState, "panel: problem exceeds the resident workgroup capacity");
@@ -455,37 +468,34 @@ class Engine final : public EngineBase {
         std::vector<int> cnt, st;
         for (auto& d : L.first) cnt.push_back(d.nwg);
         const int nt = prefix_tiles(cnt, st);
-        const PanelDesc<T>* dpd = stager_.push(L.first);
+        const PanelDesc<U>* dpd = stager_.push(L.first);
         const int* dst = stager_.push(st);
         int rc = 0;
         timed("panel", li == 0 ? pfl : 0.0, li == 0 ? pby : 0.0,
-              [&] { rc = launch_panel_r<T>(stream_, dpd, dst, (int)L.first.size(), nt, jb, L.second); });
+              [&] { rc = launch_panel_r<U>(stream_, dpd, dst, (int)L.first.size(), nt, jb, L.second); });
         if (rc != 0) throw Error(kErrHip, std::string("panel launch failed: ") + hipGetErrorString((hipError_t)rc));
       }
       // trailing matrices (Y, Z temporaries live in ws_, rewound per step)
-      std::vector<GemmDesc<T>>& g1 = qr_g1_;
-      std::vector<GemmDesc<T>>& g2 = qr_g2_;
-      std::vector<GemmDesc<T>>& g3 = qr_g3_;
-      g1.clear(); g2.clear(); g3.clear();
-      struct Tgt { T* C; int ld, cols; };
+      std::vector<GemmDesc<U>> g1, g2, g3;
+      struct Tgt { U* C; int ld, cols; };
       std::vector<Tgt> tg;
       for (int q : live) {
         auto& j = jobs[q];
         const int w = std::min(nb_, std::min(j.m, j.k) - jb);
         const int rows = j.m - jb;
-        const T* Vb = j.P + jb + (int64_t)jb * j.ld;
-        const T* Tb = j.Tb + (int64_t)jb * nb_;
+        const U* Vb = j.P + jb + (int64_t)jb * j.ld;
+        const U* Tb = j.Tb + (int64_t)jb * nb_;
         tg.clear();
         const int c0 = jb + std::min(nb_, j.k - jb);
         if (j.k > c0) tg.push_back({j.P + jb + (int64_t)c0 * j.ld, j.ld, j.k - c0});
         for (auto& b : j.extra)
           if (b.cols > 0) tg.push_back({b.p + jb, b.ld, b.cols});
         for (auto& t : tg) {
-          T* Y = (T*)ws_.alloc(sizeof(T) * (size_t)nb_ * t.cols);
-          T* Z = (T*)ws_.alloc(sizeof(T) * (size_t)nb_ * t.cols);
-          g1.push_back(gd(Vb, j.ld, t.C, t.ld, Y, nb_, w, t.cols, rows, gemm_flags(true, false, kStructUnitLower), T(1), T(0)));
-          g2.push_back(gd(Tb, nb_, Y, nb_, Z, nb_, w, t.cols, w, gemm_flags(true, false, kStructUpper), T(1), T(0)));
-          g3.push_back(gd(Vb, j.ld, Z, nb_, t.C, t.ld, rows, t.cols, w, gemm_flags(false, false, kStructUnitLower), T(-1), T(1)));
+          U* Y = (U*)ws_.alloc(sizeof(U) * (size_t)nb_ * t.cols);
+          U* Z = (U*)ws_.alloc(sizeof(U) * (size_t)nb_ * t.cols);
+          g1.push_back(gd(Vb, j.ld, t.C, t.ld, Y, nb_, w, t.cols, rows, gemm_flags(true, false, kStructUnitLower), U(1), U(0)));
+          g2.push_back(gd(Tb, nb_, Y, nb_, Z, nb_, w, t.cols, w, gemm_flags(true, false, kStructUpper), U(1), U(0)));
+          g3.push_back(gd(Vb, j.ld, Z, nb_, t.C, t.ld, rows, t.cols, w, gemm_flags(false, false, kStructUnitLower), U(-1), U(1)));
         }
       }
       gemm_batch(g1, "gemm_y");
@@ -569,7 +579,14 @@ class Engine final : public EngineBase {
     const R12Chunk<T>* r12c = nullptr;
     const int* wstart = nullptr;
     T* work = nullptr;
+    // FP64 top separator of an FP32 tree (types 4-7)
+    const ReflDesc<double>* refl64 = nullptr;
+    const BwdDesc<double>* bwd64 = nullptr;
+    const CopyDesc* cvt = nullptr;
+    double* work64 = nullptr;
   };
+  double* x64_ = nullptr;              // FP64 workspace of the top-separator ops
+  const Seg* segs64_dev_ = nullptr;
   unsigned int* solve_sync_ = nullptr;  // barrier counters of multi-workgroup solve ops
   int n_solve_sync_ = 0;
   int* solve_err_ = nullptr;
@@ -780,7 +797,13 @@ void Engine<T>::eliminate_level(const std::vector<int>& cs, int fine_wave) {
     std::vector<std::vector<int>> te;     // target edges aligned with pe
     std::vector<int> touched;             // sorted target edge ids
     T *P, *W, *tau, *Tb;
+    double *P64 = nullptr, *W64 = nullptr, *tau64 = nullptr, *Tb64 = nullptr;  // FP64 top separator
   };
+  // An FP32 tree factors its top separator in FP64 (the last level's QR):
+  // the gathered panel is converted, factored in double, and the op keeps
+  // the double factors for the solve.
+  // (square trees; the least-squares form keeps its R-only top in FP32)
+  const bool top64 = sizeof(T) == 4 && square_ && fine_wave < 0 && ilvl_ == nlevels_ - 1;
   std::vector<Plan> plans;
   std::vector<Block> zero;
   for (int c : cs) {
@@ -837,7 +860,14 @@ void Engine<T>::eliminate_level(const std::vector<int>& cs, int fine_wave) {
     p.P = alloc_perm((size_t)p.m * p.k);
     p.tau = alloc_perm(p.k);
     p.Tb = alloc_perm((size_t)nb_ * iceil(p.k, nb_) * nb_);
-    factor_bytes_ += sizeof(T) * ((double)p.m * p.k + (double)nb_ * iceil(p.k, nb_) * nb_);
+    if (top64) {
+      p.P64 = (double*)perm_.alloc(sizeof(double) * std::max<size_t>((size_t)p.m * p.k, 1));
+      p.tau64 = (double*)perm_.alloc(sizeof(double) * std::max(p.k, 1));
+      p.Tb64 = (double*)perm_.alloc(sizeof(double) * (size_t)nb_ * iceil(p.k, nb_) * nb_);
+      factor_bytes_ += sizeof(double) * ((double)p.m * p.k + (double)nb_ * iceil(p.k, nb_) * nb_);
+    } else {
+      factor_bytes_ += sizeof(T) * ((double)p.m * p.k + (double)nb_ * iceil(p.k, nb_) * nb_);
+    }
   }
   for (int w = 0; w < nw; ++w) {
     scratch_.rewind();
@@ -867,7 +897,29 @@ void Engine<T>::eliminate_level(const std::vector<int>& cs, int fine_wave) {
       jobs.push_back(std::move(j));
     }
     copy_blocks(gather);
-    blocked_qr(jobs);
+    if (top64) {
+      std::vector<QrJobT<double>> j64;
+      std::vector<CopyDesc> up, down;
+      for (size_t i = 0; i < plans.size(); ++i) {
+        if (wave[i] != w) continue;
+        Plan& p = plans[i];
+        up.push_back(CopyDesc{p.P, p.P64, p.m, p.m, p.m, p.k, 0, 0});
+        QrJobT<double> j;
+        j.P = p.P64; j.ld = p.m; j.m = p.m; j.k = p.k; j.tau = p.tau64; j.Tb = p.Tb64;
+        if (p.Nw > 0) {
+          p.W64 = (double*)scratch_.alloc(sizeof(double) * (size_t)p.m * p.Nw);
+          up.push_back(CopyDesc{p.W, p.W64, p.m, p.m, p.m, p.Nw, 0, 0});
+          down.push_back(CopyDesc{p.W64, p.W, p.m, p.m, p.m, p.Nw, 0, 0});
+          j.extra.push_back(BlockT<double>{p.W64, p.m, p.Nw, p.m});
+        }
+        j64.push_back(std::move(j));
+      }
+      copy_blocks(up, 4, 8);
+      blocked_qr(j64);
+      copy_blocks(down, 8, 4);
+    } else {
+      blocked_qr(jobs);
+    }
     copy_blocks(scatter);
   }
   // ops and bookkeeping (tree.cpp:826-829)
@@ -878,6 +930,7 @@ void Engine<T>::eliminate_level(const std::vector<int>& cs, int fine_wave) {
     o.V = p.P; o.ldv = p.m; o.m = p.m; o.k = p.k; o.Tb = p.Tb;
     for (int e : p.pe) o.segs.push_back({ed_[e].n2, cl_[ed_[e].n2].rsize});
     o.R = p.P; o.ldr = p.m; o.kr = p.k;
+    if (top64) { o.V64 = p.P64; o.Tb64 = p.Tb64; }
     for (int e : cl_[p.c].in) o.r12.push_back({ed_[e].a, {ed_[e].n1, ed_[e].a.cols}});
     ops_.push_back(std::move(o));
     cl_[p.c].elim = true;
@@ -1952,23 +2005,69 @@ void Engine<T>::build_solve_program() {
   std::vector<Seg> segs;
   std::vector<R12Desc<T>> r12;
   // buf: the x buffer a stage works on (0 = x, 1 = xt); moves copy sbuf -> buf
+  // types 4 / 5: reflectors / triangular sweeps in FP64 on x64_ (the top
+  // separator of an FP32 tree); 6 / 7: segment conversion x -> x64_ / back
   struct HStage {
     int type;  // 0 reflectors, 1 triangular sweeps, 2 moves, 3 R12 products
     std::vector<ReflDesc<T>> refl;
     std::vector<BwdDesc<T>> bwd;
     std::vector<MoveDesc> mv;
     std::vector<R12Chunk<T>> r12c;
+    std::vector<ReflDesc<double>> refl64;
+    std::vector<BwdDesc<double>> bwd64;
+    std::vector<CopyDesc> cvt;
     int mmax = 0, buf = 0, sbuf = 0;
   };
   std::vector<HStage> hf, hb, htf, htb, hpf;
   auto push = [](std::vector<HStage>& v, HStage&& st) {
-    if (!st.refl.empty() || !st.bwd.empty() || !st.mv.empty() || !st.r12c.empty()) v.push_back(std::move(st));
+    if (!st.refl.empty() || !st.bwd.empty() || !st.mv.empty() || !st.r12c.empty() || !st.refl64.empty() ||
+        !st.bwd64.empty() || !st.cvt.empty())
+      v.push_back(std::move(st));
   };
   auto stage = [](int type, int buf, int sbuf = -1) {
     HStage st;
     st.type = type; st.buf = buf; st.sbuf = sbuf < 0 ? buf : sbuf;
     return st;
+  };  // FP64 top-separator ops: their segments are converted into x64_ (one
+  // contiguous range per op), applied in double, and converted back
+  std::vector<Seg> segs64;
+  int64_t x64_len = 0;
+  for (auto& o : ops_)
+    if (o.V64) {
+      if (o.kind != OpQR || !o.r12.empty()) throw Error(kErrState, "FP64 top separator: unexpected op structure");
+      x64_len = std::max<int64_t>(x64_len, o.m);
+    }
+  if (x64_len > 0) SPQ_HIP(hipMalloc((void**)&x64_, sizeof(double) * x64_len));
+  auto qr64_stages = [&](const Op& o, int trans, bool bwd, std::vector<HStage>& out) {
+    HStage a = stage(6, 0), b = stage(bwd ? 5 : 4, 0), c = stage(7, 0);
+    int64_t off = 0;
+    if (!bwd) {
+      for (auto& sg : o.segs) {
+        a.cvt.push_back(CopyDesc{xbuf_ + cl_[sg.first].xoff, x64_ + off, sg.second, sg.second, sg.second, 1, 0, 0});
+        c.cvt.push_back(CopyDesc{x64_ + off, xbuf_ + cl_[sg.first].xoff, sg.second, sg.second, sg.second, 1, 0, 0});
+        off += sg.second;
+      }
+      ReflDesc<double> d{};
+      d.V = o.V64; d.Tb = o.Tb64; d.ldv = o.ldv; d.m = o.m; d.k = o.k; d.nb = nb_;
+      d.seg0 = (int)segs64.size(); d.nseg = 1; d.trans = trans; d.nwg = 1;
+      segs64.push_back(Seg{0, o.m, 0});
+      b.refl64.push_back(d);
+      b.mmax = o.m;
+      solve_bytes_ += sizeof(double) * ((double)o.m * o.k + (double)nb_ * o.k) + 2.0 * sizeof(double) * o.m;
+    } else {
+      const int64_t xo = cl_[o.c].xoff;
+      a.cvt.push_back(CopyDesc{xbuf_ + xo, x64_, o.kr, o.kr, o.kr, 1, 0, 0});
+      c.cvt.push_back(CopyDesc{x64_, xbuf_ + xo, o.kr, o.kr, o.kr, 1, 0, 0});
+      BwdDesc<double> d{};
+      d.R = o.V64; d.ldr = o.ldv; d.k = o.kr; d.xoff = 0; d.blk0 = 0; d.nblk = 0; d.trans = trans; d.nwg = 1;
+      b.bwd64.push_back(d);
+      solve_bytes_ += sizeof(double) * 0.5 * o.kr * o.kr + 2.0 * sizeof(double) * o.kr;
+    }
+    push(out, std::move(a));
+    push(out, std::move(b));
+    push(out, std::move(c));
   };
+
   auto refl_of = [&](const Op& o, int trans) {
     ReflDesc<T> d{};
     d.V = o.V; d.Tb = o.Tb; d.ldv = o.ldv; d.m = o.m; d.k = o.k; d.nb = nb_;
@@ -2089,8 +2188,10 @@ void Engine<T>::build_solve_program() {
     for (int w = 0; w < nw; ++w) {
       HStage st = stage(0, 0);
       for (size_t i = 0; i < L.qr.size(); ++i)
-        if (wv[i] == w) { st.refl.push_back(refl_of(*L.qr[i], 1)); st.mmax = std::max(st.mmax, L.qr[i]->m); }
+        if (wv[i] == w && !L.qr[i]->V64) { st.refl.push_back(refl_of(*L.qr[i], 1)); st.mmax = std::max(st.mmax, L.qr[i]->m); }
       push(hf, std::move(st));
+      for (size_t i = 0; i < L.qr.size(); ++i)
+        if (wv[i] == w && L.qr[i]->V64) qr64_stages(*L.qr[i], 1, false, hf);
     }
     HStage sra = stage(2, 0);
     for (auto o : L.ra) reassign(o, sra, true);
@@ -2149,9 +2250,12 @@ void Engine<T>::build_solve_program() {
     push(hb, std::move(ssc));
     // Reassign has no backward step on x (operations.cpp:70)
     HStage sqrr = stage(3, 0), sqr = stage(1, 0);
-    for (auto o : L.qr) sqr.bwd.push_back(bwd_of(*o, 0, sqrr));
+    for (auto o : L.qr)
+      if (!o->V64) sqr.bwd.push_back(bwd_of(*o, 0, sqrr));
     push(hb, std::move(sqrr));
     push(hb, std::move(sqr));
+    for (auto o : L.qr)
+      if (o->V64) qr64_stages(*o, 0, true, hb);
   }
   if (!square_) {
     // ---- normal equations (tree.cpp:1471-1517): tops forward on xt, tops
@@ -2237,6 +2341,16 @@ void Engine<T>::build_solve_program() {
         for (auto& d : st.bwd) d.trans ^= 1;
         for (auto& d : st.r12c) d.trans ^= 1;
         for (auto& d : st.mv) std::swap(d.src, d.dst);
+        for (auto& d : st.refl64) d.trans ^= 1;
+        for (auto& d : st.bwd64) d.trans ^= 1;
+        for (auto& d : st.cvt) {
+          const void* a = d.src;
+          d.src = d.dst;
+          d.dst = const_cast<void*>(a);
+          std::swap(d.lds, d.ldd);
+        }
+        if (st.type == 6) st.type = 7;
+        else if (st.type == 7) st.type = 6;
         std::swap(st.buf, st.sbuf);
         dst.push_back(std::move(st));
       }
@@ -2245,6 +2359,7 @@ void Engine<T>::build_solve_program() {
     transpose(hf, hab);
   }
   segs_dev_ = stager_.upload(perm_, segs);
+  segs64_dev_ = stager_.upload(perm_, segs64);
   r12_dev_ = stager_.upload(perm_, r12);
   // Multi-workgroup split of large ops: row slices of <= 256 rows for
   // reflector applies of >= 384 rows, or >= 4 blocks of 32 for triangular
@@ -2317,10 +2432,21 @@ void Engine<T>::build_solve_program() {
         h.bwd = ds;
       } else if (h.type == 3) {
         s.nd = (int)h.r12c.size();
+      } else if (h.type == 4 || h.type == 5) {
+        // one workgroup per FP64 op
+        s.nd = h.type == 4 ? (int)h.refl64.size() : (int)h.bwd64.size();
+        cnt.assign(s.nd, 1);
+        s.nwg_total = prefix_tiles(cnt, st);
+        if (h.type == 4 && h.mmax > refl_lds_cap(sizeof(double)))
+          s.work64 = (double*)perm_.alloc(sizeof(double) * (size_t)s.nd * h.mmax);
+      } else if (h.type == 6 || h.type == 7) {
+        s.nd = (int)h.cvt.size();
+        for (auto& d : h.cvt) cnt.push_back(iceil(std::max(d.rows, 1), 32) * iceil(std::max(d.cols, 1), 32));
+        s.nwg_total = prefix_tiles(cnt, st);
       } else {
         s.nd = (int)h.mv.size();
       }
-      if (h.type <= 1) s.wstart = stager_.upload(perm_, st);
+      if (h.type <= 1 || h.type >= 4) s.wstart = stager_.upload(perm_, st);
       out.push_back(s);
     }
   };
@@ -2352,6 +2478,14 @@ void Engine<T>::build_solve_program() {
         st[i].bwd = stager_.upload(perm_, h.bwd);
       } else if (h.type == 3) {
         st[i].r12c = stager_.upload(perm_, h.r12c);
+      } else if (h.type == 4) {
+        for (auto& d : h.refl64) d.err = solve_err_;
+        st[i].refl64 = stager_.upload(perm_, h.refl64);
+      } else if (h.type == 5) {
+        for (auto& d : h.bwd64) d.err = solve_err_;
+        st[i].bwd64 = stager_.upload(perm_, h.bwd64);
+      } else if (h.type == 6 || h.type == 7) {
+        st[i].cvt = stager_.upload(perm_, h.cvt);
       } else {
         st[i].mv = stager_.upload(perm_, h.mv);
       }
@@ -2421,6 +2555,11 @@ void Engine<T>::run_program(const double* b, double* x, bool prof, int mode) {
     if (s.type == 0) launch_refl<T>(stream_, s.refl, s.wstart, s.nd, s.nwg_total, segs_dev_, xb, s.work, s.mmax);
     else if (s.type == 1) launch_bwd<T>(stream_, s.bwd, s.wstart, s.nd, s.nwg_total, r12_dev_, xb);
     else if (s.type == 3) launch_r12<T>(stream_, s.r12c, s.nd, xb);
+    else if (s.type == 4)
+      launch_refl<double>(stream_, s.refl64, s.wstart, s.nd, s.nwg_total, segs64_dev_, x64_, s.work64, s.mmax);
+    else if (s.type == 5) launch_bwd<double>(stream_, s.bwd64, s.wstart, s.nd, s.nwg_total, nullptr, x64_);
+    else if (s.type == 6) launch_copy(stream_, (int)sizeof(T), 8, s.cvt, s.wstart, s.nd, s.nwg_total);
+    else if (s.type == 7) launch_copy(stream_, 8, (int)sizeof(T), s.cvt, s.wstart, s.nd, s.nwg_total);
     else launch_move<T>(stream_, s.mv, s.nd, xb, s.sbuf ? xtbuf_ : xbuf_);
   };
   std::vector<const std::vector<Stage>*> progs;

@@ -13,6 +13,16 @@ import pytest
 
 pytestmark = pytest.mark.gpu
 
+# torch carries its own HIP runtime next to the one libspaqr.so links; let it
+# initialise the device at collection time, before any test drives the GPU
+# through libspaqr.so in this process.
+try:
+    import torch
+    if torch.cuda.is_available():
+        torch.cuda.init()
+except ImportError:
+    pass
+
 
 def _tree(name, n, dim, tol, precision=None, order=1.0, scale=1):
     import spaqr_public_amd as sq
