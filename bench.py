@@ -200,6 +200,87 @@ def copy_calibration():
     print(f"copy calibration: {n * n * 8} bytes read and written", flush=True)
 
 
+def adjoint_batch(args, count=64):
+    """Differentiable batched solve (2-D advection-diffusion instances, grid
+    side ~ U{64..256}, |v| ~ U(10, 1000) along (1, 1)/sqrt(2), t = 1e-3):
+    per instance the spaQR factorization, x = A^{-1} b by GMRES and the
+    gradients of ||x - x_t||^2 w.r.t. b and A's values through the adjoint
+    (GMRES on A^T with M^{-T}, then dA on the pattern).  One step = all
+    instances, one after the other; the value is its wall time.  Two
+    instances are checked against the FP64 reference gradient (sparse LU on A
+    and A^T)."""
+    import numpy as np
+    import scipy.sparse.linalg as spl
+    import torch
+    import spaqr_public_amd as sq
+    from spaqr_public_amd import autograd as sg, gen
+    precision = sq.FP64 if args.precision == "fp64" else sq.FP32
+    torch.cuda.init()  # torch's HIP runtime first, then libspaqr's
+    rng = np.random.default_rng(2021)
+    inst = []
+    for _ in range(count):
+        n = int(rng.integers(64, 257))
+        vm = float(rng.uniform(10.0, 1000.0))
+        A = gen.advection_diffusion(n, 2, 1.0, [vm / np.sqrt(2.0)] * 2).tocsc()
+        A.sort_indices()
+        inst.append((n, A, rng.standard_normal(n * n), rng.standard_normal(n * n)))
+    dev = torch.device("cuda", 0)
+
+    def one(item):
+        n, A, b, xt = item
+        solver = sg.SparseSolver(A.indptr, A.indices, A.data, n, 2, tol=1e-3, precision=precision)
+        vals = torch.tensor(A.data, dtype=torch.float64, device=dev, requires_grad=True)
+        bt = torch.tensor(b, dtype=torch.float64, device=dev, requires_grad=True)
+        x = sg.solve(vals, bt, solver)
+        loss = ((x - torch.tensor(xt, device=dev)) ** 2).sum()
+        loss.backward()
+        torch.cuda.current_stream(dev).synchronize()
+        its = solver.last_iters
+        solver.close()
+        return (vals.grad.cpu(), bt.grad.cpu()), its
+
+    times, its, check = [], [], []
+    for step in range(args.warmup + args.steps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        res = [one(item) for item in inst]
+        grads = [r[0] for r in res]
+        its.extend(r[1] for r in res)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        if step >= args.warmup:
+            times.append(dt)
+        print(f"# step {step} {'timed' if step >= args.warmup else 'warmup'}: {dt:.3f} s for {count} instances",
+              file=sys.stderr, flush=True)
+        if step == args.warmup + args.steps - 1:
+            for q in (0, count // 2):
+                n, A, b, xt = inst[q]
+                lu = spl.splu(A)
+                xr = lu.solve(b)
+                lam = lu.solve(2.0 * (xr - xt), trans="T")
+                col = np.repeat(np.arange(A.shape[1]), np.diff(A.indptr))
+                dA = -lam[A.indices] * xr[col]
+                gA, gb = grads[q][0].numpy(), grads[q][1].numpy()
+                check.append(max(np.linalg.norm(gA - dA) / np.linalg.norm(dA),
+                                 np.linalg.norm(gb - lam) / np.linalg.norm(lam)))
+    value = float(np.mean(times))
+    dofs = sum(n * n for (n, _, _, _) in inst)
+    fw = [a for a, _ in its]
+    bw = [c for _, c in its]
+    out = {
+        "metric": "differentiable batched solve: factor + forward GMRES + adjoint gradients, 64 instances",
+        "value": value, "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1000.0 * value, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if precision == sq.FP64 else "f32-factor/f64-krylov", "data": "synthetic",
+        "config": {"workload": f"adjoint_batch{count}", "instances": count, "total_dofs": dofs, "tol": 1e-3,
+                   "grid": "U{64..256}^2", "velocity": "U(10,1000) along (1,1)/sqrt(2)", "loss": "||x - x_t||^2",
+                   "x_t": "N(0,1)", "b": "N(0,1)", "seed": 2021},
+        "forward_gmres_iters_mean": float(np.mean(fw)), "adjoint_gmres_iters_mean": float(np.mean(bw)),
+        "gradient_rel_err_vs_fp64_reference": max(check) if check else None,
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -228,6 +309,10 @@ def main():
         print(f"bench.py: this build measures one MI355X; --gpus {args.gpus} is not supported "
               "(the multi-GPU sharding is a design note only)", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload.startswith("adjoint_batch"):
+        adjoint_batch(args, int(args.workload[len("adjoint_batch"):] or 64))
+        return
 
     import numpy as np
     import spaqr_public_amd as sq

@@ -13,6 +13,8 @@ s_j = 10 / ||A(:, j)||_2 (the reference driver's scaling, spaQR.cpp:131-147),
 and `solve` differentiates through As_vals = A_vals * s[col] and x = s * y with
 As y = b, s held constant.  torch is used only as this thin harness: device
 memory, streams and the autograd tape; every solve runs in libspaqr.so.
+Each solver works on its own HIP stream and waits only for torch's current
+stream before it reads torch tensors, so solvers may run in parallel threads.
 """
 import numpy as np
 import scipy.sparse as sp
@@ -59,6 +61,16 @@ class SparseSolver:
         self.M = core.Matrix(t, As)
         self.last_iters = (0, 0)
 
+    def close(self):
+        """Release the device factors and matrix now (instead of at garbage
+        collection)."""
+        if getattr(self, "M", None) is not None:
+            self.M.close()
+            self.M = None
+        if getattr(self, "tree", None) is not None:
+            self.tree.close()
+            self.tree = None
+
     def _vec(self, v):
         v = v.detach().to(self.device, torch.float64).contiguous()
         if v.shape != (self.N,):
@@ -69,7 +81,7 @@ class SparseSolver:
         """y = As^{-1} b by GMRES (FP64 vectors on the device)."""
         b = self._vec(b)
         y = torch.zeros_like(b)
-        torch.cuda.synchronize(self.device)
+        torch.cuda.current_stream(self.device).synchronize()
         it, _ = core.gmres_device(self.tree, self.M, b.data_ptr(), y.data_ptr(), self.iters, self.iters, self.res)
         self.last_iters = (it, self.last_iters[1])
         return y
@@ -78,7 +90,7 @@ class SparseSolver:
         """lambda = As^{-T} g by GMRES with M^{-T}."""
         g = self._vec(g)
         lam = torch.zeros_like(g)
-        torch.cuda.synchronize(self.device)
+        torch.cuda.current_stream(self.device).synchronize()
         it, _ = core.gmres_transpose_device(self.tree, self.M, g.data_ptr(), lam.data_ptr(), self.iters,
                                             self.iters, self.res)
         self.last_iters = (self.last_iters[0], it)
@@ -88,7 +100,7 @@ class SparseSolver:
         """alpha * u_i v_j on As's nonzeros, CSC order."""
         u, v = self._vec(u), self._vec(v)
         out = torch.empty(self.nnz, dtype=torch.float64, device=self.device)
-        torch.cuda.synchronize(self.device)
+        torch.cuda.current_stream(self.device).synchronize()
         core.csc_outer_device(self.M, alpha, u.data_ptr(), v.data_ptr(), out.data_ptr())
         return out
 

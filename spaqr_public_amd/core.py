@@ -297,10 +297,13 @@ class Matrix:
         if not self.h:
             raise SpaqrError("matrix upload failed: " + load_library().spaqr_last_error().decode())
 
-    def __del__(self):
+    def close(self):
         if getattr(self, "h", None):
             load_library().spaqr_matrix_destroy(self.h)
             self.h = None
+
+    def __del__(self):
+        self.close()
 
 
 def gmres(tree: Tree, M: Matrix, b, x0=None, iters=300, restart=None, tol=1e-12, verbose=0):
