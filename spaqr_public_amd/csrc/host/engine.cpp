@@ -477,6 +477,9 @@ class Engine final : public EngineBase {
       }
       // trailing matrices (Y, Z temporaries live in ws_, rewound per step)
       std::vector<GemmDesc<U>> g1, g2, g3;
+      size_t ntg = 0;
+      for (int q : live) ntg += 1 + jobs[q].extra.size();
+      g1.reserve(ntg); g2.reserve(ntg); g3.reserve(ntg);
       struct Tgt { U* C; int ld, cols; };
       std::vector<Tgt> tg;
       for (int q : live) {

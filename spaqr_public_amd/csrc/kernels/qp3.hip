@@ -326,11 +326,12 @@ __global__ __launch_bounds__(NT, qp3r_min_blocks(NT, RC)) void k_qp3r(const Qp3D
     } else {
       // everything handed to the other workgroups goes through agent-scope
       // atomics, so the barrier needs no cache maintenance
+      // rows [i, k) of the candidate (the rows above i are never read)
       if (has && j == wbest) {
         T* __restrict__ pc = d.part_col + ((int64_t)buf * nwg + g) * k + rbase;
 #pragma unroll
         for (int q = 0; q < RC; ++q)
-          if (rbase + q < k) agent_store(pc + q, c[q]);
+          if (rbase + q >= i && rbase + q < k) agent_store(pc + q, c[q]);
       }
       if (tid == 0) {
         agent_store(d.part_v + buf * nwg + g, s_bv);
@@ -365,7 +366,7 @@ __global__ __launch_bounds__(NT, qp3r_min_blocks(NT, RC)) void k_qp3r(const Qp3D
         __syncthreads();
       }
       const T* __restrict__ pcol = d.part_col + ((int64_t)buf * nwg + s_gw) * k;
-      for (int r = tid; r < k; r += NT) vsh[vslot<RC>(r)] = agent_load(pcol + r);
+      for (int r = i + tid; r < k; r += NT) vsh[vslot<RC>(r)] = agent_load(pcol + r);
       __syncthreads();
     }
     const int piv = s_piv, ppos = s_pos, qcol = s_q;
