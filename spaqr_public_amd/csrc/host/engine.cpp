@@ -36,6 +36,7 @@ class Engine final : public EngineBase {
     SPQ_HIP(hipMemset(qp3_err_, 0, 256));
     if (const char* e = std::getenv("SPAQR_QP3_MAX_WG")) qp3_max_wg_ = std::max(1, std::atoi(e));
     if (const char* e = std::getenv("SPAQR_QP3_REGISTER")) qp3_reg_ = std::atoi(e) != 0;
+    if (const char* e = std::getenv("SPAQR_REFL_REGISTER")) reflr_ = std::atoi(e) != 0;
     SPQ_HIP(hipDeviceGetAttribute(&num_cus_, hipDeviceAttributeMultiprocessorCount, device));
     if (const char* e = std::getenv("SPAQR_SOLVE_STAGES")) stage_prof_ = std::atoi(e) != 0;
     if (const char* e = std::getenv("SPAQR_SOLVE_GRAPH")) solve_graph_ = std::atoi(e) != 0;
@@ -575,6 +576,8 @@ class Engine final : public EngineBase {
   struct Stage {
     int type;  // 0 refl, 1 bwd, 2 move
     int nd = 0, mmax = 0, nwg_total = 0;
+    int kmax = 0, mall = 0;  // largest k and m of the stage's ops (diagnostics)
+    int form = 0, nseg_max = 0;  // reflector stages: 1 = register form (k_reflr)
     int buf = 0, sbuf = 0;  // x buffer used (0 = x, 1 = xt); moves copy sbuf -> buf
     const ReflDesc<T>* refl = nullptr;
     const BwdDesc<T>* bwd = nullptr;
@@ -620,6 +623,7 @@ class Engine final : public EngineBase {
   bool stage_prof_ = false;  // SPAQR_SOLVE_STAGES=1: time every solve stage (uncaptured)
   bool solve_graph_ = true;  // SPAQR_SOLVE_GRAPH=0: launch the solve program directly (no hipGraph)
   bool qp3_reg_ = true;      // SPAQR_QP3_REGISTER=0: streaming truncated QR only
+  bool reflr_ = true;        // SPAQR_REFL_REGISTER=0: solve reflector stages in the LDS form only
   int panel_cap_ = -1;
   int panel_r_cap() {
     if (panel_cap_ < 0) panel_cap_ = panel_r_capacity<T>();
@@ -2389,9 +2393,19 @@ void Engine<T>::build_solve_program() {
       s.sbuf = h.sbuf;
       std::vector<int> cnt, st;
       if (h.type == 0) {
+        // register form for stages of few ops (one op's latency sets the
+        // pace): every op gets ceil(m / rows) workgroups, all resident
         std::vector<int> want;
-        for (auto& d : h.refl) want.push_back(d.m >= 384 ? std::min(32, iceil(d.m, 256)) : 1);
-        plan_nwg(want);
+        int reg_total = 0;
+        for (auto& d : h.refl) reg_total += iceil(std::max(d.m, 1), reflr_rows());
+        const bool reg = reflr_ && (int)h.refl.size() < num_cus_ && reg_total <= num_cus_;
+        if (reg) {
+          for (auto& d : h.refl) want.push_back(iceil(std::max(d.m, 1), reflr_rows()));
+        } else {
+          for (auto& d : h.refl) want.push_back(d.m >= 384 ? std::min(32, iceil(d.m, 256)) : 1);
+          plan_nwg(want);
+        }
+        s.form = reg ? 1 : 0;
         std::vector<int> order(h.refl.size());
         std::iota(order.begin(), order.end(), 0);
         std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return (want[a] > 1) > (want[b] > 1); });
@@ -2399,9 +2413,12 @@ void Engine<T>::build_solve_program() {
         int mmax1 = 0;
         for (int i : order) {
           ReflDesc<T> d = h.refl[i];
+          s.kmax = std::max(s.kmax, d.k);
+          s.mall = std::max(s.mall, d.m);
           d.nwg = want[i];
+          s.nseg_max = std::max(s.nseg_max, d.nseg);
           if (d.nwg > 1) {
-            d.work = alloc_perm(d.m);
+            d.work = reg ? nullptr : alloc_perm(d.m);
             d.part = alloc_perm((size_t)2 * d.nwg * 32);
             d.sync = (unsigned int*)(intptr_t)(nsync++);  // index, resolved below
           } else {
@@ -2413,7 +2430,7 @@ void Engine<T>::build_solve_program() {
         s.mmax = mmax1;
         s.nwg_total = prefix_tiles(cnt, st);
         s.nd = (int)ds.size();
-        if (mmax1 > refl_lds_cap(sizeof(T))) s.work = alloc_perm((size_t)s.nwg_total * mmax1);
+        if (!reg && mmax1 > refl_lds_cap(sizeof(T))) s.work = alloc_perm((size_t)s.nwg_total * mmax1);
         h.refl = ds;
       } else if (h.type == 1) {
         std::vector<int> want;
@@ -2425,6 +2442,7 @@ void Engine<T>::build_solve_program() {
         std::vector<BwdDesc<T>> ds;
         for (int i : order) {
           BwdDesc<T> d = h.bwd[i];
+          s.kmax = std::max(s.kmax, d.k);
           d.nwg = want[i];
           if (d.nwg > 1) d.sync = (unsigned int*)(intptr_t)(nsync++);
           ds.push_back(d);
@@ -2555,7 +2573,8 @@ void Engine<T>::run_program(const double* b, double* x, bool prof, int mode) {
       SPQ_HIP(hipEventRecord(ev.back(), stream_));
     }
     T* xb = s.buf ? xtbuf_ : xbuf_;
-    if (s.type == 0) launch_refl<T>(stream_, s.refl, s.wstart, s.nd, s.nwg_total, segs_dev_, xb, s.work, s.mmax);
+    if (s.type == 0 && s.form == 1) launch_reflr<T>(stream_, s.refl, s.wstart, s.nd, s.nwg_total, segs_dev_, xb, s.nseg_max);
+    else if (s.type == 0) launch_refl<T>(stream_, s.refl, s.wstart, s.nd, s.nwg_total, segs_dev_, xb, s.work, s.mmax);
     else if (s.type == 1) launch_bwd<T>(stream_, s.bwd, s.wstart, s.nd, s.nwg_total, r12_dev_, xb);
     else if (s.type == 3) launch_r12<T>(stream_, s.r12c, s.nd, xb);
     else if (s.type == 4)
@@ -2593,8 +2612,8 @@ void Engine<T>::run_program(const double* b, double* x, bool prof, int mode) {
       size_t j = i, pi = 0;
       while (j >= progs[pi]->size()) j -= progs[pi++]->size();
       const Stage& s = (*progs[pi])[j];
-      fprintf(stderr, "stage p%zu %3zu type %d ops %5d wgs %5d mmax %6d ms %.4f\n", pi, j, s.type, s.nd,
-              s.type == 2 ? s.nd : s.nwg_total, s.mmax, ms);
+      fprintf(stderr, "stage p%zu %3zu type %d form %d ops %5d wgs %5d mmax %6d mall %6d kmax %5d ms %.4f\n", pi, j,
+              s.type, s.form, s.nd, s.type == 2 ? s.nd : s.nwg_total, s.mmax, s.mall, s.kmax, ms);
     }
     fprintf(stderr, "stages total ms %.3f\n", tot);
     for (auto e : ev) (void)hipEventDestroy(e);

@@ -63,6 +63,12 @@ template <typename T>
 int launch_qp3(hipStream_t s, const Qp3Desc<T>* d, const int* wstart, int nd, int nwg_total, int kmax, int coop);
 
 // solve.hip
+// register form of the reflector apply: <= reflr_rows() rows per workgroup
+int reflr_rows();
+size_t reflr_lds(int nseg_max);
+template <typename T>
+void launch_reflr(hipStream_t s, const ReflDesc<T>* d, const int* wstart, int nd, int nwg_total, const Seg* segs, T* xbuf,
+                  int nseg_max);
 int refl_lds_cap(int elem);
 template <typename T>
 void launch_refl(hipStream_t s, const ReflDesc<T>* d, const int* wstart, int nd, int nwg_total, const Seg* segs,

@@ -165,6 +165,248 @@ __global__ __launch_bounds__(kSThreads) void k_refl(const ReflDesc<T>* __restric
   }
 }
 
+// ---------------------------------------------------------------------------
+// Register form of the reflector apply, for stages whose ops are few (the
+// dependency waves of the QR forward sweep and the large ops of the upper
+// levels), where one op's time, not the stage's throughput, sets the pace.
+// Each row thread owns one row of the op's slice (<= kRRows rows per
+// workgroup): its x entry and its 32 entries of the current reflector block
+// stay in registers, so V is read once, and the next block's row is loaded
+// while the block's sums are combined.  A leader wave owning no rows sums
+// the row waves' partials, exchanges them between the workgroups of a
+// split op (agent-scope stores and one relaxed barrier per block) and forms
+// z = T_b^T y (or T_b y) from the T_b column (row) it holds in registers.
+constexpr int kRRows = 448;  // 7 row waves + the leader: 2 waves per SIMD, 256 VGPRs each
+constexpr int kRRowWaves = kRRows / kWave;
+constexpr int kRThreads = kRRows + kWave;
+
+int reflr_rows() { return kRRows; }
+size_t reflr_lds(int nseg_max) {
+  return (((size_t)nseg_max + 1) * sizeof(int) + 7) / 8 * 8 + (size_t)std::max(nseg_max, 1) * sizeof(int64_t);
+}
+
+// Global-address-space view of a pointer read from a descriptor (global
+// loads instead of flat ones, which would also count against the LDS wait
+// counter).
+template <typename T>
+using gptr = const __attribute__((address_space(1))) T*;
+template <typename T>
+__device__ __forceinline__ gptr<T> to_global(const T* p) {
+  return (gptr<T>)p;
+}
+template <typename T>
+using gmut = __attribute__((address_space(1))) T*;
+template <typename T>
+__device__ __forceinline__ gmut<T> to_global_mut(T* p) {
+  return (gmut<T>)p;
+}
+// The relaxed-barrier protocol of kcommon.hpp (agent-scope stores and loads
+// of the exchanged values, relaxed counter) on global pointers: flat
+// accesses would make every later LDS wait also wait for vector memory.
+__device__ __forceinline__ void barrier_arrive_wait_g(unsigned int* ctr, unsigned int target, int* err) {
+  const gmut<unsigned int> c = to_global_mut(ctr);
+  const gmut<int> e = to_global_mut(err);
+  __hip_atomic_fetch_add(c, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  unsigned int spins = 0;
+  while (__hip_atomic_load(c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+    __builtin_amdgcn_s_sleep(2);
+    ++spins;
+    if ((spins & 255u) == 0 && __hip_atomic_load(e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return;
+    if (spins > (1u << 20)) {
+      __hip_atomic_fetch_or(e, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      return;
+    }
+  }
+}
+// base[byte_off / sizeof(T)] for a uniform base and a 32-bit byte offset
+// (one scalar base register pair plus one vector register per load)
+template <typename T>
+__device__ __forceinline__ T load_at(const T* base, unsigned byte_off) {
+  return *(gptr<T>)((const __attribute__((address_space(1))) char*)to_global(base) + byte_off);
+}
+
+// Orders one wavefront's LDS and memory accesses around an exchange among its
+// own lanes (no other wave takes part).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+template <typename T>
+__global__ __launch_bounds__(kRThreads) void k_reflr(const ReflDesc<T>* __restrict__ descs, const int* __restrict__ wstart,
+                                                     int nd, const Seg* __restrict__ segs, T* __restrict__ xbuf) {
+  extern __shared__ unsigned char smem_raw[];
+  __shared__ T red[kRNB * kRRowWaves];
+  __shared__ T tsh[kRNB * kTLd];
+  __shared__ T ysh[kRNB];
+  __shared__ T zsh[kRNB];
+  const int p = find_owner(wstart, nd, blockIdx.x);
+  const ReflDesc<T> d = descs[p];
+  const int g = blockIdx.x - wstart[p];
+  const int m = d.m, tid = threadIdx.x, lane = tid & (kWave - 1), wv = tid / kWave;
+  const bool leader = wv == kRRowWaves;
+  const int nblk = (d.k + kRNB - 1) / kRNB;
+  const bool multi = d.nwg > 1;
+  const int r0 = multi ? (int)((int64_t)g * m / d.nwg) : 0;
+  const int r1 = multi ? (int)((int64_t)(g + 1) * m / d.nwg) : m;
+  const int i = r0 + tid;
+  const bool own = !leader && i < r1;
+  const int nseg = d.nseg;
+  // segment table in LDS: row base (exclusive prefix of the lengths) and
+  // x buffer offset of every segment, built by the first wave
+  int* sbase = reinterpret_cast<int*>(smem_raw);
+  int64_t* soff = reinterpret_cast<int64_t*>(smem_raw + (((size_t)nseg + 1) * sizeof(int) + 7) / 8 * 8);
+  if (wv == 0) {
+    int carry = 0;
+    for (int s0 = 0; s0 < nseg; s0 += kWave) {
+      const int s = s0 + lane;
+      int len = 0;
+      if (s < nseg) {
+        const Seg sg = segs[d.seg0 + s];
+        len = sg.len;
+        soff[s] = sg.off;
+      }
+      int v = len;
+#pragma unroll
+      for (int o = 1; o < kWave; o <<= 1) {
+        const int u = __shfl_up(v, o, kWave);
+        if (lane >= o) v += u;
+      }
+      if (s < nseg) sbase[s] = carry + v - len;
+      carry += __shfl(v, kWave - 1, kWave);
+    }
+  }
+  __syncthreads();
+  int64_t xo = 0;
+  T xr = T(0);
+  if (own) {
+    int lo = 0, hi = nseg - 1;  // the last segment whose base is <= i holds row i
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (sbase[mid] <= i) lo = mid;
+      else hi = mid - 1;
+    }
+    xo = soff[lo] + (i - sbase[lo]);
+    xr = xbuf[xo];
+  }
+  const T* __restrict__ V = d.V;
+  const int64_t ldv = d.ldv;
+  // V(i, jb + c) of block t: load_v issues 32 unconditional global loads,
+  // each from a uniform column base plus the thread's row (rows above the
+  // block, and idle threads, read row jb, one line per wave), and mask_v,
+  // run when they are needed, applies the unit-lower structure (1 on the
+  // diagonal, 0 above, 0 past column k).
+  auto load_v = [&](int t, T (&raw)[kRNB]) {
+    const int bi = d.trans ? t : nblk - 1 - t;
+    const int jb = bi * kRNB, w = min(kRNB, d.k - jb);
+    const unsigned off = (unsigned)((own && i > jb) ? i : min(jb, m - 1)) * (unsigned)sizeof(T);
+#pragma unroll
+    for (int c = 0; c < kRNB; ++c) raw[c] = load_at(V + (int64_t)(jb + min(c, w - 1)) * ldv, off);
+  };
+  auto mask_v = [&](int t, T (&v)[kRNB]) {
+    const int bi = d.trans ? t : nblk - 1 - t;
+    const int jb = bi * kRNB, w = min(kRNB, d.k - jb);
+    const int dc = own ? i - jb : -1;  // the diagonal's column in this block, if any
+#pragma unroll
+    for (int c = 0; c < kRNB; ++c) v[c] = (c < w && c < dc) ? v[c] : ((c < w && c == dc) ? T(1) : T(0));
+  };
+  // the leader stages T_b of block t into LDS (ld kTLd)
+  auto load_t = [&](int t) {
+    const int bi = d.trans ? t : nblk - 1 - t;
+    // entry e = lane + 64 q is (e % 32, e / 32) = (lane % 32, lane / 32 + 2 q)
+    const gptr<T> tb = to_global(d.Tb + (int64_t)bi * kRNB * kRNB + lane);
+    T* dst = tsh + (lane & (kRNB - 1)) + (lane / kRNB) * kTLd;
+    T tv[kRNB * kRNB / kWave];
+#pragma unroll
+    for (int q = 0; q < kRNB * kRNB / kWave; ++q) tv[q] = tb[q * kWave];
+#pragma unroll
+    for (int q = 0; q < kRNB * kRNB / kWave; ++q) dst[q * 2 * kTLd] = tv[q];
+  };
+  unsigned int target = 0;
+  T* __restrict__ part = d.part;
+  // one block: partial sums from `cur` (block t), prefetch of block t + 1
+  // into `nxt`, the leader's reduction / exchange / T_b product, update
+  auto step = [&](int t, T (&cur)[kRNB], T (&nxt)[kRNB]) {
+    const int bi = d.trans ? t : nblk - 1 - t;
+    const int w = min(kRNB, d.k - bi * kRNB);
+    // every wave, the leader included (its loads all read row jb), keeps the
+    // register sets defined alike on all paths, so the compiler's wait
+    // counters see `cur` as complete wherever it is read
+    mask_v(t, cur);
+    if (!leader) {
+      T acc[kRNB];
+#pragma unroll
+      for (int c = 0; c < kRNB; ++c) acc[c] = cur[c] * xr;
+      const T sw = wave_reduce_scatter32(acc);
+      if (lane < kRNB) red[wv * kRNB + lane] = sw;
+    }
+    if (t + 1 < nblk) load_v(t + 1, nxt);  // in flight across the two barriers
+    __syncthreads();
+    if (leader) {
+      const int c = lane;
+      T y = T(0);
+      if (c < kRNB) {
+#pragma unroll
+        for (int q = 0; q < kRRowWaves; ++q) y += red[q * kRNB + c];
+      }
+      if (multi) {
+        const gmut<T> pb = to_global_mut(part) + (int64_t)(t & 1) * d.nwg * kRNB + c;
+        if (c < kRNB) __hip_atomic_store(pb + g * kRNB, y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __builtin_amdgcn_s_waitcnt(0);  // this wave's partials have landed
+        target += d.nwg;
+        if (lane == 0) barrier_arrive_wait_g(d.sync, target, d.err);
+        wave_sync();
+        if (c < kRNB) {
+          y = T(0);
+          for (int q = 0; q < d.nwg; ++q) y += __hip_atomic_load(pb + q * kRNB, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+      }
+      if (c < kRNB) ysh[c] = y;
+      wave_sync();
+      if (c < kRNB) {
+        T z = T(0);
+        if (c < w) {
+          if (d.trans) {
+            for (int l = 0; l <= c; ++l) z += tsh[l + c * kTLd] * ysh[l];
+          } else {
+            for (int l = c; l < w; ++l) z += tsh[c + l * kTLd] * ysh[l];
+          }
+        }
+        zsh[c] = z;
+      }
+    }
+    __syncthreads();
+    // the row waves' branch comes first in program order, so its wait on
+    // zsh does not also wait for the prefetch of `nxt`
+    if (!leader) {
+      T s = T(0);
+#pragma unroll
+      for (int c = 0; c < kRNB; ++c) s += cur[c] * zsh[c];
+      xr -= s;
+    }
+    if (leader && t + 1 < nblk) load_t(t + 1);  // read again only after the next block's first barrier
+  };
+  // two register sets that swap roles block by block (no copies)
+  T va[kRNB], vb[kRNB];
+  if (nblk > 0) {
+    load_v(0, va);
+    if (leader) load_t(0);
+  }
+  for (int t = 0; t < nblk; t += 2) {
+    step(t, va, vb);
+    if (t + 1 < nblk) step(t + 1, vb, va);
+  }
+  if (own) xbuf[xo] = xr;
+}
+
+template <typename T>
+void launch_reflr(hipStream_t s, const ReflDesc<T>* d, const int* wstart, int nd, int nwg_total, const Seg* segs, T* xbuf,
+                  int nseg_max) {
+  if (nd <= 0) return;
+  hipLaunchKernelGGL((k_reflr<T>), dim3(nwg_total), dim3(kRThreads), reflr_lds(nseg_max), s, d, wstart, nd, segs, xbuf);
+}
+
 // Solve the w x w diagonal block (staged in LDS, ld kTLd) in place on
 // xs[jb:jb+w] with the first wavefront (lane l holds x_{jb+l});
 // trans: R^T (forward substitution).
@@ -397,6 +639,7 @@ void launch_scatter_out(hipStream_t s, const int64_t* src, const int* dst, int n
 
 #define SPQ_INST(T)                                                                                            \
   template void launch_refl<T>(hipStream_t, const ReflDesc<T>*, const int*, int, int, const Seg*, T*, T*, int); \
+  template void launch_reflr<T>(hipStream_t, const ReflDesc<T>*, const int*, int, int, const Seg*, T*, int);     \
   template void launch_bwd<T>(hipStream_t, const BwdDesc<T>*, const int*, int, int, const R12Desc<T>*, T*);     \
   template void launch_move<T>(hipStream_t, const MoveDesc*, int, T*, const T*);                               \
   template void launch_r12<T>(hipStream_t, const R12Chunk<T>*, int, T*);                                        \

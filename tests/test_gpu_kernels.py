@@ -33,7 +33,7 @@ def K(ktest):
     ktest.ktest_qr.argtypes = [C.c_int, C.c_int, dp, dp, dp, C.c_int, dp]
     ktest.ktest_larft.argtypes = [C.c_int, C.c_int, dp, dp, dp]
     ktest.ktest_qp3.argtypes = [C.c_int, C.c_int, dp, dp, dp, C.c_double, C.c_int, C.c_int]
-    ktest.ktest_refl.argtypes = [C.c_int, C.c_int, dp, dp, dp, C.c_int, C.c_int, C.c_int]
+    ktest.ktest_refl.argtypes = [C.c_int, C.c_int, dp, dp, dp, C.c_int, C.c_int, C.c_int, C.c_int]
     ktest.ktest_bwd.argtypes = [C.c_int, dp, C.c_int, dp, dp, dp, C.c_int]
     ktest.ktest_trsm.argtypes = [C.c_int, C.c_int, dp, dp]
     ktest.ktest_copy.argtypes = [C.c_int, C.c_int, dp, C.c_int, dp, C.c_int, C.c_int]
@@ -254,11 +254,14 @@ def test_qp3(K, oracle, kN, true_rank, tol, form, nwg):
     assert rel((np.tril(V[:, :rank], -1) + np.eye(k, rank))[:, well], Vref[:, well]) < 1e-9
 
 
+@pytest.mark.parametrize("form", [0, 1])
 @pytest.mark.parametrize("trans", [1, 0])
-# m >= 2048 runs the row-sliced multi-workgroup form (4 workgroups)
-@pytest.mark.parametrize("mk,nseg", [((40, 40), 1), ((150, 70), 3), ((600, 100), 4), ((20000, 40), 2),
-                                     ((5100, 300), 3)])
-def test_refl(K, oracle, mk, nseg, trans):
+# form 0: m >= 2048 runs the row-sliced multi-workgroup form (4 workgroups);
+# form 1 (register form): ceil(m / 448) workgroups, so 600 rows use 2 and
+# 5100 rows 12; 97 segments exceed one wave of the segment-table scan
+@pytest.mark.parametrize("mk,nseg", [((40, 40), 1), ((150, 70), 3), ((448, 64), 2), ((600, 100), 4), ((20000, 40), 2),
+                                     ((5100, 300), 3), ((97 * 7, 90), 97)])
+def test_refl(K, oracle, mk, nseg, trans, form):
     m, k = mk
     assert m % nseg == 0  # the test splits x into nseg equal segments
     rng = np.random.default_rng(m + k)
@@ -270,7 +273,7 @@ def test_refl(K, oracle, mk, nseg, trans):
     Tfull = oracle.larft(V, tau)
     ref = oracle.larfb(V, Tfull, x.reshape(m, 1), "T" if trans else "N")[:, 0]
     xx = x.copy()
-    assert K.ktest_refl(m, k, d(V), d(Tb), d(xx), trans, nseg, 5) == 0
+    assert K.ktest_refl(m, k, d(V), d(Tb), d(xx), trans, nseg, 5, form) == 0
     assert rel(xx, ref) < 1e-12
 
 
