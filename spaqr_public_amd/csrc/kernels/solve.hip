@@ -225,6 +225,17 @@ __device__ __forceinline__ T load_at(const T* base, unsigned byte_off) {
   return *(gptr<T>)((const __attribute__((address_space(1))) char*)to_global(base) + byte_off);
 }
 
+// Value of v in lane l (l uniform) for every lane of the wave.
+__device__ __forceinline__ double lane_bcast(double v, int l) {
+  const unsigned long long b = __double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, l);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), l);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+__device__ __forceinline__ float lane_bcast(float v, int l) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+
 // Orders one wavefront's LDS and memory accesses around an exchange among its
 // own lanes (no other wave takes part).
 __device__ __forceinline__ void wave_sync() {
@@ -233,15 +244,33 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
 }
 
+// Descriptor owning workgroup `t` (start[] exclusive prefix sums, strictly
+// increasing): every lane of the wave tests 64 entries per load round, so
+// a launch of up to 256 descriptors costs one round of loads.
+__device__ __forceinline__ int owner_by_ballot(const int* __restrict__ start, int n, int t) {
+  const int lane = threadIdx.x & (kWave - 1);
+  int cnt = 0;
+  for (int s0 = 0; s0 < n; s0 += 4 * kWave) {
+    int v[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int s = s0 + q * kWave + lane;
+      v[q] = s < n ? start[s] : INT_MAX;
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) cnt += __popcll(__ballot(v[q] <= t));
+  }
+  return cnt - 1;
+}
+
 template <typename T>
 __global__ __launch_bounds__(kRThreads) void k_reflr(const ReflDesc<T>* __restrict__ descs, const int* __restrict__ wstart,
                                                      int nd, const Seg* __restrict__ segs, T* __restrict__ xbuf) {
   extern __shared__ unsigned char smem_raw[];
   __shared__ T red[kRNB * kRRowWaves];
   __shared__ T tsh[kRNB * kTLd];
-  __shared__ T ysh[kRNB];
   __shared__ T zsh[kRNB];
-  const int p = find_owner(wstart, nd, blockIdx.x);
+  const int p = owner_by_ballot(wstart, nd, blockIdx.x);
   const ReflDesc<T> d = descs[p];
   const int g = blockIdx.x - wstart[p];
   const int m = d.m, tid = threadIdx.x, lane = tid & (kWave - 1), wv = tid / kWave;
@@ -253,6 +282,75 @@ __global__ __launch_bounds__(kRThreads) void k_reflr(const ReflDesc<T>* __restri
   const int i = r0 + tid;
   const bool own = !leader && i < r1;
   const int nseg = d.nseg;
+  const T* __restrict__ V = d.V;
+  const int64_t ldv = d.ldv;
+  // V(i, jb + c) of block t: load_v issues 32 unconditional global loads,
+  // each from a uniform column base plus the thread's row (rows above the
+  // block, and idle threads, read row jb, one line per wave), and mask_v,
+  // run when they are needed, applies the unit-lower structure (1 on the
+  // diagonal, 0 above, 0 past column k).
+  auto load_v = [&](int t, T (&raw)[kRNB]) {
+    const int bi = d.trans ? t : nblk - 1 - t;
+    const int jb = bi * kRNB, w = min(kRNB, d.k - jb);
+    const unsigned off = (unsigned)((own && i > jb) ? i : min(jb, m - 1)) * (unsigned)sizeof(T);
+    const T* col = V + (int64_t)jb * ldv;  // uniform; advanced column by column
+    if (w == kRNB) {
+#pragma unroll
+      for (int c = 0; c < kRNB; ++c) {
+        raw[c] = load_at(col, off);
+        col += ldv;
+      }
+    } else {
+#pragma unroll
+      for (int c = 0; c < kRNB; ++c) {
+        raw[c] = load_at(col, off);
+        if (c + 1 < w) col += ldv;
+      }
+    }
+  };
+  // which rows of this wave the block touches (uniform per wave): none (all
+  // rows above the block), all strictly below a full block with every row
+  // owned (no masking), or some (per-entry masking)
+  const int wrow0 = r0 + wv * kWave;
+  auto wave_class = [&](int t) {
+    const int bi = d.trans ? t : nblk - 1 - t;
+    const int jb = bi * kRNB, w = min(kRNB, d.k - jb);
+    if (leader || wrow0 >= r1 || wrow0 + kWave - 1 < jb) return 0;
+    if (w == kRNB && wrow0 >= jb + kRNB && wrow0 + kWave <= r1) return 1;
+    return 2;
+  };
+  auto mask_v = [&](int t, T (&v)[kRNB]) {
+    const int bi = d.trans ? t : nblk - 1 - t;
+    const int jb = bi * kRNB, w = min(kRNB, d.k - jb);
+    const int dc = own ? i - jb : -1;  // the diagonal's column in this block, if any
+#pragma unroll
+    for (int c = 0; c < kRNB; ++c) v[c] = (c < w && c < dc) ? v[c] : ((c < w && c == dc) ? T(1) : T(0));
+  };
+  // the leader loads T_b of block t into registers (entry e = lane + 64 q is
+  // (e % 32, e / 32) = (lane % 32, lane / 32 + 2 q)) and later stages it
+  // into LDS (ld kTLd)
+  constexpr int kTQ = kRNB * kRNB / kWave;
+  auto load_t = [&](int t, T (&tv)[kTQ]) {
+    const int bi = d.trans ? t : nblk - 1 - t;
+    const gptr<T> tb = to_global(d.Tb + (int64_t)bi * kRNB * kRNB + lane);
+#pragma unroll
+    for (int q = 0; q < kTQ; ++q) tv[q] = tb[q * kWave];
+  };
+  auto store_t = [&](const T (&tv)[kTQ]) {
+    T* dst = tsh + (lane & (kRNB - 1)) + (lane / kRNB) * kTLd;
+#pragma unroll
+    for (int q = 0; q < kTQ; ++q) dst[q * 2 * kTLd] = tv[q];
+  };
+  // the first block's loads go out before the gather; every wave, the
+  // leader included (its loads all read row jb), keeps the register sets
+  // defined alike on all paths, so the compiler's wait counters see them
+  // as complete wherever they are read
+  T va[kRNB], vb[kRNB];
+  T tv[kTQ];
+  if (nblk > 0) {
+    load_v(0, va);
+    if (leader) load_t(0, tv);
+  }
   // segment table in LDS: row base (exclusive prefix of the lengths) and
   // x buffer offset of every segment, built by the first wave
   int* sbase = reinterpret_cast<int*>(smem_raw);
@@ -290,60 +388,33 @@ __global__ __launch_bounds__(kRThreads) void k_reflr(const ReflDesc<T>* __restri
     xo = soff[lo] + (i - sbase[lo]);
     xr = xbuf[xo];
   }
-  const T* __restrict__ V = d.V;
-  const int64_t ldv = d.ldv;
-  // V(i, jb + c) of block t: load_v issues 32 unconditional global loads,
-  // each from a uniform column base plus the thread's row (rows above the
-  // block, and idle threads, read row jb, one line per wave), and mask_v,
-  // run when they are needed, applies the unit-lower structure (1 on the
-  // diagonal, 0 above, 0 past column k).
-  auto load_v = [&](int t, T (&raw)[kRNB]) {
-    const int bi = d.trans ? t : nblk - 1 - t;
-    const int jb = bi * kRNB, w = min(kRNB, d.k - jb);
-    const unsigned off = (unsigned)((own && i > jb) ? i : min(jb, m - 1)) * (unsigned)sizeof(T);
-#pragma unroll
-    for (int c = 0; c < kRNB; ++c) raw[c] = load_at(V + (int64_t)(jb + min(c, w - 1)) * ldv, off);
-  };
-  auto mask_v = [&](int t, T (&v)[kRNB]) {
-    const int bi = d.trans ? t : nblk - 1 - t;
-    const int jb = bi * kRNB, w = min(kRNB, d.k - jb);
-    const int dc = own ? i - jb : -1;  // the diagonal's column in this block, if any
-#pragma unroll
-    for (int c = 0; c < kRNB; ++c) v[c] = (c < w && c < dc) ? v[c] : ((c < w && c == dc) ? T(1) : T(0));
-  };
-  // the leader stages T_b of block t into LDS (ld kTLd)
-  auto load_t = [&](int t) {
-    const int bi = d.trans ? t : nblk - 1 - t;
-    // entry e = lane + 64 q is (e % 32, e / 32) = (lane % 32, lane / 32 + 2 q)
-    const gptr<T> tb = to_global(d.Tb + (int64_t)bi * kRNB * kRNB + lane);
-    T* dst = tsh + (lane & (kRNB - 1)) + (lane / kRNB) * kTLd;
-    T tv[kRNB * kRNB / kWave];
-#pragma unroll
-    for (int q = 0; q < kRNB * kRNB / kWave; ++q) tv[q] = tb[q * kWave];
-#pragma unroll
-    for (int q = 0; q < kRNB * kRNB / kWave; ++q) dst[q * 2 * kTLd] = tv[q];
-  };
+  if (leader && nblk > 0) store_t(tv);
   unsigned int target = 0;
   T* __restrict__ part = d.part;
   // one block: partial sums from `cur` (block t), prefetch of block t + 1
-  // into `nxt`, the leader's reduction / exchange / T_b product, update
+  // into `nxt` (and of T_{t+1} by the leader, staged after the second
+  // barrier), the leader's reduction / exchange / T_b product, update
   auto step = [&](int t, T (&cur)[kRNB], T (&nxt)[kRNB]) {
     const int bi = d.trans ? t : nblk - 1 - t;
     const int w = min(kRNB, d.k - bi * kRNB);
-    // every wave, the leader included (its loads all read row jb), keeps the
-    // register sets defined alike on all paths, so the compiler's wait
-    // counters see `cur` as complete wherever it is read
-    mask_v(t, cur);
+    const int cls = wave_class(t);
+    if (cls == 2) mask_v(t, cur);
+    if (t + 1 < nblk) load_v(t + 1, nxt);  // in flight across this block's work
     if (!leader) {
-      T acc[kRNB];
+      T sw = T(0);
+      if (cls != 0) {
+        T acc[kRNB];
 #pragma unroll
-      for (int c = 0; c < kRNB; ++c) acc[c] = cur[c] * xr;
-      const T sw = wave_reduce_scatter32(acc);
+        for (int c = 0; c < kRNB; ++c) acc[c] = cur[c] * xr;
+        sw = wave_reduce_scatter32(acc);
+      }
       if (lane < kRNB) red[wv * kRNB + lane] = sw;
     }
-    if (t + 1 < nblk) load_v(t + 1, nxt);  // in flight across the two barriers
     __syncthreads();
     if (leader) {
+      // T_{t+1} is loaded while this block's sums are combined (after the
+      // partials' store in a split op, whose wait would include it)
+      if (!multi && t + 1 < nblk) load_t(t + 1, tv);
       const int c = lane;
       T y = T(0);
       if (c < kRNB) {
@@ -354,45 +425,53 @@ __global__ __launch_bounds__(kRThreads) void k_reflr(const ReflDesc<T>* __restri
         const gmut<T> pb = to_global_mut(part) + (int64_t)(t & 1) * d.nwg * kRNB + c;
         if (c < kRNB) __hip_atomic_store(pb + g * kRNB, y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __builtin_amdgcn_s_waitcnt(0);  // this wave's partials have landed
+        if (t + 1 < nblk) load_t(t + 1, tv);
         target += d.nwg;
         if (lane == 0) barrier_arrive_wait_g(d.sync, target, d.err);
         wave_sync();
-        if (c < kRNB) {
-          y = T(0);
-          for (int q = 0; q < d.nwg; ++q) y += __hip_atomic_load(pb + q * kRNB, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        // the partials of all workgroups, 8 loads in flight at a time,
+        // summed in workgroup order (every workgroup forms the same y)
+        y = T(0);
+        for (int q0 = 0; q0 < d.nwg; q0 += 8) {
+          T pv[8];
+#pragma unroll
+          for (int u = 0; u < 8; ++u)
+            pv[u] = __hip_atomic_load(pb + min(q0 + u, d.nwg - 1) * kRNB, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+#pragma unroll
+          for (int u = 0; u < 8; ++u) y += (q0 + u < d.nwg) ? pv[u] : T(0);
         }
       }
-      if (c < kRNB) ysh[c] = y;
-      wave_sync();
-      if (c < kRNB) {
-        T z = T(0);
-        if (c < w) {
-          if (d.trans) {
-            for (int l = 0; l <= c; ++l) z += tsh[l + c * kTLd] * ysh[l];
-          } else {
-            for (int l = c; l < w; ++l) z += tsh[c + l * kTLd] * ysh[l];
-          }
+      // z = T_b^T y (trans) or T_b y: y_l broadcast from lane l, T_b from
+      // LDS (conflict-free through the padded leading dimension); entries
+      // outside the triangle are never combined
+      // lane c combines entries l in [lo, hi] of column c of T_b (trans)
+      // or of row c (no trans)
+      const int cc = c & (kRNB - 1);
+      const int tb0 = d.trans ? cc * kTLd : cc, tstr = d.trans ? 1 : kTLd;
+      const int lo = d.trans ? 0 : cc, hi = d.trans ? cc : w - 1;
+      T z = T(0);
+#pragma nounroll
+      for (int l0 = 0; l0 < kRNB; l0 += 8) {
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int l = l0 + u;
+          const T yl = lane_bcast(y, l);
+          const T tl = tsh[tb0 + l * tstr];
+          z = (l >= lo && l <= hi) ? fma(tl, yl, z) : z;
         }
-        zsh[c] = z;
       }
+      if (c < kRNB) zsh[c] = (c < w) ? z : T(0);
     }
     __syncthreads();
-    // the row waves' branch comes first in program order, so its wait on
-    // zsh does not also wait for the prefetch of `nxt`
-    if (!leader) {
+    if (cls != 0) {
       T s = T(0);
 #pragma unroll
       for (int c = 0; c < kRNB; ++c) s += cur[c] * zsh[c];
       xr -= s;
     }
-    if (leader && t + 1 < nblk) load_t(t + 1);  // read again only after the next block's first barrier
+    if (leader && t + 1 < nblk) store_t(tv);  // read again only after the next block's first barrier
   };
   // two register sets that swap roles block by block (no copies)
-  T va[kRNB], vb[kRNB];
-  if (nblk > 0) {
-    load_v(0, va);
-    if (leader) load_t(0);
-  }
   for (int t = 0; t < nblk; t += 2) {
     step(t, va, vb);
     if (t + 1 < nblk) step(t + 1, vb, va);
