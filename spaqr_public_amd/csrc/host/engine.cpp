@@ -155,6 +155,8 @@ class Engine final : public EngineBase {
     std::vector<int> children;
     int rpos = 0, cpos = 0;
     std::vector<int> out, in;  // edge ids in reference list order
+    std::vector<int> outn;     // n2 of each out edge (same order as out; scanned by find_out)
+    std::vector<int> inn;      // n1 of each in edge (same order as in)
     std::vector<int> rsp;      // rsparsity, creation order
     std::vector<int> cnbrs;    // sorted, creation order
     int64_t xoff = -1;
@@ -235,17 +237,25 @@ class Engine final : public EngineBase {
       if (it == cn.end() || *it != c2) cn.insert(it, c2);
     }
     cl_[c1].out.push_back(id);
-    if (c1 != c2 && link_in) cl_[c2].in.push_back(id);
+    cl_[c1].outn.push_back(c2);
+    if (c1 != c2 && link_in) {
+      cl_[c2].in.push_back(id);
+      cl_[c2].inn.push_back(c1);
+    }
     return id;
   }
   int find_out(int c, int n2) const {
-    for (int e : cl_[c].out)
-      if (ed_[e].n2 == n2) return e;
+    const Clu& k = cl_[c];
+    for (size_t q = 0; q < k.outn.size(); ++q)
+      if (k.outn[q] == n2) return k.out[q];
     return -1;
   }
   void sort_out(int c) {
     auto& o = cl_[c].out;
     std::stable_sort(o.begin(), o.end(), [this](int a, int b) { return cl_[ed_[a].n2].order < cl_[ed_[b].n2].order; });
+    auto& on = cl_[c].outn;
+    on.resize(o.size());
+    for (size_t q = 0; q < o.size(); ++q) on[q] = ed_[o[q]].n2;
   }
   std::pair<int, int> topsize() const {
     int r = 0, c = 0;
@@ -577,7 +587,7 @@ class Engine final : public EngineBase {
     int type;  // 0 refl, 1 bwd, 2 move
     int nd = 0, mmax = 0, nwg_total = 0;
     int kmax = 0, mall = 0;  // largest k and m of the stage's ops (diagnostics)
-    int form = 0, nseg_max = 0;  // reflector stages: 1 = register form (k_reflr)
+    int form = 0, nseg_max = 0;  // 1: register form (k_reflr) / latency form (k_bwdr)
     int buf = 0, sbuf = 0;  // x buffer used (0 = x, 1 = xt); moves copy sbuf -> buf
     const ReflDesc<T>* refl = nullptr;
     const BwdDesc<T>* bwd = nullptr;
@@ -624,6 +634,10 @@ class Engine final : public EngineBase {
   bool solve_graph_ = true;  // SPAQR_SOLVE_GRAPH=0: launch the solve program directly (no hipGraph)
   bool qp3_reg_ = true;      // SPAQR_QP3_REGISTER=0: streaming truncated QR only
   bool reflr_ = true;        // SPAQR_REFL_REGISTER=0: solve reflector stages in the LDS form only
+  // per-cluster scratch of the host bookkeeping (sized to cl_, reused)
+  std::vector<unsigned> mark_;  // == mark_stamp_: in the set being built
+  unsigned mark_stamp_ = 0;
+  std::vector<int> pos_;        // panel position of a cluster, -1 when none
   int panel_cap_ = -1;
   int panel_r_cap() {
     if (panel_cap_ < 0) panel_cap_ = panel_r_capacity<T>();
@@ -738,7 +752,11 @@ void Engine<T>::assemble(const CscMat& A) {  // tree.cpp:725-778
       const int id = (int)ed_.size() - 1;
       pend.push_back({id, off});
       cl_[self].out.push_back(id);
-      if (self != n) cl_[n].in.push_back(id);
+      cl_[self].outn.push_back(n);
+      if (self != n) {
+        cl_[n].in.push_back(id);
+        cl_[n].inn.push_back(self);
+      }
     }
     sort_out(self);
     cl_[self].cnbrs = nb;
@@ -778,18 +796,25 @@ void Engine<T>::get_sparsity(int c) {  // tree.cpp:834-870 (+ check_if_valid 168
     if (!has(nid.l) && !has(nid.r)) return false;
     return true;
   };
+  // the distinct clusters found, marked as they are first accepted (a
+  // cluster already in the set needs no further test), then sorted
   std::vector<int> rs;
+  if (mark_.size() < cl_.size()) mark_.resize(cl_.size(), 0);
+  const unsigned stamp = ++mark_stamp_;
+  auto take = [&](int n) {
+    if (mark_[n] != stamp) { mark_[n] = stamp; rs.push_back(n); }
+  };
   for (int e : cl_[c].out) {
     if (!ed_[e].alive) continue;
     const int n = ed_[e].n2;
-    rs.push_back(n);
+    take(n);
     for (int ein : cl_[n].in) {
       const int nin = ed_[ein].n1;
-      if (ed_[ein].alive && nin != c && !cl_[nin].elim && valid(cl_[nin].id)) rs.push_back(nin);
+      if (mark_[nin] == stamp) continue;
+      if (ed_[ein].alive && nin != c && !cl_[nin].elim && valid(cl_[nin].id)) take(nin);
     }
   }
   std::sort(rs.begin(), rs.end());
-  rs.erase(std::unique(rs.begin(), rs.end()), rs.end());
   cl_[c].rsp = rs;
 }
 
@@ -820,21 +845,31 @@ void Engine<T>::eliminate_level(const std::vector<int>& cs, int fine_wave) {
     for (int e : cl_[c].out)
       if (ed_[e].alive) { p.pe.push_back(e); p.m += ed_[e].a.rows; }
     if (ed_[p.pe[0]].n2 != c) throw Error(kErrState, "eliminate: self block is not first");
+    // the target blocks of row n in the panel's columns: one scan of n's
+    // out edges finds the existing ones (pe's clusters are marked with their
+    // panel position), the missing ones are added in panel order
+    if (pos_.size() < cl_.size()) pos_.resize(cl_.size(), -1);
+    for (size_t q = 0; q < p.pe.size(); ++q) pos_[ed_[p.pe[q]].n2] = (int)q;
     for (int n : cl_[c].rsp) {
       if (cl_[n].elim || n == c) continue;
-      std::vector<int> row;
-      for (int e : p.pe) {
-        const int kc = ed_[e].n2;
-        int f = find_out(n, kc);
-        if (f < 0) f = add_edge(n, kc, &zero);
-        row.push_back(f);
-        p.touched.push_back(f);
+      std::vector<int> row(p.pe.size(), -1);
+      {
+        const Clu& cn = cl_[n];
+        for (size_t q = 0; q < cn.outn.size(); ++q) {
+          const int kc = cn.outn[q];
+          if (kc < (int)pos_.size() && pos_[kc] >= 0 && row[pos_[kc]] < 0) row[pos_[kc]] = cn.out[q];
+        }
+      }
+      for (size_t q = 0; q < p.pe.size(); ++q) {
+        if (row[q] < 0) row[q] = add_edge(n, ed_[p.pe[q]].n2, &zero);
+        p.touched.push_back(row[q]);
       }
       p.tg.push_back(n);
       p.te.push_back(row);
       p.Nw += cl_[n].csize;
       nnzR_ += (unsigned long long)(cl_[c].csize * cl_[n].csize);
     }
+    for (int e : p.pe) pos_[ed_[e].n2] = -1;
     std::sort(p.touched.begin(), p.touched.end());
     nnzH_ += (unsigned long long)(p.m * p.k);
     const int kk = std::min(p.m, p.k);
@@ -1244,8 +1279,8 @@ void Engine<T>::sparsify_level(const std::vector<int>& cs) {
       const int jj = n >= 0 ? slot[n] : -1;
       if (jj >= 0 && jj < (int)i) w = std::max(w, wave[jj] + 1);
     };
-    for (int e : cl_[S[i]].out) if (ed_[e].n2 != S[i]) see(ed_[e].n2);
-    for (int e : cl_[S[i]].in) if (ed_[e].n1 != S[i]) see(ed_[e].n1);
+    for (int n : cl_[S[i]].outn) if (n != S[i]) see(n);
+    for (int n : cl_[S[i]].inn) if (n != S[i]) see(n);
     wave[i] = w;
     nw = std::max(nw, w + 1);
   }
@@ -1275,10 +1310,13 @@ void Engine<T>::sparsify_level(const std::vector<int>& cs) {
       Job jb;
       jb.c = c; jb.k = cl_[c].csize; jb.smp = 0;
       int spm = 0;
-      for (int e : cl_[c].in)
-        if (!cl_[ed_[e].n1].elim) { jb.apm.push_back(e); spm += ed_[e].a.cols; }
-      for (int e : cl_[c].out)
-        if (!cl_[ed_[e].n2].elim && ed_[e].n2 != c) { jb.amp.push_back(e); jb.smp += ed_[e].a.rows; }
+      {
+        const Clu& k = cl_[c];
+        for (size_t q = 0; q < k.in.size(); ++q)
+          if (!cl_[k.inn[q]].elim) { jb.apm.push_back(k.in[q]); spm += ed_[k.in[q]].a.cols; }
+        for (size_t q = 0; q < k.out.size(); ++q)
+          if (!cl_[k.outn[q]].elim && k.outn[q] != c) { jb.amp.push_back(k.out[q]); jb.smp += ed_[k.out[q]].a.rows; }
+      }
       jb.N = jb.smp + spm;
       const int N = std::max(jb.N, 1);
       jb.Ct = alloc_scr((size_t)N * jb.k);
@@ -1494,8 +1532,8 @@ void Engine<T>::sparsify0_level(const std::vector<int>& cs) {
       const int jj = n >= 0 && n < (int)slot.size() ? slot[n] : -1;
       if (jj >= 0 && jj < (int)i) w = std::max(w, wave[jj] + 1);
     };
-    for (int e : cl_[S[i]].out) if (ed_[e].n2 != S[i]) see(ed_[e].n2);
-    for (int e : cl_[S[i]].in) if (ed_[e].n1 != S[i]) see(ed_[e].n1);
+    for (int n : cl_[S[i]].outn) if (n != S[i]) see(n);
+    for (int n : cl_[S[i]].inn) if (n != S[i]) see(n);
     wave[i] = w;
     nw = std::max(nw, w + 1);
   }
@@ -2433,9 +2471,14 @@ void Engine<T>::build_solve_program() {
         if (!reg && mmax1 > refl_lds_cap(sizeof(T))) s.work = alloc_perm((size_t)s.nwg_total * mmax1);
         h.refl = ds;
       } else if (h.type == 1) {
+        // latency form (one workgroup per op) for stages of few ops without
+        // in-kernel R12 products
+        bool lat = reflr_ && (int)h.bwd.size() < num_cus_;
+        for (auto& d : h.bwd) lat = lat && d.nblk == 0 && d.k <= bwdr_max_k();
+        s.form = lat ? 1 : 0;
         std::vector<int> want;
-        for (auto& d : h.bwd) want.push_back(d.k >= 512 ? std::min(32, (d.k / 32) / 4) : 1);
-        plan_nwg(want);
+        for (auto& d : h.bwd) want.push_back(!lat && d.k >= 512 ? std::min(32, (d.k / 32) / 4) : 1);
+        if (!lat) plan_nwg(want);
         std::vector<int> order(h.bwd.size());
         std::iota(order.begin(), order.end(), 0);
         std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return (want[a] > 1) > (want[b] > 1); });
@@ -2456,6 +2499,11 @@ void Engine<T>::build_solve_program() {
       } else if (h.type == 4 || h.type == 5) {
         // one workgroup per FP64 op
         s.nd = h.type == 4 ? (int)h.refl64.size() : (int)h.bwd64.size();
+        if (h.type == 5) {
+          bool lat = reflr_;
+          for (auto& d : h.bwd64) { lat = lat && d.nblk == 0 && d.k <= bwdr_max_k(); s.kmax = std::max(s.kmax, d.k); }
+          s.form = lat ? 1 : 0;
+        }
         cnt.assign(s.nd, 1);
         s.nwg_total = prefix_tiles(cnt, st);
         if (h.type == 4 && h.mmax > refl_lds_cap(sizeof(double)))
@@ -2575,10 +2623,12 @@ void Engine<T>::run_program(const double* b, double* x, bool prof, int mode) {
     T* xb = s.buf ? xtbuf_ : xbuf_;
     if (s.type == 0 && s.form == 1) launch_reflr<T>(stream_, s.refl, s.wstart, s.nd, s.nwg_total, segs_dev_, xb, s.nseg_max);
     else if (s.type == 0) launch_refl<T>(stream_, s.refl, s.wstart, s.nd, s.nwg_total, segs_dev_, xb, s.work, s.mmax);
+    else if (s.type == 1 && s.form == 1) launch_bwdr<T>(stream_, s.bwd, s.nd, s.kmax, xb);
     else if (s.type == 1) launch_bwd<T>(stream_, s.bwd, s.wstart, s.nd, s.nwg_total, r12_dev_, xb);
     else if (s.type == 3) launch_r12<T>(stream_, s.r12c, s.nd, xb);
     else if (s.type == 4)
       launch_refl<double>(stream_, s.refl64, s.wstart, s.nd, s.nwg_total, segs64_dev_, x64_, s.work64, s.mmax);
+    else if (s.type == 5 && s.form == 1) launch_bwdr<double>(stream_, s.bwd64, s.nd, s.kmax, x64_);
     else if (s.type == 5) launch_bwd<double>(stream_, s.bwd64, s.wstart, s.nd, s.nwg_total, nullptr, x64_);
     else if (s.type == 6) launch_copy(stream_, (int)sizeof(T), 8, s.cvt, s.wstart, s.nd, s.nwg_total);
     else if (s.type == 7) launch_copy(stream_, 8, (int)sizeof(T), s.cvt, s.wstart, s.nd, s.nwg_total);

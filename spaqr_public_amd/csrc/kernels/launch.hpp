@@ -69,6 +69,11 @@ size_t reflr_lds(int nseg_max);
 template <typename T>
 void launch_reflr(hipStream_t s, const ReflDesc<T>* d, const int* wstart, int nd, int nwg_total, const Seg* segs, T* xbuf,
                   int nseg_max);
+// latency form of the triangular sweep: one workgroup per op, no in-kernel
+// R12 products, k <= bwdr_max_k()
+int bwdr_max_k();
+template <typename T>
+void launch_bwdr(hipStream_t s, const BwdDesc<T>* d, int nd, int kmax, T* xbuf);
 int refl_lds_cap(int elem);
 template <typename T>
 void launch_refl(hipStream_t s, const ReflDesc<T>* d, const int* wstart, int nd, int nwg_total, const Seg* segs,

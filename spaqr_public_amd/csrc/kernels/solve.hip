@@ -175,7 +175,7 @@ __global__ __launch_bounds__(kSThreads) void k_refl(const ReflDesc<T>* __restric
 // while the block's sums are combined.  A leader wave owning no rows sums
 // the row waves' partials, exchanges them between the workgroups of a
 // split op (agent-scope stores and one relaxed barrier per block) and forms
-// z = T_b^T y (or T_b y) from the T_b column (row) it holds in registers.
+// z = T_b^T y (or T_b y) from T_b, which it stages into LDS one block ahead.
 constexpr int kRRows = 448;  // 7 row waves + the leader: 2 waves per SIMD, 256 VGPRs each
 constexpr int kRRowWaves = kRRows / kWave;
 constexpr int kRThreads = kRRows + kWave;
@@ -631,6 +631,122 @@ __global__ __launch_bounds__(kSThreads) void k_bwd(const BwdDesc<T>* __restrict_
   }
 }
 
+// ---------------------------------------------------------------------------
+// Latency form of the triangular sweep (ops without in-kernel R12 products,
+// one workgroup each, k <= bwdr_max_k()): x_c stays in LDS; per 32-column
+// block the first wave solves the diagonal block from registers (its row of
+// R_bb, or column for R^T, loaded one block ahead; reciprocals of the
+// diagonal formed in parallel; x_i broadcast by readlane), while the other
+// waves load their R entries of the block's off-diagonal update, which they
+// apply once x_b is known.
+constexpr int kBThreads = 512;
+constexpr int kBRows = kBThreads - kWave;  // update rows per pass (waves 1..7)
+constexpr int kBMaxK = 8192;
+
+int bwdr_max_k() { return kBMaxK; }
+
+template <typename T>
+__global__ __launch_bounds__(kBThreads) void k_bwdr(const BwdDesc<T>* __restrict__ descs, T* __restrict__ xbuf) {
+  extern __shared__ unsigned char smem_raw[];
+  T* xs = reinterpret_cast<T*>(smem_raw);  // k
+  const BwdDesc<T> d = descs[blockIdx.x];
+  const int k = d.k, tid = threadIdx.x, lane = tid & (kWave - 1), wv = tid / kWave;
+  const int nblk = (k + kRNB - 1) / kRNB;
+  const int64_t ldr = d.ldr;
+  const T* __restrict__ R = d.R;
+  T* __restrict__ xg = xbuf + d.xoff;
+  for (int r = tid; r < k; r += kBThreads) xs[r] = xg[r];
+  const bool tr = d.trans != 0;
+  // wave 0, lane l < 32: entries (l, c) of the block (no trans: row l of
+  // R_bb; trans: column l, i.e. R[jb + c, jb + l]); entries outside the
+  // triangle or past k read an in-bounds element and are never used
+  auto load_diag = [&](int bi, T (&rr)[kRNB]) {
+    const int jb = bi * kRNB, w = min(kRNB, k - jb);
+    const int l = min(lane & (kRNB - 1), w - 1);
+#pragma unroll
+    for (int c = 0; c < kRNB; ++c) {
+      const int cc = min(c, w - 1);
+      rr[c] = tr ? R[(jb + cc) + (int64_t)(jb + l) * ldr] : R[(jb + l) + (int64_t)(jb + cc) * ldr];
+    }
+  };
+  // update thread (waves 1..7) q-th row of the pass starting at row0:
+  // entries R[r, jb + c] (no trans) or R[jb + c, r] (trans)
+  auto load_upd = [&](int bi, int r, T (&ru)[kRNB]) {
+    const int jb = bi * kRNB, w = min(kRNB, k - jb);
+    const int rc = min(max(r, 0), k - 1);
+#pragma unroll
+    for (int c = 0; c < kRNB; ++c) {
+      const int cc = min(c, w - 1);
+      ru[c] = tr ? R[(jb + cc) + (int64_t)rc * ldr] : R[rc + (int64_t)(jb + cc) * ldr];
+    }
+  };
+  T rr[kRNB];
+  if (wv == 0 && nblk > 0) load_diag(tr ? 0 : nblk - 1, rr);
+  __syncthreads();
+  for (int t = 0; t < nblk; ++t) {
+    const int bi = tr ? t : nblk - 1 - t;
+    const int jb = bi * kRNB, w = min(kRNB, k - jb);
+    // rows updated by this block: [0, jb) (no trans) or [jb + w, k) (trans)
+    const int ulo = tr ? jb + w : 0, uhi = tr ? k : jb;
+    T ru[kRNB];
+    const int r1st = ulo + (tid - kWave);  // first pass row of an update thread
+    if (wv > 0 && r1st < uhi) load_upd(bi, r1st, ru);
+    if (wv == 0) {
+      // diagonal solve: lane l (< w) holds x_{jb+l}
+      T xl = lane < w ? xs[jb + lane] : T(0);
+      // 1 / R[jb+l, jb+l]: entry c = l of the lane's row (column)
+      T diag = rr[0];
+#pragma unroll
+      for (int c = 1; c < kRNB; ++c) diag = (c == (lane & (kRNB - 1))) ? rr[c] : diag;
+      const T dinv = T(1) / diag;
+      if (!tr) {
+#pragma unroll
+        for (int i = kRNB - 1; i >= 0; --i) {
+          if (i < w) {
+            if (lane == i) xl *= dinv;
+            const T xi = lane_bcast(xl, i);
+            if (lane < i) xl -= rr[i] * xi;
+          }
+        }
+      } else {
+#pragma unroll
+        for (int i = 0; i < kRNB; ++i) {
+          if (i < w) {
+            if (lane == i) xl *= dinv;
+            const T xi = lane_bcast(xl, i);
+            if (lane > i && lane < w) xl -= rr[i] * xi;
+          }
+        }
+      }
+      if (lane < w) xs[jb + lane] = xl;
+      // the next block's diagonal, in flight across the update
+      if (t + 1 < nblk) load_diag(tr ? t + 1 : nblk - 2 - t, rr);
+    }
+    __syncthreads();
+    if (wv > 0) {
+      for (int r0 = ulo; r0 < uhi; r0 += kBRows) {
+        const int r = r0 + (tid - kWave);
+        if (r0 > ulo && r < uhi) load_upd(bi, r, ru);
+        if (r < uhi) {
+          T s = T(0);
+#pragma unroll
+          for (int c = 0; c < kRNB; ++c)
+            if (c < w) s += ru[c] * xs[jb + c];
+          xs[r] -= s;
+        }
+      }
+    }
+    __syncthreads();
+  }
+  for (int r = tid; r < k; r += kBThreads) xg[r] = xs[r];
+}
+
+template <typename T>
+void launch_bwdr(hipStream_t s, const BwdDesc<T>* d, int nd, int kmax, T* xbuf) {
+  if (nd <= 0) return;
+  hipLaunchKernelGGL((k_bwdr<T>), dim3(nd), dim3(kBThreads), sizeof(T) * (size_t)std::max(kmax, 1), s, d, xbuf);
+}
+
 // R12 products of a solve stage as one batched GEMV over column chunks
 // (one workgroup per chunk), accumulated with atomics: the products of a
 // stage are independent of its triangular sweeps and stream far more data.
@@ -720,6 +836,7 @@ void launch_scatter_out(hipStream_t s, const int64_t* src, const int* dst, int n
   template void launch_refl<T>(hipStream_t, const ReflDesc<T>*, const int*, int, int, const Seg*, T*, T*, int); \
   template void launch_reflr<T>(hipStream_t, const ReflDesc<T>*, const int*, int, int, const Seg*, T*, int);     \
   template void launch_bwd<T>(hipStream_t, const BwdDesc<T>*, const int*, int, int, const R12Desc<T>*, T*);     \
+  template void launch_bwdr<T>(hipStream_t, const BwdDesc<T>*, int, int, T*);                                 \
   template void launch_move<T>(hipStream_t, const MoveDesc*, int, T*, const T*);                               \
   template void launch_r12<T>(hipStream_t, const R12Chunk<T>*, int, T*);                                        \
   template void launch_gather_in<T>(hipStream_t, const int64_t*, const int*, int, const double*, T*);          \

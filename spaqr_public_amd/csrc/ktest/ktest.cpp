@@ -311,7 +311,8 @@ int ktest_refl(int m, int k, const double* V, const double* Tb, double* x, int t
 }
 
 // xs <- R^{-1}(xs - A1 x1) (trans=0) ; or xs <- R^{-T} xs, x1 -= A1^T xs (trans=1)
-int ktest_bwd(int k, const double* R, int s, const double* A1, double* xs, double* x1, int trans) {
+// form 1: the latency form (one workgroup, no R12 products: s must be 0)
+int ktest_bwd(int k, const double* R, int s, const double* A1, double* xs, double* x1, int trans, int form) {
   double *dR = dup(R, (size_t)k * k), *dA = dup(A1, (size_t)k * (s > 0 ? s : 1));
   std::vector<double> xb((size_t)k + s);
   std::memcpy(xb.data(), xs, sizeof(double) * k);
@@ -326,7 +327,14 @@ int ktest_bwd(int k, const double* R, int s, const double* A1, double* xs, doubl
   BwdDesc<double>* dd = dup(&d, 1);
   int zero = 0;
   int* ws = dup(&zero, 1);
-  launch_bwd<double>(0, dd, ws, 1, nwg, dq, dx);
+  if (form) {
+    BwdDesc<double> d1 = d;
+    d1.nwg = 1;
+    KT(hipMemcpy(dd, &d1, sizeof(d1), hipMemcpyHostToDevice));
+    launch_bwdr<double>(0, dd, 1, k, dx);
+  } else {
+    launch_bwd<double>(0, dd, ws, 1, nwg, dq, dx);
+  }
   KT(hipDeviceSynchronize());
   back(xb.data(), dx, xb.size());
   std::memcpy(xs, xb.data(), sizeof(double) * k);

@@ -34,7 +34,7 @@ def K(ktest):
     ktest.ktest_larft.argtypes = [C.c_int, C.c_int, dp, dp, dp]
     ktest.ktest_qp3.argtypes = [C.c_int, C.c_int, dp, dp, dp, C.c_double, C.c_int, C.c_int]
     ktest.ktest_refl.argtypes = [C.c_int, C.c_int, dp, dp, dp, C.c_int, C.c_int, C.c_int, C.c_int]
-    ktest.ktest_bwd.argtypes = [C.c_int, dp, C.c_int, dp, dp, dp, C.c_int]
+    ktest.ktest_bwd.argtypes = [C.c_int, dp, C.c_int, dp, dp, dp, C.c_int, C.c_int]
     ktest.ktest_trsm.argtypes = [C.c_int, C.c_int, dp, dp]
     ktest.ktest_copy.argtypes = [C.c_int, C.c_int, dp, C.c_int, dp, C.c_int, C.c_int]
     return ktest
@@ -278,9 +278,12 @@ def test_refl(K, oracle, mk, nseg, trans, form):
 
 
 @pytest.mark.parametrize("trans", [0, 1])
-# k >= 512 runs the multi-workgroup form (4 workgroups over whole 32-row blocks)
-@pytest.mark.parametrize("ks", [(1, 0), (50, 30), (300, 10), (700, 40), (1000, 0)])
-def test_bwd(K, ks, trans):
+# form 0: k >= 512 runs the multi-workgroup form (4 workgroups over whole
+# 32-row blocks); form 1 (latency form) has no R12 products: s = 0 only
+@pytest.mark.parametrize("ks,form", [((1, 0), 0), ((50, 30), 0), ((300, 10), 0), ((700, 40), 0), ((1000, 0), 0),
+                                     ((1, 0), 1), ((31, 0), 1), ((50, 0), 1), ((300, 0), 1), ((733, 0), 1),
+                                     ((1000, 0), 1), ((2100, 0), 1)])
+def test_bwd(K, ks, trans, form):
     k, s = ks
     rng = np.random.default_rng(k)
     R = np.asfortranarray(np.triu(rng.standard_normal((k, k))) / np.sqrt(k) + 4 * np.eye(k))
@@ -288,7 +291,7 @@ def test_bwd(K, ks, trans):
     xs = rng.standard_normal(k)
     x1 = rng.standard_normal(max(s, 1))
     xs2, x12 = xs.copy(), x1.copy()
-    assert K.ktest_bwd(k, d(R), s, d(A1), d(xs2), d(x12), trans) == 0
+    assert K.ktest_bwd(k, d(R), s, d(A1), d(xs2), d(x12), trans, form) == 0
     if trans == 0:
         ref = np.linalg.solve(R, xs - (A1[:, :s] @ x1[:s] if s else 0))
         assert rel(xs2, ref) < 1e-12
