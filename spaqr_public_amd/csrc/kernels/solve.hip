@@ -269,6 +269,7 @@ __global__ __launch_bounds__(kRThreads) void k_reflr(const ReflDesc<T>* __restri
   extern __shared__ unsigned char smem_raw[];
   __shared__ T red[kRNB * kRRowWaves];
   __shared__ T tsh[kRNB * kTLd];
+  __shared__ T ysh[kRNB];
   __shared__ T zsh[kRNB];
   const int p = owner_by_ballot(wstart, nd, blockIdx.x);
   const ReflDesc<T> d = descs[p];
@@ -412,9 +413,6 @@ __global__ __launch_bounds__(kRThreads) void k_reflr(const ReflDesc<T>* __restri
     }
     __syncthreads();
     if (leader) {
-      // T_{t+1} is loaded while this block's sums are combined (after the
-      // partials' store in a split op, whose wait would include it)
-      if (!multi && t + 1 < nblk) load_t(t + 1, tv);
       const int c = lane;
       T y = T(0);
       if (c < kRNB) {
@@ -425,7 +423,6 @@ __global__ __launch_bounds__(kRThreads) void k_reflr(const ReflDesc<T>* __restri
         const gmut<T> pb = to_global_mut(part) + (int64_t)(t & 1) * d.nwg * kRNB + c;
         if (c < kRNB) __hip_atomic_store(pb + g * kRNB, y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __builtin_amdgcn_s_waitcnt(0);  // this wave's partials have landed
-        if (t + 1 < nblk) load_t(t + 1, tv);
         target += d.nwg;
         if (lane == 0) barrier_arrive_wait_g(d.sync, target, d.err);
         wave_sync();
@@ -441,35 +438,48 @@ __global__ __launch_bounds__(kRThreads) void k_reflr(const ReflDesc<T>* __restri
           for (int u = 0; u < 8; ++u) y += (q0 + u < d.nwg) ? pv[u] : T(0);
         }
       }
-      // z = T_b^T y (trans) or T_b y: y_l broadcast from lane l, T_b from
-      // LDS (conflict-free through the padded leading dimension); entries
-      // outside the triangle are never combined
-      // lane c combines entries l in [lo, hi] of column c of T_b (trans)
-      // or of row c (no trans)
-      const int cc = c & (kRNB - 1);
+      // z = T_b^T y (trans) or T_b y.  y goes through LDS; lane c (and its
+      // partner c + 32) combines entries l of column c of T_b (trans) or of
+      // row c (no trans), the lower lanes l < 16 and the upper ones l >= 16,
+      // with all 16 LDS reads of T_b (conflict-free through the padded
+      // leading dimension) and of y issued before the FMAs; entries outside
+      // the triangle [lo, hi] are never combined
+      if (c < kRNB) ysh[c] = y;
+      wave_sync();
+      const int cc = c & (kRNB - 1), half = (c >> 5) * (kRNB / 2);
       const int tb0 = d.trans ? cc * kTLd : cc, tstr = d.trans ? 1 : kTLd;
       const int lo = d.trans ? 0 : cc, hi = d.trans ? cc : w - 1;
-      T z = T(0);
-#pragma nounroll
-      for (int l0 = 0; l0 < kRNB; l0 += 8) {
+      T tl[kRNB / 2], yl[kRNB / 2];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) {
-          const int l = l0 + u;
-          const T yl = lane_bcast(y, l);
-          const T tl = tsh[tb0 + l * tstr];
-          z = (l >= lo && l <= hi) ? fma(tl, yl, z) : z;
-        }
+      for (int u = 0; u < kRNB / 2; ++u) {
+        tl[u] = tsh[tb0 + (half + u) * tstr];
+        yl[u] = ysh[half + u];
       }
+      T z0 = T(0), z1 = T(0);
+#pragma unroll
+      for (int u = 0; u < kRNB / 2; u += 2) {
+        const int l = half + u;
+        z0 = (l >= lo && l <= hi) ? fma(tl[u], yl[u], z0) : z0;
+        z1 = (l + 1 >= lo && l + 1 <= hi) ? fma(tl[u + 1], yl[u + 1], z1) : z1;
+      }
+      T z = z0 + z1;
+      z += __shfl_xor(z, kRNB, kWave);
       if (c < kRNB) zsh[c] = (c < w) ? z : T(0);
     }
     __syncthreads();
     if (cls != 0) {
-      T s = T(0);
+      T s4[4] = {T(0), T(0), T(0), T(0)};
 #pragma unroll
-      for (int c = 0; c < kRNB; ++c) s += cur[c] * zsh[c];
-      xr -= s;
+      for (int c = 0; c < kRNB; ++c) s4[c & 3] += cur[c] * zsh[c];
+      xr -= (s4[0] + s4[1]) + (s4[2] + s4[3]);
     }
-    if (leader && t + 1 < nblk) store_t(tv);  // read again only after the next block's first barrier
+    // T_{t+1}, loaded while the row waves update and form the next sums;
+    // read again only after the next block's first barrier
+    if (leader && t + 1 < nblk) {
+      T tn[kTQ];
+      load_t(t + 1, tn);
+      store_t(tn);
+    }
   };
   // two register sets that swap roles block by block (no copies)
   for (int t = 0; t < nblk; t += 2) {
