@@ -588,6 +588,7 @@ class Engine final : public EngineBase {
     int nd = 0, mmax = 0, nwg_total = 0;
     int kmax = 0, mall = 0;  // largest k and m of the stage's ops (diagnostics)
     int form = 0, nseg_max = 0;  // 1: register form (k_reflr) / latency form (k_bwdr)
+    int nrw = 7;                 // register form: row waves per workgroup
     int buf = 0, sbuf = 0;  // x buffer used (0 = x, 1 = xt); moves copy sbuf -> buf
     const ReflDesc<T>* refl = nullptr;
     const BwdDesc<T>* bwd = nullptr;
@@ -2433,12 +2434,17 @@ void Engine<T>::build_solve_program() {
       if (h.type == 0) {
         // register form for stages of few ops (one op's latency sets the
         // pace): every op gets ceil(m / rows) workgroups, all resident
+        // stages of many small ops (every op within one workgroup of the
+        // smallest register form that holds it) use the register form too
         std::vector<int> want;
-        int reg_total = 0;
-        for (auto& d : h.refl) reg_total += iceil(std::max(d.m, 1), reflr_rows());
-        const bool reg = reflr_ && (int)h.refl.size() < num_cus_ && reg_total <= num_cus_;
+        int reg_total = 0, mstage = 1;
+        for (auto& d : h.refl) { reg_total += iceil(std::max(d.m, 1), reflr_rows()); mstage = std::max(mstage, d.m); }
+        const bool few = (int)h.refl.size() < num_cus_ && reg_total <= num_cus_;
+        const bool many = !few && mstage <= reflr_rows();
+        const bool reg = reflr_ && (few || many);
+        s.nrw = few ? 7 : reflr_row_waves(mstage);
         if (reg) {
-          for (auto& d : h.refl) want.push_back(iceil(std::max(d.m, 1), reflr_rows()));
+          for (auto& d : h.refl) want.push_back(iceil(std::max(d.m, 1), 64 * s.nrw));
         } else {
           for (auto& d : h.refl) want.push_back(d.m >= 384 ? std::min(32, iceil(d.m, 256)) : 1);
           plan_nwg(want);
@@ -2621,7 +2627,8 @@ void Engine<T>::run_program(const double* b, double* x, bool prof, int mode) {
       SPQ_HIP(hipEventRecord(ev.back(), stream_));
     }
     T* xb = s.buf ? xtbuf_ : xbuf_;
-    if (s.type == 0 && s.form == 1) launch_reflr<T>(stream_, s.refl, s.wstart, s.nd, s.nwg_total, segs_dev_, xb, s.nseg_max);
+    if (s.type == 0 && s.form == 1)
+      launch_reflr<T>(stream_, s.refl, s.wstart, s.nd, s.nwg_total, segs_dev_, xb, s.nseg_max, s.nrw);
     else if (s.type == 0) launch_refl<T>(stream_, s.refl, s.wstart, s.nd, s.nwg_total, segs_dev_, xb, s.work, s.mmax);
     else if (s.type == 1 && s.form == 1) launch_bwdr<T>(stream_, s.bwd, s.nd, s.kmax, xb);
     else if (s.type == 1) launch_bwd<T>(stream_, s.bwd, s.wstart, s.nd, s.nwg_total, r12_dev_, xb);
@@ -2662,8 +2669,8 @@ void Engine<T>::run_program(const double* b, double* x, bool prof, int mode) {
       size_t j = i, pi = 0;
       while (j >= progs[pi]->size()) j -= progs[pi++]->size();
       const Stage& s = (*progs[pi])[j];
-      fprintf(stderr, "stage p%zu %3zu type %d form %d ops %5d wgs %5d mmax %6d mall %6d kmax %5d ms %.4f\n", pi, j,
-              s.type, s.form, s.nd, s.type == 2 ? s.nd : s.nwg_total, s.mmax, s.mall, s.kmax, ms);
+      fprintf(stderr, "stage p%zu %3zu type %d form %d ops %5d wgs %5d mmax %6d mall %6d kmax %5d nrw %d ms %.4f\n", pi, j,
+              s.type, s.form, s.nd, s.type == 2 ? s.nd : s.nwg_total, s.mmax, s.mall, s.kmax, s.nrw, ms);
     }
     fprintf(stderr, "stages total ms %.3f\n", tot);
     for (auto e : ev) (void)hipEventDestroy(e);

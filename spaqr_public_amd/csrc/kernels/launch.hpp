@@ -63,12 +63,15 @@ template <typename T>
 int launch_qp3(hipStream_t s, const Qp3Desc<T>* d, const int* wstart, int nd, int nwg_total, int kmax, int coop);
 
 // solve.hip
-// register form of the reflector apply: <= reflr_rows() rows per workgroup
+// register form of the reflector apply: nrw row waves per workgroup (1, 2
+// or 7), so <= 64 nrw rows per workgroup; reflr_rows() = 448 (nrw = 7) and
+// reflr_row_waves(m) the smallest form for ops of at most m rows
 int reflr_rows();
+int reflr_row_waves(int mmax);
 size_t reflr_lds(int nseg_max);
 template <typename T>
 void launch_reflr(hipStream_t s, const ReflDesc<T>* d, const int* wstart, int nd, int nwg_total, const Seg* segs, T* xbuf,
-                  int nseg_max);
+                  int nseg_max, int nrw);
 // latency form of the triangular sweep: one workgroup per op, no in-kernel
 // R12 products, k <= bwdr_max_k()
 int bwdr_max_k();

@@ -169,18 +169,18 @@ __global__ __launch_bounds__(kSThreads) void k_refl(const ReflDesc<T>* __restric
 // Register form of the reflector apply, for stages whose ops are few (the
 // dependency waves of the QR forward sweep and the large ops of the upper
 // levels), where one op's time, not the stage's throughput, sets the pace.
-// Each row thread owns one row of the op's slice (<= kRRows rows per
+// Each row thread owns one row of the op's slice (<= 64 NRW rows per
 // workgroup): its x entry and its 32 entries of the current reflector block
 // stay in registers, so V is read once, and the next block's row is loaded
 // while the block's sums are combined.  A leader wave owning no rows sums
 // the row waves' partials, exchanges them between the workgroups of a
 // split op (agent-scope stores and one relaxed barrier per block) and forms
 // z = T_b^T y (or T_b y) from T_b, which it stages into LDS one block ahead.
-constexpr int kRRows = 448;  // 7 row waves + the leader: 2 waves per SIMD, 256 VGPRs each
-constexpr int kRRowWaves = kRRows / kWave;
-constexpr int kRThreads = kRRows + kWave;
-
-int reflr_rows() { return kRRows; }
+// NRW row waves + the leader per workgroup: 7 (448 rows; 8 waves, 2 per
+// SIMD, 256 VGPRs each) for stages of few ops, 1 or 2 (64 / 128 rows,
+// several workgroups per CU) for stages of many small ops.
+int reflr_rows() { return 7 * kWave; }
+int reflr_row_waves(int mmax) { return mmax <= kWave ? 1 : (mmax <= 2 * kWave ? 2 : 7); }
 size_t reflr_lds(int nseg_max) {
   return (((size_t)nseg_max + 1) * sizeof(int) + 7) / 8 * 8 + (size_t)std::max(nseg_max, 1) * sizeof(int64_t);
 }
@@ -263,9 +263,11 @@ __device__ __forceinline__ int owner_by_ballot(const int* __restrict__ start, in
   return cnt - 1;
 }
 
-template <typename T>
-__global__ __launch_bounds__(kRThreads) void k_reflr(const ReflDesc<T>* __restrict__ descs, const int* __restrict__ wstart,
-                                                     int nd, const Seg* __restrict__ segs, T* __restrict__ xbuf) {
+template <typename T, int NRW>
+__global__ __launch_bounds__((NRW + 1) * kWave) void k_reflr(const ReflDesc<T>* __restrict__ descs,
+                                                             const int* __restrict__ wstart, int nd,
+                                                             const Seg* __restrict__ segs, T* __restrict__ xbuf) {
+  constexpr int kRRowWaves = NRW;
   extern __shared__ unsigned char smem_raw[];
   __shared__ T red[kRNB * kRRowWaves];
   __shared__ T tsh[kRNB * kTLd];
@@ -491,9 +493,15 @@ __global__ __launch_bounds__(kRThreads) void k_reflr(const ReflDesc<T>* __restri
 
 template <typename T>
 void launch_reflr(hipStream_t s, const ReflDesc<T>* d, const int* wstart, int nd, int nwg_total, const Seg* segs, T* xbuf,
-                  int nseg_max) {
+                  int nseg_max, int nrw) {
   if (nd <= 0) return;
-  hipLaunchKernelGGL((k_reflr<T>), dim3(nwg_total), dim3(kRThreads), reflr_lds(nseg_max), s, d, wstart, nd, segs, xbuf);
+  const size_t lds = reflr_lds(nseg_max);
+  if (nrw == 1)
+    hipLaunchKernelGGL((k_reflr<T, 1>), dim3(nwg_total), dim3(2 * kWave), lds, s, d, wstart, nd, segs, xbuf);
+  else if (nrw == 2)
+    hipLaunchKernelGGL((k_reflr<T, 2>), dim3(nwg_total), dim3(3 * kWave), lds, s, d, wstart, nd, segs, xbuf);
+  else
+    hipLaunchKernelGGL((k_reflr<T, 7>), dim3(nwg_total), dim3(8 * kWave), lds, s, d, wstart, nd, segs, xbuf);
 }
 
 // Solve the w x w diagonal block (staged in LDS, ld kTLd) in place on
@@ -844,7 +852,7 @@ void launch_scatter_out(hipStream_t s, const int64_t* src, const int* dst, int n
 
 #define SPQ_INST(T)                                                                                            \
   template void launch_refl<T>(hipStream_t, const ReflDesc<T>*, const int*, int, int, const Seg*, T*, T*, int); \
-  template void launch_reflr<T>(hipStream_t, const ReflDesc<T>*, const int*, int, int, const Seg*, T*, int);     \
+  template void launch_reflr<T>(hipStream_t, const ReflDesc<T>*, const int*, int, int, const Seg*, T*, int, int); \
   template void launch_bwd<T>(hipStream_t, const BwdDesc<T>*, const int*, int, int, const R12Desc<T>*, T*);     \
   template void launch_bwdr<T>(hipStream_t, const BwdDesc<T>*, int, int, T*);                                 \
   template void launch_move<T>(hipStream_t, const MoveDesc*, int, T*, const T*);                               \

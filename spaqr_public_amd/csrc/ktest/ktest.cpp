@@ -271,7 +271,8 @@ int ktest_qp3_bench(int k, int N, const double* Ct, double tol, int form, int nw
 // x <- H^T x (trans=1) or H x for blocked reflectors (V m x k, Tb), with x
 // split into `nseg` equal-length segments stored at stride `gap` in a buffer.
 // form 0: LDS form (4 row-slice workgroups for m >= 2048); form 1: register
-// form (ceil(m / reflr_rows()) workgroups)
+// form, 7 row waves (ceil(m / 448) workgroups); form 2: register form with
+// the row waves reflr_row_waves(m) gives (1 or 2 for m <= 128, else 7)
 int ktest_refl(int m, int k, const double* V, const double* Tb, double* x, int trans, int nseg, int gap, int form) {
   const int nb = panel_nb(), nblk = (k + nb - 1) / nb;
   const int seglen = m / nseg;
@@ -285,7 +286,8 @@ int ktest_refl(int m, int k, const double* V, const double* Tb, double* x, int t
   }
   double* dx = dup(xb.data(), xlen);
   Seg* dsg = dup(segs.data(), segs.size());
-  const int nwg = form ? (m + reflr_rows() - 1) / reflr_rows() : (m >= 2048 ? 4 : 1);
+  const int nrw = form == 2 ? reflr_row_waves(m) : 7;
+  const int nwg = form ? (m + 64 * nrw - 1) / (64 * nrw) : (m >= 2048 ? 4 : 1);
   double* wk = dup<double>(nullptr, m);
   double* part = dup<double>(nullptr, 2 * 32 * (size_t)nwg);
   unsigned int* sync = dup<unsigned int>(nullptr, 1);
@@ -295,7 +297,7 @@ int ktest_refl(int m, int k, const double* V, const double* Tb, double* x, int t
   double* work = dup<double>(nullptr, m);
   int zero = 0;
   int* ws = dup(&zero, 1);
-  if (form) launch_reflr<double>(0, dd, ws, 1, nwg, dsg, dx, nseg);
+  if (form) launch_reflr<double>(0, dd, ws, 1, nwg, dsg, dx, nseg, nrw);
   else launch_refl<double>(0, dd, ws, 1, nwg, dsg, dx, work, nwg > 1 ? 0 : m);
   KT(hipDeviceSynchronize());
   back(xb.data(), dx, xlen);

@@ -254,13 +254,14 @@ def test_qp3(K, oracle, kN, true_rank, tol, form, nwg):
     assert rel((np.tril(V[:, :rank], -1) + np.eye(k, rank))[:, well], Vref[:, well]) < 1e-9
 
 
-@pytest.mark.parametrize("form", [0, 1])
+@pytest.mark.parametrize("form", [0, 1, 2])
 @pytest.mark.parametrize("trans", [1, 0])
 # form 0: m >= 2048 runs the row-sliced multi-workgroup form (4 workgroups);
-# form 1 (register form): ceil(m / 448) workgroups, so 600 rows use 2 and
-# 5100 rows 12; 97 segments exceed one wave of the segment-table scan
-@pytest.mark.parametrize("mk,nseg", [((40, 40), 1), ((150, 70), 3), ((448, 64), 2), ((600, 100), 4), ((20000, 40), 2),
-                                     ((5100, 300), 3), ((97 * 7, 90), 97)])
+# form 1 (register form, 7 row waves): ceil(m / 448) workgroups, so 600 rows
+# use 2 and 5100 rows 12; form 2: 1 row wave for 40 rows, 2 for 64 < m <= 128,
+# 7 beyond; 97 segments exceed one wave of the segment-table scan
+@pytest.mark.parametrize("mk,nseg", [((40, 40), 1), ((100, 60), 4), ((150, 70), 3), ((448, 64), 2), ((600, 100), 4),
+                                     ((20000, 40), 2), ((5100, 300), 3), ((97 * 7, 90), 97)])
 def test_refl(K, oracle, mk, nseg, trans, form):
     m, k = mk
     assert m % nseg == 0  # the test splits x into nseg equal segments
