@@ -53,9 +53,11 @@ KrylovResult gmres_dev(EngineBase& M, const DevMatrix& A, const double* b, doubl
   SPQ_HIP(hipMalloc((void**)&p0, sizeof(double) * n));
   SPQ_HIP(hipMalloc((void**)&h, sizeof(double) * (restart + 2) * 2));
   SPQ_HIP(hipHostMalloc((void**)&hh, sizeof(double) * (restart + 2) * 2, hipHostMallocDefault));
-  auto cleanup = [&]() { hipFree(V); hipFree(w); hipFree(p0); hipFree(h); hipHostFree(hh); };
+  double* dpart = nullptr;
+  SPQ_HIP(hipMalloc((void**)&dpart, sizeof(double) * dots_part_elems(restart + 1)));
+  auto cleanup = [&]() { hipFree(V); hipFree(w); hipFree(p0); hipFree(h); hipFree(dpart); hipHostFree(hh); };
   auto norm_dev = [&](const double* v) {
-    launch_dots(s, n, 1, v, n, v, h);
+    launch_dots(s, n, 1, v, n, v, h, dpart);
     SPQ_HIP(hipMemcpyAsync(hh, h, sizeof(double), hipMemcpyDeviceToHost, s));
     SPQ_HIP(hipStreamSynchronize(s));
     return std::sqrt(hh[0]);
@@ -95,11 +97,11 @@ KrylovResult gmres_dev(EngineBase& M, const DevMatrix& A, const double* b, doubl
       launch_spmv(s, n, rp, ci, av, vk, w, 1.0, 0.0);
       precond(w, vn);
       // CGS2 against V(:, 0:k)
-      launch_dots(s, n, k, V, n, vn, h);
+      launch_dots(s, n, k, V, n, vn, h, dpart);
       launch_gemv_sub(s, n, k, V, n, h, vn);
-      launch_dots(s, n, k, V, n, vn, h + k);
+      launch_dots(s, n, k, V, n, vn, h + k, dpart);
       launch_gemv_sub(s, n, k, V, n, h + k, vn);
-      launch_dots(s, n, 1, vn, n, vn, h + 2 * k);
+      launch_dots(s, n, 1, vn, n, vn, h + 2 * k, dpart);
       SPQ_HIP(hipMemcpyAsync(hh, h, sizeof(double) * (2 * k + 1), hipMemcpyDeviceToHost, s));
       SPQ_HIP(hipStreamSynchronize(s));
       double* col = &H[(size_t)(k - 1) * (restart + 1)];
@@ -155,13 +157,15 @@ KrylovResult gmres_ir_dev(EngineBase& M, const DevMatrix& A, const double* b, do
   SPQ_HIP(hipMalloc((void**)&d, sizeof(double) * std::max(n, 1)));
   SPQ_HIP(hipMalloc((void**)&h, sizeof(double) * 2));
   SPQ_HIP(hipHostMalloc((void**)&hh, sizeof(double) * 2, hipHostMallocDefault));
+  double* dpart = nullptr;
+  SPQ_HIP(hipMalloc((void**)&dpart, sizeof(double) * dots_part_elems(1)));
   struct Free {
     std::vector<void*> dv;
     void* hp;
     ~Free() { for (void* q : dv) hipFree(q); hipHostFree(hp); }
-  } fr{{r, d, h}, hh};
+  } fr{{r, d, h, dpart}, hh};
   auto norm = [&](const double* v) {
-    launch_dots(s, n, 1, v, n, v, h);
+    launch_dots(s, n, 1, v, n, v, h, dpart);
     SPQ_HIP(hipMemcpyAsync(hh, h, sizeof(double), hipMemcpyDeviceToHost, s));
     SPQ_HIP(hipStreamSynchronize(s));
     return std::sqrt(hh[0]);
@@ -223,13 +227,15 @@ KrylovResult cgls_dev(EngineBase& M, const DevMatrix& A, const double* b, double
   SPQ_HIP(hipMalloc((void**)&zold, sizeof(double) * (flexible ? std::max(n, 1) : 1)));
   SPQ_HIP(hipMalloc((void**)&h, sizeof(double) * 4));
   SPQ_HIP(hipHostMalloc((void**)&hh, sizeof(double) * 4, hipHostMallocDefault));
+  double* dpart = nullptr;
+  SPQ_HIP(hipMalloc((void**)&dpart, sizeof(double) * dots_part_elems(1)));
   struct Free {
     std::vector<void*> d;
     void* hp;
     ~Free() { for (void* q : d) hipFree(q); hipHostFree(hp); }
-  } fr{{r, tmp, nr, p, z, zold, h}, hh};
+  } fr{{r, tmp, nr, p, z, zold, h, dpart}, hh};
   auto dot = [&](int len, const double* u, const double* v) {
-    launch_dots(s, len, 1, u, len, v, h);
+    launch_dots(s, len, 1, u, len, v, h, dpart);
     SPQ_HIP(hipMemcpyAsync(hh, h, sizeof(double), hipMemcpyDeviceToHost, s));
     SPQ_HIP(hipStreamSynchronize(s));
     return hh[0];

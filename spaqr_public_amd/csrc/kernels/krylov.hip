@@ -21,15 +21,27 @@ __global__ __launch_bounds__(256) void k_spmv(int n, const int* __restrict__ rp,
 }
 
 // out[j] = V(:, j) . w  for j < ncol (V is n x ncol, ld)
-__global__ __launch_bounds__(256) void k_dots(int n, const double* __restrict__ V, int64_t ld,
-                                              const double* __restrict__ w, double* __restrict__ out) {
+// out[j] = V(:, j) . w for j < ncol, in two passes: blocks (b, j) sum rows
+// [b*chunk, (b+1)*chunk) into part[j*nb + b], then one wave per column adds
+// the nb partials in a fixed butterfly (the result does not depend on
+// scheduling).
+constexpr int kDotParts = 64;
+__global__ __launch_bounds__(256) void k_dots_part(int n, int chunk, int nb, const double* __restrict__ V, int64_t ld,
+                                                   const double* __restrict__ w, double* __restrict__ part) {
   __shared__ double red[4];
-  const int j = blockIdx.x;
+  const int b = blockIdx.x, j = blockIdx.y;
   const double* __restrict__ vj = V + (int64_t)j * ld;
+  const int lo = b * chunk, hi = min(n, lo + chunk);
   double s = 0;
-  for (int i = threadIdx.x; i < n; i += 256) s += vj[i] * w[i];
+  for (int i = lo + threadIdx.x; i < hi; i += 256) s += vj[i] * w[i];
   s = block_sum(s, red);
-  if (threadIdx.x == 0) out[j] = s;
+  if (threadIdx.x == 0) part[(int64_t)j * nb + b] = s;
+}
+__global__ __launch_bounds__(64) void k_dots_fin(int nb, const double* __restrict__ part, double* __restrict__ out) {
+  const int j = blockIdx.x, l = threadIdx.x;
+  double s = l < nb ? part[(int64_t)j * nb + l] : 0.0;
+  s = wave_sum(s);
+  if (l == 0) out[j] = s;
 }
 
 // w -= V(:, 0:ncol) h
@@ -86,9 +98,14 @@ void launch_spmv(hipStream_t s, int n, const int* rp, const int* ci, const doubl
   if (n <= 0) return;
   hipLaunchKernelGGL(k_spmv, dim3((n + 3) / 4), dim3(256), 0, s, n, rp, ci, v, x, y, alpha, beta);
 }
-void launch_dots(hipStream_t s, int n, int ncol, const double* V, int64_t ld, const double* w, double* out) {
+size_t dots_part_elems(int ncol) { return (size_t)std::max(ncol, 1) * kDotParts; }
+void launch_dots(hipStream_t s, int n, int ncol, const double* V, int64_t ld, const double* w, double* out,
+                 double* part) {
   if (ncol <= 0) return;
-  hipLaunchKernelGGL(k_dots, dim3(ncol), dim3(256), 0, s, n, V, ld, w, out);
+  const int nb = std::max(1, std::min(kDotParts, (n + 4095) / 4096));
+  const int chunk = std::max(1, (n + nb - 1) / nb);
+  hipLaunchKernelGGL(k_dots_part, dim3(nb, ncol), dim3(256), 0, s, n, chunk, nb, V, ld, w, part);
+  hipLaunchKernelGGL(k_dots_fin, dim3(ncol), dim3(64), 0, s, nb, part, out);
 }
 void launch_gemv_sub(hipStream_t s, int n, int ncol, const double* V, int64_t ld, const double* h, double* w) {
   if (n <= 0 || ncol <= 0) return;

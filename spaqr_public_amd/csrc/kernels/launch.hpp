@@ -105,7 +105,10 @@ void launch_scale_inv(hipStream_t s, const ScaleInvDesc<T>* d, const int* tstart
 // krylov.hip
 void launch_spmv(hipStream_t s, int n, const int* rp, const int* ci, const double* v, const double* x, double* y,
                  double alpha, double beta);
-void launch_dots(hipStream_t s, int n, int ncol, const double* V, int64_t ld, const double* w, double* out);
+// part: dots_part_elems(ncol) doubles of workspace
+size_t dots_part_elems(int ncol);
+void launch_dots(hipStream_t s, int n, int ncol, const double* V, int64_t ld, const double* w, double* out,
+                 double* part);
 void launch_gemv_sub(hipStream_t s, int n, int ncol, const double* V, int64_t ld, const double* h, double* w);
 void launch_gemv(hipStream_t s, int n, int ncol, const double* V, int64_t ld, const double* h, double* y);
 void launch_axpby(hipStream_t s, int n, double a, const double* x, double b, double* y);
