@@ -182,11 +182,22 @@ __device__ __forceinline__ void group_barrier2_rx(unsigned int* sync, int g, int
 
 // Find the descriptor owning flattened tile `t` given exclusive prefix sums
 // `start[0..n)` (start[0] == 0, nondecreasing).
+// A 64-ary search by the whole wave: each round every lane reads one sample
+// of the current range (one load round trip instead of one per halving), so
+// n <= 4096 takes two rounds.  Must be called by full, converged waves with
+// the same t (every caller passes its block's tile index at kernel entry).
 __device__ __forceinline__ int find_owner(const int* start, int n, int t) {
-  int lo = 0, hi = n - 1;
-  while (lo < hi) {
-    int mid = (lo + hi + 1) >> 1;
-    if (start[mid] <= t) lo = mid; else hi = mid - 1;
+  const int lane = threadIdx.x & (kWave - 1);
+  int lo = 0, hi = n;  // start[lo] <= t, and hi == n or start[hi] > t
+  while (hi - lo > 1) {
+    const int step = (hi - lo + kWave - 1) / kWave;
+    const int idx = lo + lane * step;
+    const bool le = idx < hi && start[idx] <= t;
+    const unsigned long long m = __ballot(le);
+    const int last = kWave - 1 - __clzll((long long)m);  // lane 0 always qualifies
+    const int nlo = lo + last * step;
+    hi = min(hi, nlo + step);
+    lo = nlo;
   }
   return lo;
 }
