@@ -152,6 +152,39 @@ def trace_summary(d, out, cmd, top=14):
         f.write("\n".join(lines) + "\n")
 
 
+def stage_summary(err_path, out, cmd):
+    """Markdown summary of the per-stage solve timing (SPAQR_SOLVE_STAGES=1
+    prints one line per stage of every apply; the last apply is summarised)."""
+    kinds = {0: "reflectors", 1: "triangular sweeps", 2: "moves", 3: "R12 products", 4: "reflectors (FP64 top)",
+             5: "triangular (FP64 top)", 6: "convert to FP64", 7: "convert back"}
+    forms = {(0, 0): "k_refl", (0, 1): "k_reflr", (1, 0): "k_bwd", (1, 1): "k_bwdr", (2, 0): "k_move",
+             (3, 0): "k_r12"}
+    lines = [ln for ln in open(err_path) if ln.startswith("stage ")]
+    starts = [i for i, ln in enumerate(lines) if ln.split()[1] == "p0" and ln.split()[2] == "0"]
+    last = lines[starts[-1]:] if starts else lines
+    groups = {}
+    for ln in last:
+        f = ln.split()
+        typ, form, ops, ms = int(f[4]), int(f[6]), int(f[8]), float(f[-1])
+        key = (typ, form, "< 256 ops" if ops < 256 else ">= 256 ops")
+        g = groups.setdefault(key, [0, 0.0, 0.0])
+        g[0] += 1
+        g[1] += ms
+        g[2] = max(g[2], ms)
+    total = sum(g[1] for g in groups.values())
+    rows = [f"# Solve stages: {os.path.basename(out).rsplit('.', 1)[0]}", "",
+            "Per-stage HIP-event timing of one preconditioner apply (the last of the run), stages launched "
+            "one by one (the timed runs replay them as one graph). Command (from the repository root, on one "
+            "MI355X):", "", "```", cmd, "```", "",
+            f"{len(last)} stages, {total:.2f} ms in total (event overhead included).", "",
+            "| stage kind | kernel | ops per stage | stages | total ms | max ms |", "|---|---|---|---|---|---|"]
+    for (typ, form, sz), g in sorted(groups.items(), key=lambda kv: -kv[1][1]):
+        rows.append(f"| {kinds.get(typ, typ)} | `{forms.get((typ, form), '-')}` | {sz} | {g[0]} | {g[1]:.3f} | "
+                    f"{g[2]:.3f} |")
+    with open(out, "w") as fo:
+        fo.write("\n".join(rows) + "\n")
+
+
 def pmc_summary(fetch_dir, write_dir, cal_fetch_dir, cal_write_dir, out, cmds, copy_bytes=4096 * 4096 * 8):
     """Per-kernel HBM bytes per dispatch from FETCH_SIZE / WRITE_SIZE (KiB),
     each scaled so that a plain copy of copy_bytes reads and writes copy_bytes."""
@@ -294,6 +327,7 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true", help="run the PMC copy calibration and exit")
     ap.add_argument("--trace-summary", nargs=2, metavar=("STATS_DIR", "OUT_MD"))
     ap.add_argument("--pmc-summary", nargs=5, metavar=("FETCH", "WRITE", "CAL_FETCH", "CAL_WRITE", "OUT_JSON"))
+    ap.add_argument("--stage-summary", nargs=2, metavar=("STAGES_ERR", "OUT_MD"))
     ap.add_argument("--cmd", action="append", default=[], help="profiling command(s) recorded in the summary")
     args = ap.parse_args()
     if args.copy_calibration:
@@ -304,6 +338,9 @@ def main():
         return
     if args.pmc_summary:
         pmc_summary(*args.pmc_summary[:4], args.pmc_summary[4], args.cmd)
+        return
+    if args.stage_summary:
+        stage_summary(args.stage_summary[0], args.stage_summary[1], " && ".join(args.cmd))
         return
     if args.gpus != 1:
         print(f"bench.py: this build measures one MI355X; --gpus {args.gpus} is not supported "

@@ -98,7 +98,7 @@ __global__ __launch_bounds__(kSThreads) void k_refl(const ReflDesc<T>* __restric
   __shared__ T tsh[kRNB * kTLd];
   __shared__ T ysh[kRNB];
   __shared__ T zsh[kRNB];
-  const int p = find_owner(wstart, nd, blockIdx.x);
+  const int p = ((int)gridDim.x == nd) ? (int)blockIdx.x : find_owner(wstart, nd, blockIdx.x);
   const ReflDesc<T> d = descs[p];
   const int g = blockIdx.x - wstart[p];
   const int m = d.m, tid = threadIdx.x, lane = tid & (kWave - 1), wv = tid / kWave;
@@ -273,7 +273,8 @@ __global__ __launch_bounds__((NRW + 1) * kWave) void k_reflr(const ReflDesc<T>* 
   __shared__ T tsh[kRNB * kTLd];
   __shared__ T ysh[kRNB];
   __shared__ T zsh[kRNB];
-  const int p = owner_by_ballot(wstart, nd, blockIdx.x);
+  // one workgroup per op (no op split): the descriptor is the block's own
+  const int p = ((int)gridDim.x == nd) ? (int)blockIdx.x : owner_by_ballot(wstart, nd, blockIdx.x);
   const ReflDesc<T> d = descs[p];
   const int g = blockIdx.x - wstart[p];
   const int m = d.m, tid = threadIdx.x, lane = tid & (kWave - 1), wv = tid / kWave;
@@ -539,7 +540,7 @@ __global__ __launch_bounds__(kSThreads) void k_bwd(const BwdDesc<T>* __restrict_
   __shared__ T rsh[kRNB * kTLd];
   __shared__ T xsh[kRNB];
   __shared__ T red[kSThreads];
-  const int p = find_owner(wstart, nd, blockIdx.x);
+  const int p = ((int)gridDim.x == nd) ? (int)blockIdx.x : find_owner(wstart, nd, blockIdx.x);
   const BwdDesc<T> d = descs[p];
   const int g = blockIdx.x - wstart[p], nwg = d.nwg;
   const int k = d.k, tid = threadIdx.x, lane = tid & (kWave - 1), wv = tid / kWave;
