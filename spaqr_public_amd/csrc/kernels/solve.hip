@@ -225,6 +225,84 @@ __device__ __forceinline__ T load_at(const T* base, unsigned byte_off) {
   return *(gptr<T>)((const __attribute__((address_space(1))) char*)to_global(base) + byte_off);
 }
 
+// ---------------------------------------------------------------------------
+// Reduce-scatter of 32 per-lane values over the wave without LDS: the
+// distance-32 and distance-16 steps swap half-waves / rows with
+// v_permlane32_swap / v_permlane16_swap (no selects), the steps within a
+// 16-lane row exchange through DPP (xor 8: row_ror:8; xor 4: row_half_mirror
+// then quad_perm 3210; xor 2, xor 1: quad_perm).  On return lanes 2c and
+// 2c + 1 hold the sum over the wave of a[c].
+template <int N>
+struct Dw {  // a value as N 32-bit words
+  unsigned w[N];
+};
+template <typename T>
+__device__ __forceinline__ Dw<sizeof(T) / 4> to_dw(T v) {
+  Dw<sizeof(T) / 4> r;
+  __builtin_memcpy(r.w, &v, sizeof(T));
+  return r;
+}
+template <typename T>
+__device__ __forceinline__ T from_dw(const Dw<sizeof(T) / 4>& r) {
+  T v;
+  __builtin_memcpy(&v, r.w, sizeof(T));
+  return v;
+}
+// x <- x + y after swapping x's upper half-wave with y's lower one (32) or
+// x's odd rows with y's even rows (16): lanes then hold the pair sums of x
+// (lower half / even rows) and of y (upper half / odd rows)
+template <typename T, int D>
+__device__ __forceinline__ T swap_add(T x, T y) {
+  auto a = to_dw(x), b = to_dw(y);
+#pragma unroll
+  for (int q = 0; q < (int)(sizeof(T) / 4); ++q) {
+    const auto r = D == 32 ? __builtin_amdgcn_permlane32_swap(a.w[q], b.w[q], false, false)
+                           : __builtin_amdgcn_permlane16_swap(a.w[q], b.w[q], false, false);
+    a.w[q] = r[0];
+    b.w[q] = r[1];
+  }
+  return from_dw<T>(a) + from_dw<T>(b);
+}
+template <int CTRL>
+__device__ __forceinline__ unsigned dpp_w(unsigned v) {
+  return (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, 0xF, 0xF, false);
+}
+// value of lane l ^ X (X in 1, 2, 4, 8) within 16-lane rows
+template <typename T, int X>
+__device__ __forceinline__ T xor_row(T v) {
+  auto a = to_dw(v);
+#pragma unroll
+  for (int q = 0; q < (int)(sizeof(T) / 4); ++q) {
+    if (X == 1) a.w[q] = dpp_w<0xB1>(a.w[q]);                 // quad_perm [1,0,3,2]
+    else if (X == 2) a.w[q] = dpp_w<0x4E>(a.w[q]);            // quad_perm [2,3,0,1]
+    else if (X == 4) a.w[q] = dpp_w<0x1B>(dpp_w<0x141>(a.w[q]));  // half_mirror (xor 7), then xor 3
+    else a.w[q] = dpp_w<0x128>(a.w[q]);                       // row_ror:8
+  }
+  return from_dw<T>(a);
+}
+template <typename T, int H, int X>
+__device__ __forceinline__ void rs_row_step(T (&a)[32], int lane) {
+  const bool up = (lane & X) != 0;
+#pragma unroll
+  for (int j = 0; j < H; ++j) {
+    const T keep = up ? a[j + H] : a[j];
+    const T send = up ? a[j] : a[j + H];
+    a[j] = keep + xor_row<T, X>(send);
+  }
+}
+template <typename T>
+__device__ __forceinline__ T wave_reduce_scatter32_nolds(T (&a)[32]) {
+  const int lane = threadIdx.x & (kWave - 1);
+#pragma unroll
+  for (int j = 0; j < 16; ++j) a[j] = swap_add<T, 32>(a[j], a[j + 16]);
+#pragma unroll
+  for (int j = 0; j < 8; ++j) a[j] = swap_add<T, 16>(a[j], a[j + 8]);
+  rs_row_step<T, 4, 8>(a, lane);
+  rs_row_step<T, 2, 4>(a, lane);
+  rs_row_step<T, 1, 2>(a, lane);
+  return a[0] + xor_row<T, 1>(a[0]);
+}
+
 // Value of v in lane l (l uniform) for every lane of the wave.
 __device__ __forceinline__ double lane_bcast(double v, int l) {
   const unsigned long long b = __double_as_longlong(v);
@@ -410,9 +488,9 @@ __global__ __launch_bounds__((NRW + 1) * kWave) void k_reflr(const ReflDesc<T>* 
         T acc[kRNB];
 #pragma unroll
         for (int c = 0; c < kRNB; ++c) acc[c] = cur[c] * xr;
-        sw = wave_reduce_scatter32(acc);
+        sw = wave_reduce_scatter32_nolds(acc);
       }
-      if (lane < kRNB) red[wv * kRNB + lane] = sw;
+      if ((lane & 1) == 0) red[wv * kRNB + (lane >> 1)] = sw;
     }
     __syncthreads();
     if (leader) {
