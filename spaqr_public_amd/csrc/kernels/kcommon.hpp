@@ -16,7 +16,77 @@ __device__ __forceinline__ T wave_sum(T v) {
   return v;
 }
 
+// ---------------------------------------------------------------------------
+// Cross-lane exchange without LDS (DPP and the gfx950 permlane swaps).
+template <int N>
+struct Dw {  // a value as N 32-bit words
+  unsigned w[N];
+};
+template <typename T>
+__device__ __forceinline__ Dw<sizeof(T) / 4> to_dw(T v) {
+  Dw<sizeof(T) / 4> r;
+  __builtin_memcpy(r.w, &v, sizeof(T));
+  return r;
+}
+template <typename T>
+__device__ __forceinline__ T from_dw(const Dw<sizeof(T) / 4>& r) {
+  T v;
+  __builtin_memcpy(&v, r.w, sizeof(T));
+  return v;
+}
+template <int CTRL>
+__device__ __forceinline__ unsigned dpp_w(unsigned v) {
+  return (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, 0xF, 0xF, false);
+}
+// Value of lane l ^ X (X a power of two below 64): DPP within 16-lane rows
+// (quad_perm for 1 and 2, row_half_mirror then quad_perm 3210 for 4,
+// row_ror:8 for 8), a permlane16 / permlane32 swap of the value with itself
+// for 16 / 32.  The partner lane must be active (callers run it converged,
+// or within groups of lanes that are active together).
+template <int X, typename T>
+__device__ __forceinline__ T lane_xor(T v) {
+  auto a = to_dw(v);
+  const int lane = threadIdx.x & (kWave - 1);
+#pragma unroll
+  for (int q = 0; q < (int)(sizeof(T) / 4); ++q) {
+    if (X == 1) a.w[q] = dpp_w<0xB1>(a.w[q]);
+    else if (X == 2) a.w[q] = dpp_w<0x4E>(a.w[q]);
+    else if (X == 4) a.w[q] = dpp_w<0x1B>(dpp_w<0x141>(a.w[q]));
+    else if (X == 8) a.w[q] = dpp_w<0x128>(a.w[q]);
+    else if (X == 16) {
+      const auto r = __builtin_amdgcn_permlane16_swap(a.w[q], a.w[q], false, false);
+      a.w[q] = (lane & 16) ? r[0] : r[1];
+    } else {
+      const auto r = __builtin_amdgcn_permlane32_swap(a.w[q], a.w[q], false, false);
+      a.w[q] = (lane & 32) ? r[0] : r[1];
+    }
+  }
+  return from_dw<T>(a);
+}
+// wave_sum with the same pairing order through lane_xor (converged waves)
+template <typename T>
+__device__ __forceinline__ T wave_sum_x(T v) {
+  v += lane_xor<32>(v);
+  v += lane_xor<16>(v);
+  v += lane_xor<8>(v);
+  v += lane_xor<4>(v);
+  v += lane_xor<2>(v);
+  v += lane_xor<1>(v);
+  return v;
+}
+
 // Block-wide sum of one value; `red` must hold >= blockDim.x/64 entries.
+template <typename T>
+__device__ __forceinline__ T block_sum_x(T v, T* red) {
+  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave, nw = blockDim.x / kWave;
+  v = wave_sum_x(v);
+  __syncthreads();
+  if (lane == 0) red[w] = v;
+  __syncthreads();
+  T s = 0;
+  for (int i = 0; i < nw; ++i) s += red[i];
+  return s;
+}
 template <typename T>
 __device__ __forceinline__ T block_sum(T v, T* red) {
   const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave, nw = blockDim.x / kWave;

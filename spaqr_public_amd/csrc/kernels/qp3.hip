@@ -27,6 +27,8 @@
 
 #include <algorithm>
 
+#include <type_traits>
+
 #include "kcommon.hpp"
 #include "descs.hpp"
 #include "launch.hpp"
@@ -229,7 +231,12 @@ __device__ __forceinline__ void argmax4(double& v, int& p, int& c, int& w, doubl
 // Butterfly sum over aligned groups of G lanes; every lane of a group ends
 // with the same bits (IEEE addition is commutative).
 __device__ __forceinline__ double group_sum(double v, int G) {
-  for (int o = 1; o < G; o <<= 1) v += __shfl_xor(v, o, kWave);
+  if (G > 1) v += lane_xor<1>(v);
+  if (G > 2) v += lane_xor<2>(v);
+  if (G > 4) v += lane_xor<4>(v);
+  if (G > 8) v += lane_xor<8>(v);
+  if (G > 16) v += lane_xor<16>(v);
+  if (G > 32) v += lane_xor<32>(v);
   return v;
 }
 // LDS slot of row r in the padded reflector layout
@@ -294,12 +301,18 @@ __global__ __launch_bounds__(NT, qp3r_min_blocks(NT, RC)) void k_qp3r(const Qp3D
   for (int i = 0; i < kmin; ++i) {
     const int buf = i & 1;
     // ---- local argmax of the workgroup
-#pragma unroll
-    for (int o = kWave / 2; o > 0; o >>= 1) {
-      const double ov = __shfl_xor(bv, o, kWave);
-      const int op = __shfl_xor(bp, o, kWave), oc = __shfl_xor(bc, o, kWave);
+    auto am_round = [&](auto xo) {
+      constexpr int O = decltype(xo)::value;
+      const double ov = lane_xor<O>(bv);
+      const int op = lane_xor<O>(bp), oc = lane_xor<O>(bc);
       argmax_combine(bv, bp, bc, ov, op, oc);
-    }
+    };
+    am_round(std::integral_constant<int, 32>{});
+    am_round(std::integral_constant<int, 16>{});
+    am_round(std::integral_constant<int, 8>{});
+    am_round(std::integral_constant<int, 4>{});
+    am_round(std::integral_constant<int, 2>{});
+    am_round(std::integral_constant<int, 1>{});
     if (lane == 0) { redv[tid / kWave] = bv; redp[tid / kWave] = bp; redc[tid / kWave] = bc; }
     __syncthreads();
     if (tid == 0) {
@@ -350,12 +363,18 @@ __global__ __launch_bounds__(NT, qp3r_min_blocks(NT, RC)) void k_qp3r(const Qp3D
         for (int w = tid; w < nwg; w += NT)
           argmax4(v, pp, cc, ww, agent_load(d.part_v + buf * nwg + w), agent_load(d.part_p + 2 * (buf * nwg + w)),
                   agent_load(d.part_p + 2 * (buf * nwg + w) + 1), w);
-#pragma unroll
-        for (int o = kWave / 2; o > 0; o >>= 1) {
-          const double ov = __shfl_xor(v, o, kWave);
-          const int op = __shfl_xor(pp, o, kWave), oc = __shfl_xor(cc, o, kWave), ow = __shfl_xor(ww, o, kWave);
+        auto am4_round = [&](auto xo) {
+          constexpr int O = decltype(xo)::value;
+          const double ov = lane_xor<O>(v);
+          const int op = lane_xor<O>(pp), oc = lane_xor<O>(cc), ow = lane_xor<O>(ww);
           argmax4(v, pp, cc, ww, ov, op, oc, ow);
-        }
+        };
+        am4_round(std::integral_constant<int, 32>{});
+        am4_round(std::integral_constant<int, 16>{});
+        am4_round(std::integral_constant<int, 8>{});
+        am4_round(std::integral_constant<int, 4>{});
+        am4_round(std::integral_constant<int, 2>{});
+        am4_round(std::integral_constant<int, 1>{});
         if (lane == 0) { redv[tid / kWave] = v; redp[tid / kWave] = pp; redc[tid / kWave] = cc; redw[tid / kWave] = ww; }
         __syncthreads();
         if (tid == 0) {
@@ -373,7 +392,7 @@ __global__ __launch_bounds__(NT, qp3r_min_blocks(NT, RC)) void k_qp3r(const Qp3D
     // ---- Householder vector of the pivot column (identical in every workgroup)
     double ss = 0;
     for (int r = i + 1 + tid; r < k; r += NT) { const double v = (double)vsh[vslot<RC>(r)]; ss += v * v; }
-    ss = block_sum(ss, sred);
+    ss = block_sum_x(ss, sred);
     const double alpha = (double)vsh[vslot<RC>(i)];
     double beta = alpha, tau = 0, scal = 0;
     if (ss > 0) {

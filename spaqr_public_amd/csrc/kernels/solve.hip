@@ -229,25 +229,8 @@ __device__ __forceinline__ T load_at(const T* base, unsigned byte_off) {
 // Reduce-scatter of 32 per-lane values over the wave without LDS: the
 // distance-32 and distance-16 steps swap half-waves / rows with
 // v_permlane32_swap / v_permlane16_swap (no selects), the steps within a
-// 16-lane row exchange through DPP (xor 8: row_ror:8; xor 4: row_half_mirror
-// then quad_perm 3210; xor 2, xor 1: quad_perm).  On return lanes 2c and
+// 16-lane row exchange through DPP (lane_xor).  On return lanes 2c and
 // 2c + 1 hold the sum over the wave of a[c].
-template <int N>
-struct Dw {  // a value as N 32-bit words
-  unsigned w[N];
-};
-template <typename T>
-__device__ __forceinline__ Dw<sizeof(T) / 4> to_dw(T v) {
-  Dw<sizeof(T) / 4> r;
-  __builtin_memcpy(r.w, &v, sizeof(T));
-  return r;
-}
-template <typename T>
-__device__ __forceinline__ T from_dw(const Dw<sizeof(T) / 4>& r) {
-  T v;
-  __builtin_memcpy(&v, r.w, sizeof(T));
-  return v;
-}
 // x <- x + y after swapping x's upper half-wave with y's lower one (32) or
 // x's odd rows with y's even rows (16): lanes then hold the pair sums of x
 // (lower half / even rows) and of y (upper half / odd rows)
@@ -263,23 +246,6 @@ __device__ __forceinline__ T swap_add(T x, T y) {
   }
   return from_dw<T>(a) + from_dw<T>(b);
 }
-template <int CTRL>
-__device__ __forceinline__ unsigned dpp_w(unsigned v) {
-  return (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, 0xF, 0xF, false);
-}
-// value of lane l ^ X (X in 1, 2, 4, 8) within 16-lane rows
-template <typename T, int X>
-__device__ __forceinline__ T xor_row(T v) {
-  auto a = to_dw(v);
-#pragma unroll
-  for (int q = 0; q < (int)(sizeof(T) / 4); ++q) {
-    if (X == 1) a.w[q] = dpp_w<0xB1>(a.w[q]);                 // quad_perm [1,0,3,2]
-    else if (X == 2) a.w[q] = dpp_w<0x4E>(a.w[q]);            // quad_perm [2,3,0,1]
-    else if (X == 4) a.w[q] = dpp_w<0x1B>(dpp_w<0x141>(a.w[q]));  // half_mirror (xor 7), then xor 3
-    else a.w[q] = dpp_w<0x128>(a.w[q]);                       // row_ror:8
-  }
-  return from_dw<T>(a);
-}
 template <typename T, int H, int X>
 __device__ __forceinline__ void rs_row_step(T (&a)[32], int lane) {
   const bool up = (lane & X) != 0;
@@ -287,7 +253,7 @@ __device__ __forceinline__ void rs_row_step(T (&a)[32], int lane) {
   for (int j = 0; j < H; ++j) {
     const T keep = up ? a[j + H] : a[j];
     const T send = up ? a[j] : a[j + H];
-    a[j] = keep + xor_row<T, X>(send);
+    a[j] = keep + lane_xor<X>(send);
   }
 }
 template <typename T>
@@ -300,7 +266,7 @@ __device__ __forceinline__ T wave_reduce_scatter32_nolds(T (&a)[32]) {
   rs_row_step<T, 4, 8>(a, lane);
   rs_row_step<T, 2, 4>(a, lane);
   rs_row_step<T, 1, 2>(a, lane);
-  return a[0] + xor_row<T, 1>(a[0]);
+  return a[0] + lane_xor<1>(a[0]);
 }
 
 // Value of v in lane l (l uniform) for every lane of the wave.
