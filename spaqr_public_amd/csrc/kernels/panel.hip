@@ -70,8 +70,8 @@ __global__ __launch_bounds__(kThreads) void k_panel_r(const PanelDesc<T>* __rest
     T dd[kNB];
 #pragma unroll
     for (int c = 0; c < kNB; ++c) dd[c] = below ? x * a[c] : T(0);
-    const T sw = wave_reduce_scatter32(dd);
-    if (lane < kNB) red[wv * kNB + lane] = sw;
+    const T sw = wave_reduce_scatter32_nolds(dd);
+    if ((lane & 1) == 0) red[wv * kNB + (lane >> 1)] = sw;
     if (g == 0 && tid == jj) {
 #pragma unroll
       for (int c = 0; c < kNB; ++c) rowsh[c] = a[c];
@@ -97,8 +97,17 @@ __global__ __launch_bounds__(kThreads) void k_panel_r(const PanelDesc<T>* __rest
         group_barrier_rx(d.sync, target, d.err);
       }
       if (tid < kNB) {
+        // the partials of all workgroups, 8 loads in flight at a time, summed
+        // in workgroup order
         T s = T(0);
-        for (int q = 0; q < nwg; ++q) s += agent_load(d.part + ((int64_t)buf * nwg + q) * (2 * kNB) + tid);
+        const T* pb = d.part + (int64_t)buf * nwg * (2 * kNB) + tid;
+        for (int q0 = 0; q0 < nwg; q0 += 8) {
+          T pv[8];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) pv[u] = agent_load(pb + (int64_t)min(q0 + u, nwg - 1) * (2 * kNB));
+#pragma unroll
+          for (int u = 0; u < 8; ++u) s += (q0 + u < nwg) ? pv[u] : T(0);
+        }
         Dsh[tid] = s;
         if (g != 0) rowsh[tid] = agent_load(d.part + (int64_t)buf * nwg * (2 * kNB) + kNB + tid);
       }

@@ -225,50 +225,6 @@ __device__ __forceinline__ T load_at(const T* base, unsigned byte_off) {
   return *(gptr<T>)((const __attribute__((address_space(1))) char*)to_global(base) + byte_off);
 }
 
-// ---------------------------------------------------------------------------
-// Reduce-scatter of 32 per-lane values over the wave without LDS: the
-// distance-32 and distance-16 steps swap half-waves / rows with
-// v_permlane32_swap / v_permlane16_swap (no selects), the steps within a
-// 16-lane row exchange through DPP (lane_xor).  On return lanes 2c and
-// 2c + 1 hold the sum over the wave of a[c].
-// x <- x + y after swapping x's upper half-wave with y's lower one (32) or
-// x's odd rows with y's even rows (16): lanes then hold the pair sums of x
-// (lower half / even rows) and of y (upper half / odd rows)
-template <typename T, int D>
-__device__ __forceinline__ T swap_add(T x, T y) {
-  auto a = to_dw(x), b = to_dw(y);
-#pragma unroll
-  for (int q = 0; q < (int)(sizeof(T) / 4); ++q) {
-    const auto r = D == 32 ? __builtin_amdgcn_permlane32_swap(a.w[q], b.w[q], false, false)
-                           : __builtin_amdgcn_permlane16_swap(a.w[q], b.w[q], false, false);
-    a.w[q] = r[0];
-    b.w[q] = r[1];
-  }
-  return from_dw<T>(a) + from_dw<T>(b);
-}
-template <typename T, int H, int X>
-__device__ __forceinline__ void rs_row_step(T (&a)[32], int lane) {
-  const bool up = (lane & X) != 0;
-#pragma unroll
-  for (int j = 0; j < H; ++j) {
-    const T keep = up ? a[j + H] : a[j];
-    const T send = up ? a[j] : a[j + H];
-    a[j] = keep + lane_xor<X>(send);
-  }
-}
-template <typename T>
-__device__ __forceinline__ T wave_reduce_scatter32_nolds(T (&a)[32]) {
-  const int lane = threadIdx.x & (kWave - 1);
-#pragma unroll
-  for (int j = 0; j < 16; ++j) a[j] = swap_add<T, 32>(a[j], a[j + 16]);
-#pragma unroll
-  for (int j = 0; j < 8; ++j) a[j] = swap_add<T, 16>(a[j], a[j + 8]);
-  rs_row_step<T, 4, 8>(a, lane);
-  rs_row_step<T, 2, 4>(a, lane);
-  rs_row_step<T, 1, 2>(a, lane);
-  return a[0] + lane_xor<1>(a[0]);
-}
-
 // Value of v in lane l (l uniform) for every lane of the wave.
 __device__ __forceinline__ double lane_bcast(double v, int l) {
   const unsigned long long b = __double_as_longlong(v);
