@@ -185,8 +185,18 @@ __global__ __launch_bounds__(kThreads) void k_larft(const LarftDesc<T>* __restri
       for (int c = 0; c < kNB; ++c)
         if (c < jj) acc[c] += vi * d.V[i + (int64_t)(jb + c) * ld];
     }
-    block_sum_n<T, kNB>(acc, red, jj);
-    if (tid < kNB) gsh[tid] = (tid < jj) ? (d.V[j + (int64_t)(jb + tid) * ld] + acc[tid]) : T(0);
+    // one reduce-scatter for all jj dots (entries c >= jj are zero), then
+    // the waves' partials
+    const int lane = tid & (kWave - 1), wv = tid / kWave;
+    const T sw = wave_reduce_scatter32_nolds(acc);
+    if ((lane & 1) == 0) red[wv * kNB + (lane >> 1)] = sw;
+    __syncthreads();
+    if (tid < kNB) {
+      T s = T(0);
+#pragma unroll
+      for (int q = 0; q < kThreads / kWave; ++q) s += red[q * kNB + tid];
+      gsh[tid] = (tid < jj) ? (d.V[j + (int64_t)(jb + tid) * ld] + s) : T(0);
+    }
     __syncthreads();
     tcol_from_dots(Ts, jj, d.tau[j], gsh);
     __syncthreads();
