@@ -64,17 +64,36 @@ void Stager::upload_raw(void* dst, const void* src, size_t bytes) {
     up_off_ = 0;
   }
   if (bytes > up_cap_) {  // larger than the buffer: a synchronous copy
+    flush_uploads();
     SPQ_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream_));
     SPQ_HIP(hipStreamSynchronize(stream_));
     return;
   }
   if (up_off_ + bytes > up_cap_) {  // every earlier upload must have landed
+    flush_uploads();
     SPQ_HIP(hipStreamSynchronize(stream_));
     up_off_ = 0;
   }
   std::memcpy(up_host_ + up_off_, src, bytes);
-  SPQ_HIP(hipMemcpyAsync(dst, up_host_ + up_off_, bytes, hipMemcpyHostToDevice, stream_));
+  char* d = (char*)dst;
+  // the pending copy grows when this table sits at the same distance from
+  // its start on the device as in the staging buffer (the bytes in between
+  // are alignment padding of the earlier destination)
+  if (up_pend_bytes_ && d == up_pend_dev_ + (up_off_ - up_pend_host_)) {
+    up_pend_bytes_ = up_off_ - up_pend_host_ + bytes;
+  } else {
+    flush_uploads();
+    up_pend_host_ = up_off_;
+    up_pend_dev_ = d;
+    up_pend_bytes_ = bytes;
+  }
   up_off_ += (bytes + 255) / 256 * 256;
+}
+
+void Stager::flush_uploads() {
+  if (!up_pend_bytes_) return;
+  SPQ_HIP(hipMemcpyAsync(up_pend_dev_, up_host_ + up_pend_host_, up_pend_bytes_, hipMemcpyHostToDevice, stream_));
+  up_pend_bytes_ = 0;
 }
 
 void Stager::init(hipStream_t s, size_t bytes) {
@@ -103,6 +122,7 @@ const void* Stager::push_raw(const void* src, size_t bytes) {
 }
 
 void Stager::flush() {
+  flush_uploads();
   if (off_ > pend_) SPQ_HIP(hipMemcpyAsync(dev_ + pend_, host_ + pend_, off_ - pend_, hipMemcpyHostToDevice, stream_));
   pend_ = off_;
 }

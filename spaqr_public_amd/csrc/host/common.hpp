@@ -108,8 +108,15 @@ class Stager {
   // Copies everything pushed since the last flush with one transfer; every
   // launch that reads pushed tables flushes first.
   void flush();
-  // only after the stream has been synchronised
-  void reset() { off_ = 0; pend_ = 0; up_off_ = 0; }
+  // only after the stream has been synchronised (uploads still pending are
+  // issued and waited for first)
+  void reset() {
+    if (up_pend_bytes_) {
+      flush_uploads();
+      SPQ_HIP(hipStreamSynchronize(stream_));
+    }
+    off_ = 0; pend_ = 0; up_off_ = 0;
+  }
   // Persistent copy (for solve programs): allocated from `perm` and kept.
   // The table goes through a pinned buffer with an asynchronous copy; the
   // buffer is reused after the next synchronisation (or a forced one when
@@ -121,7 +128,11 @@ class Stager {
     upload_raw(d, v.data(), sizeof(X) * v.size());
     return d;
   }
+  // Persistent uploads whose device destinations follow one another (the
+  // arena hands them out in order) are merged into one copy, issued here;
+  // flush() and every synchronisation issue it first.
   void upload_raw(void* dst, const void* src, size_t bytes);
+  void flush_uploads();
 
  private:
   hipStream_t stream_ = nullptr;
@@ -130,6 +141,8 @@ class Stager {
   size_t cap_ = 0, off_ = 0, pend_ = 0;  // [pend_, off_) not yet copied
   char* up_host_ = nullptr;               // pinned source of persistent uploads
   size_t up_cap_ = 0, up_off_ = 0;
+  size_t up_pend_host_ = 0, up_pend_bytes_ = 0;  // merged upload not yet issued:
+  char* up_pend_dev_ = nullptr;                   // [up_pend_host_, +bytes) -> up_pend_dev_
 };
 
 // Prefix sums of per-descriptor tile counts (tstart table of tiled kernels).

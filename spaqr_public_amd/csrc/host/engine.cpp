@@ -268,6 +268,7 @@ class Engine final : public EngineBase {
     return s.level() > ilvl_ && s.id.l.lvl <= ilvl_ && s.id.r.lvl <= ilvl_;
   }
   void sync() {
+    stager_.flush_uploads();
     const double a = wall();
     SPQ_HIP(hipStreamSynchronize(stream_));
     t_wait_ += wall() - a;
