@@ -718,7 +718,14 @@ __global__ __launch_bounds__(kBThreads) void k_bwdr(const BwdDesc<T>* __restrict
 #pragma unroll
       for (int c = 1; c < kRNB; ++c) diag = (c == (lane & (kRNB - 1))) ? rr[c] : diag;
       const T dinv = T(1) / diag;
-      if (!tr) {
+      if (!tr && w == kRNB) {
+#pragma unroll
+        for (int i = kRNB - 1; i >= 0; --i) {
+          if (lane == i) xl *= dinv;
+          const T xi = lane_bcast(xl, i);
+          if (lane < i) xl -= rr[i] * xi;
+        }
+      } else if (!tr) {
 #pragma unroll
         for (int i = kRNB - 1; i >= 0; --i) {
           if (i < w) {
@@ -726,6 +733,13 @@ __global__ __launch_bounds__(kBThreads) void k_bwdr(const BwdDesc<T>* __restrict
             const T xi = lane_bcast(xl, i);
             if (lane < i) xl -= rr[i] * xi;
           }
+        }
+      } else if (w == kRNB) {
+#pragma unroll
+        for (int i = 0; i < kRNB; ++i) {
+          if (lane == i) xl *= dinv;
+          const T xi = lane_bcast(xl, i);
+          if (lane > i && lane < kRNB) xl -= rr[i] * xi;
         }
       } else {
 #pragma unroll
@@ -747,11 +761,17 @@ __global__ __launch_bounds__(kBThreads) void k_bwdr(const BwdDesc<T>* __restrict
         const int r = r0 + (tid - kWave);
         if (r0 > ulo && r < uhi) load_upd(bi, r, ru);
         if (r < uhi) {
-          T s = T(0);
+          // four chains; a full block needs no column test
+          T s4[4] = {T(0), T(0), T(0), T(0)};
+          if (w == kRNB) {
 #pragma unroll
-          for (int c = 0; c < kRNB; ++c)
-            if (c < w) s += ru[c] * xs[jb + c];
-          xs[r] -= s;
+            for (int c = 0; c < kRNB; ++c) s4[c & 3] += ru[c] * xs[jb + c];
+          } else {
+#pragma unroll
+            for (int c = 0; c < kRNB; ++c)
+              if (c < w) s4[c & 3] += ru[c] * xs[jb + c];
+          }
+          xs[r] -= (s4[0] + s4[1]) + (s4[2] + s4[3]);
         }
       }
     }
