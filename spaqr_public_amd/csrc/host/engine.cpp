@@ -487,11 +487,14 @@ class Engine final : public EngineBase {
               [&] { rc = launch_panel_r<U>(stream_, dpd, dst, (int)L.first.size(), nt, jb, L.second); });
         if (rc != 0) throw Error(kErrHip, std::string("panel launch failed: ") + hipGetErrorString((hipError_t)rc));
       }
-      // trailing matrices (Y, Z temporaries live in ws_, rewound per step)
-      std::vector<GemmDesc<U>> g1, g2, g3;
+      // trailing matrices: C -= (V_b T_b^T)(V_b^T C).  W = V_b T_b^T is formed
+      // once per problem in the same launch as the Y = V_b^T C of its
+      // targets (W and Y live in ws_, rewound per step); T_b's strict lower
+      // part is stored as zeros, so it enters as a dense factor
+      std::vector<GemmDesc<U>> g1, g3;
       size_t ntg = 0;
       for (int q : live) ntg += 1 + jobs[q].extra.size();
-      g1.reserve(ntg); g2.reserve(ntg); g3.reserve(ntg);
+      g1.reserve(ntg + live.size()); g3.reserve(ntg);
       struct Tgt { U* C; int ld, cols; };
       std::vector<Tgt> tg;
       for (int q : live) {
@@ -505,16 +508,17 @@ class Engine final : public EngineBase {
         if (j.k > c0) tg.push_back({j.P + jb + (int64_t)c0 * j.ld, j.ld, j.k - c0});
         for (auto& b : j.extra)
           if (b.cols > 0) tg.push_back({b.p + jb, b.ld, b.cols});
+        if (tg.empty()) continue;
+        U* W = (U*)ws_.alloc(sizeof(U) * (size_t)std::max(rows, 1) * nb_);
+        g1.push_back(gd(Vb, j.ld, Tb, nb_, W, std::max(rows, 1), rows, w, w,
+                        gemm_flags(false, true, kStructUnitLower, kStructNone), U(1), U(0)));
         for (auto& t : tg) {
           U* Y = (U*)ws_.alloc(sizeof(U) * (size_t)nb_ * t.cols);
-          U* Z = (U*)ws_.alloc(sizeof(U) * (size_t)nb_ * t.cols);
           g1.push_back(gd(Vb, j.ld, t.C, t.ld, Y, nb_, w, t.cols, rows, gemm_flags(true, false, kStructUnitLower), U(1), U(0)));
-          g2.push_back(gd(Tb, nb_, Y, nb_, Z, nb_, w, t.cols, w, gemm_flags(true, false, kStructUpper), U(1), U(0)));
-          g3.push_back(gd(Vb, j.ld, Z, nb_, t.C, t.ld, rows, t.cols, w, gemm_flags(false, false, kStructUnitLower), U(-1), U(1)));
+          g3.push_back(gd(W, std::max(rows, 1), Y, nb_, t.C, t.ld, rows, t.cols, w, gemm_flags(false, false), U(-1), U(1)));
         }
       }
       gemm_batch(g1, "gemm_y");
-      gemm_batch(g2, "gemm_z");
       gemm_batch(g3, "gemm_c");
     }
   }
