@@ -511,21 +511,31 @@ void launch_reflr(hipStream_t s, const ReflDesc<T>* d, const int* wstart, int nd
 template <typename T>
 __device__ __forceinline__ void diag_solve(const T* rsh, T* __restrict__ xs, int jb, int w, int trans, bool shared) {
   if (threadIdx.x >= kWave) return;
-  const int l = threadIdx.x;
+  const int l = threadIdx.x, lc = l & (kRNB - 1);
   T xl = (l < w) ? xs[jb + l] : T(0);
+  // lane l's row of the block (its column for R^T) and 1 / R[l, l], read
+  // together; x_i is broadcast by readlane (all 64 lanes of the wave run)
+  T rr[kRNB];
+#pragma unroll
+  for (int i = 0; i < kRNB; ++i) rr[i] = trans ? rsh[i + lc * kTLd] : rsh[lc + i * kTLd];
+  const T dinv = T(1) / rsh[lc + lc * kTLd];
   if (!trans) {
-    for (int i = w - 1; i >= 0; --i) {
-      T xi = __shfl(xl, i, kWave);
-      xi = xi / rsh[i + i * kTLd];
-      if (l == i) xl = xi;
-      if (l < i) xl -= rsh[l + i * kTLd] * xi;
+#pragma unroll
+    for (int i = kRNB - 1; i >= 0; --i) {
+      if (i < w) {
+        if (l == i) xl *= dinv;
+        const T xi = lane_bcast(xl, i);
+        if (l < i) xl -= rr[i] * xi;
+      }
     }
   } else {
-    for (int i = 0; i < w; ++i) {
-      T xi = __shfl(xl, i, kWave);
-      xi = xi / rsh[i + i * kTLd];
-      if (l == i) xl = xi;
-      if (l > i && l < w) xl -= rsh[i + l * kTLd] * xi;
+#pragma unroll
+    for (int i = 0; i < kRNB; ++i) {
+      if (i < w) {
+        if (l == i) xl *= dinv;
+        const T xi = lane_bcast(xl, i);
+        if (l > i && l < w) xl -= rr[i] * xi;
+      }
     }
   }
   if (l < w) {
