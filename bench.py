@@ -32,6 +32,13 @@ FAMILY_KERNEL = {
 }
 
 
+MATRIX_DESC = {
+    "advdiff3d": "-Lap u + v.grad u, 7-pt, a=1, upwind v=(100,100,100), h=1/(n+1)",
+    "advdiff2d": "-Lap u + v.grad u, 5-pt, a=1, v=(1000,1000) (upwind; _central: central differences), h=1/(n+1)",
+    "hicontrast3d": "-div(a grad u), 7-pt FV, a piecewise constant on 8^3 blocks, log10 a ~ U(-2,2)",
+}
+
+
 def family_kernel(fam):
     """Kernel symbol (as rocprofv3 prints it) of a timing family; the register
     RRQR families are qp3r<threads>_<rows per lane>."""
@@ -82,10 +89,11 @@ def setup_tree(sq, As, n, dim, tol, nlevels, precision):
     return t
 
 
-def cpu_baseline(workload_flops, gpu_iters, threads, budget_s=40.0):
-    """Oracle (CPU restatement, OpenBLAS LAPACK) on the 32^3 instance of the
-    same workload family, scaled to the 64^3 workload by the factor-flop ratio
-    of the two ledgers."""
+def cpu_baseline(name, tol, dim, threads):
+    """The oracle (the CPU restatement of the reference, OpenBLAS LAPACK when
+    the host has it) on the benchmarked workload itself: same matrix, levels,
+    tolerance and right-hand side; partition and assemble untimed, then one
+    factorize + GMRES to 1e-12 timed, as one GPU step is."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as po
     import numpy as np
@@ -97,13 +105,14 @@ def cpu_baseline(workload_flops, gpu_iters, threads, budget_s=40.0):
         backend = po.set_backend("openblas", threads=threads)
     except Exception:
         po.set_backend("own")
-    A = gen.config_matrix("advdiff3d_32")
+    n = int(name.split("_")[1])
+    A = gen.config_matrix(name)
     As, _ = po.colscale(A)
     rng = np.random.default_rng(2021)
     b = rng.standard_normal(As.shape[0])
     t = po.Tree(po.default_levels(As.shape[1]), 0)
-    t.configure(1e-2, scale=1, order=1.0, square=True)
-    t.set_tensor_coords(32, 3)
+    t.configure(tol, scale=1, order=1.0, square=True, verbose=3)  # per-level progress on stderr
+    t.set_tensor_coords(n, dim)
     t.partition(As)
     t.assemble()
     t0 = time.perf_counter()
@@ -112,17 +121,18 @@ def cpu_baseline(workload_flops, gpu_iters, threads, budget_s=40.0):
     x, it, est = t.gmres(b, 300, 1e-12)
     t2 = time.perf_counter()
     c = t.counters()
-    sample_s = t2 - t0
-    scale = workload_flops / c["flops"] if c["flops"] > 0 else float("nan")
+    res = float(np.linalg.norm(As @ x - b) / np.linalg.norm(b))
     return {
-        "value": (t1 - t0) * scale + (t2 - t1) * 8.0,  # factor by flop ratio, GMRES by n ratio (x8)
+        "value": t2 - t0,
         "unit": "s",
         "cores": threads if backend == "openblas" else 1,
         "kind": "port",
-        "sample": (f"oracle (CPU restatement, LAPACK={backend}) factor+GMRES on advdiff3d_32 (n=32768, t=1e-2): "
-                   f"factor {t1 - t0:.2f} s ({c['flops'] / 1e9:.1f} GFLOP ledger), GMRES {t2 - t1:.2f} s "
-                   f"({it} its); scaled to 64^3 by factor-flop ratio x{scale:.1f} and GMRES by n ratio x8"),
-        "sample_seconds": sample_s,
+        "sample": (f"oracle (CPU restatement of the reference, LAPACK={backend}, {threads} threads) on the same "
+                   f"workload {name} (t={tol:g}): one factorize + GMRES to 1e-12, factor {t1 - t0:.2f} s "
+                   f"({c['flops'] / 1e9:.1f} GFLOP ledger), GMRES {t2 - t1:.2f} s ({it} its, final {res:.1e})"),
+        "factor_s": t1 - t0,
+        "gmres_s": t2 - t1,
+        "gmres_iters": it,
     }
 
 
@@ -323,6 +333,7 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-2)
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=0, help="threads of the CPU baseline (default: all, <= 16)")
     # measurement bookkeeping (no benchmark run)
     ap.add_argument("--copy-calibration", action="store_true", help="run the PMC copy calibration and exit")
     ap.add_argument("--trace-summary", nargs=2, metavar=("STATS_DIR", "OUT_MD"))
@@ -403,6 +414,13 @@ def main():
         print(f"# step {step} {'timed' if timed else 'warmup'}: factor {t1 - t0:.3f} s, GMRES {t2 - t1:.3f} s "
               f"({it} its, final {info['final_res']:.2e})", file=sys.stderr, flush=True)
 
+    # release device memory before the interpreter exits: buffers left to
+    # finalisation would be freed after the HIP runtime (or a profiler
+    # attached to it) has begun its own teardown
+    hip.synchronize()
+    M.close()
+    for arr in (db, dx, dx_host):
+        arr.free()
     ms = 1000.0 * float(np.mean(times))
     fac = float(np.mean(fac_t))
     kry = float(np.mean(kry_t))
@@ -439,8 +457,10 @@ def main():
                             bytes_per_launch=v["bytes"] / max(v["launches"], 1))
     cpu = None
     if not args.no_cpu_baseline:
+        threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
+        print(f"# cpu baseline: oracle on {name}, {threads} threads", file=sys.stderr, flush=True)
         try:
-            cpu = cpu_baseline(info["flops"], info["iters"], threads=min(os.cpu_count() or 1, 16))
+            cpu = cpu_baseline(name, args.tol, dim, threads)
         except Exception as e:  # report, never fail the GPU measurement
             cpu = {"error": str(e)}
     model, ncpu = cpu_info()
@@ -459,7 +479,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": name, "n": N, "nnz": int(As.nnz), "tol": args.tol, "levels": nlevels,
                    "precision": args.precision, "rhs": "N(0,1) seed 2021",
-                   "matrix": "-Lap u + v.grad u, 7-pt, a=1, upwind v=(100,100,100), h=1/(n+1)"},
+                   "matrix": MATRIX_DESC.get(name.split("_")[0], name)},
         "factor_s": fac,
         "gmres_s": kry,
         "gmres_iters": info["iters"],

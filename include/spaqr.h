@@ -154,6 +154,10 @@ int spaqr_counters(const spaqr_tree* t, unsigned long long* nnz4, double* dbl8);
 /* Sparsified interfaces in call order: ilvl, sep lvl, sep index, cols before, cols after. */
 int spaqr_num_sparsified(const spaqr_tree* t);
 int spaqr_sparsified(const spaqr_tree* t, int* out5);
+/* Least squares: every sparsify_extra compression (tree.cpp:1260-1307) in
+ * call order: ilvl, sep lvl, sep index, extra rows before, extra rows kept. */
+int spaqr_num_extra_sparsified(const spaqr_tree* t);
+int spaqr_extra_sparsified(const spaqr_tree* t, int* out5);
 /* Cluster table: per cluster of bottoms[l] (12 ints): l, self(lvl,sep),
  * left(lvl,sep), right(lvl,sep), cstart, csize, rstart, rsize, order.      */
 int spaqr_num_clusters(const spaqr_tree* t);

@@ -155,10 +155,10 @@ void orc_solve_nrml(void* p, const double* b, double* x) {
   t->solve_nrml(bb, xx);
   std::memcpy(x, xx.data(), sizeof(double) * t->cols());
 }
-int orc_gmres(void* p, const double* b, double* x, int iters, double tol, double* tol_out, int verbose) {
+int orc_gmres(void* p, const double* b, double* x, int iters, int restart, double tol, double* tol_out, int verbose) {
   Handle* h = (Handle*)p;
   std::vector<double> bb(b, b + h->A.m), xx(x, x + h->A.n);
-  int it = gmres(h->A, bb, xx, *h->t, iters, iters, tol, tol_out, verbose);
+  int it = gmres(h->A, bb, xx, *h->t, iters, restart > 0 ? restart : iters, tol, tol_out, verbose);
   std::memcpy(x, xx.data(), sizeof(double) * h->A.n);
   return it;
 }
@@ -196,6 +196,14 @@ void orc_counters(void* p, unsigned long long* nnz4, double* dbl5) {
 int orc_num_sparsified(void* p) { return (int)((Handle*)p)->t->sparsified.size(); }
 void orc_sparsified(void* p, int* out) {
   auto& s = ((Handle*)p)->t->sparsified;
+  for (size_t i = 0; i < s.size(); ++i) {
+    out[5 * i] = s[i].ilvl; out[5 * i + 1] = s[i].lvl; out[5 * i + 2] = s[i].sep;
+    out[5 * i + 3] = s[i].before; out[5 * i + 4] = s[i].after;
+  }
+}
+int orc_num_extra_sparsified(void* p) { return (int)((Handle*)p)->t->extra_sparsified.size(); }
+void orc_extra_sparsified(void* p, int* out) {
+  auto& s = ((Handle*)p)->t->extra_sparsified;
   for (size_t i = 0; i < s.size(); ++i) {
     out[5 * i] = s[i].ilvl; out[5 * i + 1] = s[i].lvl; out[5 * i + 2] = s[i].sep;
     out[5 * i + 3] = s[i].before; out[5 * i + 4] = s[i].after;

@@ -434,7 +434,7 @@ void Tree::partition(const Csc& A) {  // tree.cpp:595-723
     }
   }
   t_partition = now_s() - t0;
-  if (verbose) printf("Time to partition: %g\n", t_partition);
+  if (verbose && verbose < 3) printf("Time to partition: %g\n", t_partition);
 }
 
 void Tree::assemble(const Csc& A) {  // tree.cpp:725-778
@@ -466,7 +466,7 @@ void Tree::assemble(const Csc& A) {  // tree.cpp:725-778
     self->cnbrs = cn;
   }
   t_assemble = now_s() - t0;
-  if (verbose) {
+  if (verbose && verbose < 3) {
     printf("Time to assemble: %g\n", t_assemble);
     auto ts = topsize();
     printf("Aspect ratio of top separator: %g\n", (double)ts.first / (double)ts.second);
@@ -924,6 +924,7 @@ void Tree::sparsify_extra(Cluster* c) {  // tree.cpp:1260-1307 (+ sparsify_extra
     for (int j = 0; j < C.cols(); ++j)
       for (int r = 0; r < rank_rows; ++r) Cr(r, j) = (r <= pos[j]) ? C(r, pos[j]) : 0.0;
     rest = Cr;
+    extra_sparsified.push_back({ilvl, c->id.self.lvl, c->id.self.sep, (int)C.rows(), rank_rows});
     rank_rows += rank;
   }
   c->reset_size(rank_rows, rank);
@@ -1010,14 +1011,17 @@ int Tree::factorize() {  // tree.cpp:1309-1437
     st.t_sprsfy = t5 - t4; st.t_merge = t6 - t5;
     auto ts = topsize();
     st.top_rows = ts.first; st.top_cols = ts.second;
+    // verbose >= 3: the level lines go to stderr (progress of a long run
+    // whose stdout is kept for a result line)
     if (verbose)
-      printf("lvl: %d    update fill-in: %.3f  elmn: %.3f  shift: %.3f  scale: %.3f  sprsfy: %.3f  merge: %.3f  "
+      fprintf(verbose >= 3 ? stderr : stdout,
+             "lvl: %d    update fill-in: %.3f  elmn: %.3f  shift: %.3f  scale: %.3f  sprsfy: %.3f  merge: %.3f  "
              "size top_sep: %d, %d   a.r top_sep: %.3f\n",
              ilvl, st.t_fill, st.t_elmn, st.t_shift, st.t_scale, st.t_sprsfy, st.t_merge, ts.first, ts.second,
              (double)ts.first / (double)ts.second);
   }
   t_factorize = now_s() - fstart;
-  if (verbose) {
+  if (verbose && verbose < 3) {
     printf("Tolerance set: %.3e\n", tol);
     printf("Time to factorize:  %.3e\n", t_factorize);
     printf("Size of top separator: %d\n", top_size());
@@ -1309,7 +1313,7 @@ int gmres(const Csc& A, const std::vector<double>& rhs, std::vector<double>& x, 
   for (int i = 0; i < m; ++i) p0[i] = rhs[i] - Ax[i];
   M.solve(p0, r0);
   double r0n = norm(r0);
-  if (verbose) {
+  if (verbose && verbose < 3) {
     csc_matvec(A, r0.data(), Ax.data());
     double s = 0;
     for (int i = 0; i < m; ++i) s += (Ax[i] - p0[i]) * (Ax[i] - p0[i]);

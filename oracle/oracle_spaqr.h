@@ -150,6 +150,9 @@ class Tree {
   std::vector<int> cperm, rperm;
   std::vector<LevelStat> stats;
   std::vector<SparsifyRec> sparsified;
+  // sparsify_extra calls that ran an RRQR (tree.cpp:1283-1286): level, sep,
+  // extra rows before and after the compression
+  std::vector<SparsifyRec> extra_sparsified;
   unsigned long long nnzA = 0, nnzR = 0, nnzH = 0, nnzQ = 0;
   double flops = 0;  // LAWN-41 ledger of the dense kernels
   double t_partition = 0, t_assemble = 0, t_factorize = 0;

@@ -46,13 +46,15 @@ def lib():
         L.orc_factorize.argtypes = [C.c_void_p]
         L.orc_solve.argtypes = [C.c_void_p, _dp, _dp]
         L.orc_solve_nrml.argtypes = [C.c_void_p, _dp, _dp]
-        L.orc_gmres.argtypes = [C.c_void_p, _dp, _dp, C.c_int, C.c_double, _dp, C.c_int]
+        L.orc_gmres.argtypes = [C.c_void_p, _dp, _dp, C.c_int, C.c_int, C.c_double, _dp, C.c_int]
         L.orc_cgls.argtypes = [C.c_void_p, _dp, _dp, C.c_int, C.c_double, _dp, C.c_int]
         L.orc_perms.argtypes = [C.c_void_p, _ip, _ip]
         L.orc_level_stats.argtypes = [C.c_void_p, _dp]
         L.orc_counters.argtypes = [C.c_void_p, C.POINTER(C.c_ulonglong), _dp]
         L.orc_num_sparsified.argtypes = [C.c_void_p]
         L.orc_sparsified.argtypes = [C.c_void_p, _ip]
+        L.orc_num_extra_sparsified.argtypes = [C.c_void_p]
+        L.orc_extra_sparsified.argtypes = [C.c_void_p, _ip]
         L.orc_num_clusters.argtypes = [C.c_void_p]
         L.orc_cluster_table.argtypes = [C.c_void_p, _ip]
         L.orc_use_openblas.argtypes = [C.c_char_p]
@@ -241,11 +243,12 @@ class Tree:
         lib().orc_solve_nrml(self.h, _d(b), _d(x))
         return x
 
-    def gmres(self, b, iters=300, tol=1e-12, verbose=0):
+    def gmres(self, b, iters=300, tol=1e-12, verbose=0, restart=None):
+        """is.cpp:95-272; restart defaults to iters (spaQR.cpp:304)."""
         b = np.ascontiguousarray(b, dtype=np.float64)
         x = np.zeros(self.n)
         err = C.c_double(0)
-        it = lib().orc_gmres(self.h, _d(b), _d(x), iters, tol, C.byref(err), verbose)
+        it = lib().orc_gmres(self.h, _d(b), _d(x), iters, restart or iters, tol, C.byref(err), verbose)
         return x, it, err.value
 
     def cgls(self, b, iters=300, tol=1e-12, verbose=0):
@@ -277,6 +280,14 @@ class Tree:
         k = lib().orc_num_sparsified(self.h)
         out = np.zeros(5 * max(k, 1), dtype=np.int32)
         lib().orc_sparsified(self.h, _i(out))
+        return out[: 5 * k].reshape(k, 5)
+
+    def extra_sparsified(self):
+        """Least squares: (ilvl, lvl, sep, extra rows before, after) of every
+        sparsify_extra compression (tree.cpp:1260-1307)."""
+        k = lib().orc_num_extra_sparsified(self.h)
+        out = np.zeros(5 * max(k, 1), dtype=np.int32)
+        lib().orc_extra_sparsified(self.h, _i(out))
         return out[: 5 * k].reshape(k, 5)
 
     def cluster_table(self):

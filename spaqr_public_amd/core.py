@@ -40,7 +40,7 @@ SYMBOLS = [
     "spaqr_set_phase_sync", "spaqr_kernel_timing", "spaqr_kernel_stat_count", "spaqr_kernel_stat",
     "spaqr_kernel_stats_reset", "spaqr_solve_nrml_device", "spaqr_cgls", "spaqr_cgls_device",
     "spaqr_solve_transpose", "spaqr_solve_transpose_device", "spaqr_gmres_transpose_device",
-    "spaqr_csc_outer_device",
+    "spaqr_csc_outer_device", "spaqr_num_extra_sparsified", "spaqr_extra_sparsified",
 ]
 
 
@@ -62,7 +62,8 @@ def load_library(path: str = LIB_PATH):
         getattr(L, n).argtypes = [C.c_void_p, t]
     L.spaqr_set_coordinates.argtypes = [C.c_void_p, _dp, C.c_int, C.c_int]
     L.spaqr_set_tensor_coordinates.argtypes = [C.c_void_p, C.c_int, C.c_int]
-    for n in ("spaqr_rows", "spaqr_cols", "spaqr_levels", "spaqr_num_sparsified", "spaqr_num_clusters"):
+    for n in ("spaqr_rows", "spaqr_cols", "spaqr_levels", "spaqr_num_sparsified", "spaqr_num_clusters",
+              "spaqr_num_extra_sparsified"):
         getattr(L, n).argtypes = [C.c_void_p]
     L.spaqr_partition.argtypes = [C.c_void_p, C.c_int, C.c_int, _ip, _ip, _dp]
     L.spaqr_assemble.argtypes = [C.c_void_p, C.c_int, C.c_int, _ip, _ip, _dp]
@@ -90,6 +91,7 @@ def load_library(path: str = LIB_PATH):
     L.spaqr_level_stats.argtypes = [C.c_void_p, _dp]
     L.spaqr_counters.argtypes = [C.c_void_p, C.POINTER(C.c_ulonglong), _dp]
     L.spaqr_sparsified.argtypes = [C.c_void_p, _ip]
+    L.spaqr_extra_sparsified.argtypes = [C.c_void_p, _ip]
     L.spaqr_cluster_table.argtypes = [C.c_void_p, _ip]
     L.spaqr_set_phase_sync.argtypes = [C.c_void_p, C.c_int]
     L.spaqr_kernel_timing.argtypes = [C.c_void_p, C.c_int]
@@ -255,6 +257,15 @@ class Tree:
         k = L.spaqr_num_sparsified(self.h)
         out = np.zeros(5 * max(k, 1), dtype=np.int32)
         _check(L.spaqr_sparsified(self.h, _i(out)), "sparsified")
+        return out[: 5 * k].reshape(k, 5)
+
+    def extra_sparsified(self):
+        """Least squares: (ilvl, lvl, sep, extra rows before, kept) of every
+        sparsify_extra compression (tree.cpp:1260-1307), in call order."""
+        L = load_library()
+        k = L.spaqr_num_extra_sparsified(self.h)
+        out = np.zeros(5 * max(k, 1), dtype=np.int32)
+        _check(L.spaqr_extra_sparsified(self.h, _i(out)), "extra_sparsified")
         return out[: 5 * k].reshape(k, 5)
 
     def set_phase_sync(self, on):

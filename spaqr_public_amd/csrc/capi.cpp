@@ -5,41 +5,19 @@
 #include <random>
 #include <string>
 
-#include "../../include/spaqr.h"
-#include "host/engine.hpp"
-#include "host/krylov.hpp"
+#include "capi_impl.hpp"
 #include "kernels/launch.hpp"
 
 using namespace spq;
 
-struct spaqr_tree {
-  std::unique_ptr<EngineBase> e;
-  int device = 0;
-};
-struct spaqr_matrix {
-  std::unique_ptr<DevMatrix> d;
-};
+namespace spq {
+std::string& last_error() {
+  thread_local std::string msg;
+  return msg;
+}
+}  // namespace spq
 
 namespace {
-thread_local std::string g_err;
-
-template <typename F>
-int guard(F&& f) {
-  try {
-    f();
-    return SPAQR_OK;
-  } catch (const Error& e) {
-    g_err = e.what();
-    return e.code;
-  } catch (const std::bad_alloc&) {
-    g_err = "host allocation failed";
-    return SPAQR_E_OOM;
-  } catch (const std::exception& e) {
-    g_err = e.what();
-    return SPAQR_E_ARG;
-  }
-}
-
 CscMat make_csc(int m, int n, const int* colptr, const int* rowind, const double* val) {
   if (m < 0 || n < 0 || !colptr || (colptr[n] > 0 && (!rowind || !val)))
     throw Error(kErrArg, "invalid CSC arrays");
@@ -60,19 +38,11 @@ CscMat make_csc(int m, int n, const int* colptr, const int* rowind, const double
   return A;
 }
 
-EngineBase& eng(spaqr_tree* t) {
-  if (!t || !t->e) throw Error(kErrArg, "null tree");
-  return *t->e;
-}
-const EngineBase& eng(const spaqr_tree* t) {
-  if (!t || !t->e) throw Error(kErrArg, "null tree");
-  return *t->e;
-}
 }  // namespace
 
 extern "C" {
 
-const char* spaqr_last_error(void) { return g_err.c_str(); }
+const char* spaqr_last_error(void) { return last_error().c_str(); }
 
 spaqr_tree* spaqr_tree_create(int nlevels, int skip, int precision, int device) {
   spaqr_tree* t = nullptr;
@@ -354,6 +324,19 @@ int spaqr_sparsified(const spaqr_tree* t, int* out) {
   return guard([&] {
     int i = 0;
     for (auto& s : eng(t).sparsified()) {
+      out[5 * i] = s.ilvl; out[5 * i + 1] = s.lvl; out[5 * i + 2] = s.sep;
+      out[5 * i + 3] = s.before; out[5 * i + 4] = s.after;
+      ++i;
+    }
+  });
+}
+int spaqr_num_extra_sparsified(const spaqr_tree* t) {
+  return t && t->e ? (int)t->e->extra_sparsified().size() : SPAQR_E_ARG;
+}
+int spaqr_extra_sparsified(const spaqr_tree* t, int* out) {
+  return guard([&] {
+    int i = 0;
+    for (auto& s : eng(t).extra_sparsified()) {
       out[5 * i] = s.ilvl; out[5 * i + 1] = s.lvl; out[5 * i + 2] = s.sep;
       out[5 * i + 3] = s.before; out[5 * i + 4] = s.after;
       ++i;

@@ -76,6 +76,7 @@ class Engine final : public EngineBase {
   const std::vector<int>& rperm() const override { return rperm_; }
   const std::vector<LevelInfo>& level_info() const override { return info_; }
   const std::vector<SparsifyInfo>& sparsified() const override { return sprs_; }
+  const std::vector<SparsifyInfo>& extra_sparsified() const override { return extra_; }
   int precision_bytes() const override { return (int)sizeof(T); }
 
   void set_kernel_timing(int on) override { kprof_ = on != 0; }
@@ -495,8 +496,8 @@ class Engine final : public EngineBase {
       size_t ntg = 0;
       for (int q : live) ntg += 1 + jobs[q].extra.size();
       g1.reserve(ntg + live.size()); g3.reserve(ntg);
-      struct Tgt { U* C; int ld, cols; };
-      std::vector<Tgt> tg;
+      std::vector<LarfbTarget<U>> tg;
+      auto ws_alloc = [this](int64_t elems) { return (U*)ws_.alloc(sizeof(U) * (size_t)std::max<int64_t>(elems, 1)); };
       for (int q : live) {
         auto& j = jobs[q];
         const int w = std::min(nb_, std::min(j.m, j.k) - jb);
@@ -508,15 +509,7 @@ class Engine final : public EngineBase {
         if (j.k > c0) tg.push_back({j.P + jb + (int64_t)c0 * j.ld, j.ld, j.k - c0});
         for (auto& b : j.extra)
           if (b.cols > 0) tg.push_back({b.p + jb, b.ld, b.cols});
-        if (tg.empty()) continue;
-        U* W = (U*)ws_.alloc(sizeof(U) * (size_t)std::max(rows, 1) * nb_);
-        g1.push_back(gd(Vb, j.ld, Tb, nb_, W, std::max(rows, 1), rows, w, w,
-                        gemm_flags(false, true, kStructUnitLower, kStructNone), U(1), U(0)));
-        for (auto& t : tg) {
-          U* Y = (U*)ws_.alloc(sizeof(U) * (size_t)nb_ * t.cols);
-          g1.push_back(gd(Vb, j.ld, t.C, t.ld, Y, nb_, w, t.cols, rows, gemm_flags(true, false, kStructUnitLower), U(1), U(0)));
-          g3.push_back(gd(W, std::max(rows, 1), Y, nb_, t.C, t.ld, rows, t.cols, w, gemm_flags(false, false), U(-1), U(1)));
-        }
+        larfb_step_descs<U>(Vb, j.ld, Tb, nb_, rows, w, tg.data(), (int)tg.size(), ws_alloc, g1, g3);
       }
       gemm_batch(g1, "gemm_y");
       gemm_batch(g3, "gemm_c");
@@ -559,6 +552,7 @@ class Engine final : public EngineBase {
   std::vector<Op> ops_;
   std::vector<LevelInfo> info_;
   std::vector<SparsifyInfo> sprs_;
+  std::vector<SparsifyInfo> extra_;  // sparsify_extra: extra rows before / after (least squares)
   unsigned long long nnzA_ = 0, nnzR_ = 0, nnzH_ = 0, nnzQ_ = 0;
   double flops_ = 0, t_partition_ = 0, t_assemble_ = 0, t_factorize_ = 0, top_size_ = 0;
   double factor_bytes_ = 0, solve_bytes_ = 0;
@@ -571,7 +565,8 @@ class Engine final : public EngineBase {
   int qp3_max_wg_ = 64;
   Stager stager_;
   int* rank_host_ = nullptr;
-  bool factored_ = false;
+  bool partitioned_ = false, assembled_ = false, factored_ = false;
+  bool factorizing_ = false;  // set for good once factorize starts (a failed run leaves the tree unusable)
   bool kprof_ = false;
   struct KEv { std::vector<std::pair<hipEvent_t, hipEvent_t>> ev; double flops = 0, bytes = 0; };
   std::map<const int*, const char*> qp3_family_;  // problem -> timing family of its launch
@@ -626,6 +621,7 @@ class Engine final : public EngineBase {
   double* xout_ = nullptr;
   // mode 0: M^{-1}; 1: normal equations (W^T W)^{-1}; 2: M^{-T}
   void run_program(const double* b, double* x, bool prof = false, int mode = 0);
+  hipGraphExec_t capture_program(int mode);
   // M^{-T} (square trees): transposes of the backward, then forward programs
   std::vector<Stage> afwd_, abwd_;
   hipGraphExec_t graph_exec_t_ = nullptr;
@@ -663,6 +659,7 @@ class Engine final : public EngineBase {
 // ======================================================================
 template <typename T>
 void Engine<T>::partition(const CscMat& A) {  // tree.cpp:595-723
+  if (partitioned_) throw Error(kErrState, "partition: the tree is already partitioned (create a new tree)");
   const double t0 = wall();
   nrows_ = A.m;
   ncols_ = A.n;
@@ -715,12 +712,15 @@ void Engine<T>::partition(const CscMat& A) {  // tree.cpp:595-723
       bottoms_[lvl].push_back(p);
     }
   }
+  partitioned_ = true;
   t_partition_ = wall() - t0;
   if (verbose_) printf("Time to partition: %g\n", t_partition_);
 }
 
 template <typename T>
 void Engine<T>::assemble(const CscMat& A) {  // tree.cpp:725-778
+  if (!partitioned_) throw Error(kErrState, "assemble: partition has not run");
+  if (assembled_) throw Error(kErrState, "assemble: the tree is already assembled (create a new tree)");
   const double t0 = wall();
   nnzA_ = A.nnz();
   const CscMat App = csc_permute(A, &rperm_, &cperm_);
@@ -771,6 +771,7 @@ void Engine<T>::assemble(const CscMat& A) {  // tree.cpp:725-778
   SPQ_HIP(hipMemcpyAsync(dev, host.data(), sizeof(T) * host.size(), hipMemcpyHostToDevice, stream_));
   for (auto& p : pend) ed_[p.edge].a.p = dev + p.off;
   sync();
+  assembled_ = true;
   t_assemble_ = wall() - t0;
   if (verbose_) {
     auto ts = topsize();
@@ -1253,6 +1254,7 @@ void Engine<T>::sparsify_extra_level(const std::vector<int>& cs) {
   size_t q = 0;
   for (auto& x : xs) {
     const int r = x.N > 0 ? rj[q++].rank : 0;
+    extra_.push_back({ilvl_, cl_[x.c].id.self.lvl, cl_[x.c].id.self.sep, x.k, r});
     int off = 0;
     for (int e : x.apm) {
       Block& b = ed_[e].a;
@@ -1957,6 +1959,11 @@ void Engine<T>::merge_all() {  // tree.cpp:872-881, 414-475
 
 template <typename T>
 int Engine<T>::factorize() {  // tree.cpp:1309-1437
+  // the level loop consumes the assembled blocks and the cluster lists: one
+  // factorization per assembled tree
+  if (!assembled_) throw Error(kErrState, "factorize: assemble has not run");
+  if (factored_ || factorizing_) throw Error(kErrState, "factorize: the tree is already factored (create a new tree)");
+  factorizing_ = true;
   const double fstart = wall();
   for (ilvl_ = 0; ilvl_ < nlevels_; ++ilvl_) {
     LevelInfo& li = info_[ilvl_];
@@ -2021,6 +2028,12 @@ int Engine<T>::factorize() {  // tree.cpp:1309-1437
   }
   const double tbp = wall();
   build_solve_program();
+  // the apply's graph is instantiated here, so solves (and the Krylov drivers
+  // around them) allocate nothing on the device
+  if (solve_graph_ && !stage_prof_) {
+    if (square_) graph_exec_fwd_ = capture_program(0);
+    else graph_exec_nrml_ = capture_program(1);
+  }
   if (verbose_ > 1) fprintf(stderr, "solve program built in %.1f ms\n", 1e3 * (wall() - tbp));
   // a multi-workgroup kernel whose barrier wait gave up leaves the flag set
   SPQ_HIP(hipMemcpyAsync(rank_host_ + 65535, qp3_err_, sizeof(int), hipMemcpyDeviceToHost, stream_));
@@ -2684,6 +2697,21 @@ void Engine<T>::run_program(const double* b, double* x, bool prof, int mode) {
   else launch_scatter_out<T>(stream_, out_src_, out_dst_, n_out_, xbuf_, x);
 }
 
+// One solve program (mode as in run_program) captured into an executable
+// graph that reads bin_ and writes xout_.
+template <typename T>
+hipGraphExec_t Engine<T>::capture_program(int mode) {
+  hipGraph_t g = nullptr;
+  hipGraphExec_t ex = nullptr;
+  SPQ_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+  run_program(bin_, xout_, false, mode);
+  SPQ_HIP(hipStreamEndCapture(stream_, &g));
+  const hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  SPQ_HIP(e);
+  return ex;
+}
+
 template <typename T>
 void Engine<T>::solve_dev(const double* b, double* x) {
   if (!factored_) throw Error(kErrState, "solve before factorize");
@@ -2696,14 +2724,7 @@ void Engine<T>::solve_dev(const double* b, double* x) {
     SPQ_HIP(hipMemcpyAsync(x, xout_, sizeof(double) * ncols_, hipMemcpyDeviceToDevice, stream_));
     return;
   }
-  if (!graph_exec_fwd_) {
-    hipGraph_t g = nullptr;
-    SPQ_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
-    run_program(bin_, xout_);
-    SPQ_HIP(hipStreamEndCapture(stream_, &g));
-    SPQ_HIP(hipGraphInstantiate(&graph_exec_fwd_, g, nullptr, nullptr, 0));
-    hipGraphDestroy(g);
-  }
+  if (!graph_exec_fwd_) graph_exec_fwd_ = capture_program(0);
   timed("apply", 0.0, solve_bytes_, [&] { SPQ_HIP(hipGraphLaunch(graph_exec_fwd_, stream_)); });
   SPQ_HIP(hipMemcpyAsync(x, xout_, sizeof(double) * ncols_, hipMemcpyDeviceToDevice, stream_));
 }
@@ -2718,14 +2739,7 @@ void Engine<T>::solve_nrml_dev(const double* b, double* x) {
   if (stage_prof_ || !solve_graph_) {
     timed("apply", 0.0, solve_bytes_, [&] { run_program(bin_, xout_, stage_prof_, 1); });
   } else {
-    if (!graph_exec_nrml_) {
-      hipGraph_t g = nullptr;
-      SPQ_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
-      run_program(bin_, xout_, false, 1);
-      SPQ_HIP(hipStreamEndCapture(stream_, &g));
-      SPQ_HIP(hipGraphInstantiate(&graph_exec_nrml_, g, nullptr, nullptr, 0));
-      hipGraphDestroy(g);
-    }
+    if (!graph_exec_nrml_) graph_exec_nrml_ = capture_program(1);
     timed("apply", 0.0, solve_bytes_, [&] { SPQ_HIP(hipGraphLaunch(graph_exec_nrml_, stream_)); });
   }
   SPQ_HIP(hipMemcpyAsync(x, xout_, sizeof(double) * ncols_, hipMemcpyDeviceToDevice, stream_));
@@ -2741,14 +2755,7 @@ void Engine<T>::solve_t_dev(const double* b, double* x) {
   if (stage_prof_ || !solve_graph_) {
     timed("apply", 0.0, solve_bytes_, [&] { run_program(bin_, xout_, stage_prof_, 2); });
   } else {
-    if (!graph_exec_t_) {
-      hipGraph_t g = nullptr;
-      SPQ_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
-      run_program(bin_, xout_, false, 2);
-      SPQ_HIP(hipStreamEndCapture(stream_, &g));
-      SPQ_HIP(hipGraphInstantiate(&graph_exec_t_, g, nullptr, nullptr, 0));
-      hipGraphDestroy(g);
-    }
+    if (!graph_exec_t_) graph_exec_t_ = capture_program(2);
     timed("apply", 0.0, solve_bytes_, [&] { SPQ_HIP(hipGraphLaunch(graph_exec_t_, stream_)); });
   }
   SPQ_HIP(hipMemcpyAsync(x, xout_, sizeof(double) * nrows_, hipMemcpyDeviceToDevice, stream_));

@@ -35,7 +35,14 @@ struct KernelStat {
 // Interface the C ABI needs, independent of the factor precision.
 class EngineBase {
  public:
-  virtual ~EngineBase() {}
+  virtual ~EngineBase();
+  // Work vectors of the Krylov drivers, kept with the tree across calls:
+  // slot `slot` holds at least `elems` doubles (device, or pinned host for
+  // work_host).  A slot grows only when asked for more than it holds; the
+  // caller synchronises its stream before returning, so a slot is never
+  // reallocated under work in flight.
+  double* work_dev(int slot, size_t elems);
+  double* work_host(int slot, size_t elems);
   // setters (tree.cpp:14-20)
   virtual void set_tol(double t) = 0;
   virtual void set_skip(int s) = 0;
@@ -62,6 +69,7 @@ class EngineBase {
   virtual const std::vector<int>& rperm() const = 0;
   virtual const std::vector<LevelInfo>& level_info() const = 0;
   virtual const std::vector<SparsifyInfo>& sparsified() const = 0;
+  virtual const std::vector<SparsifyInfo>& extra_sparsified() const = 0;
   virtual std::vector<std::array<int, 12>> cluster_table() const = 0;
   virtual void counters(unsigned long long* nnz4, double* dbl8) const = 0;
   virtual int precision_bytes() const = 0;
@@ -71,6 +79,10 @@ class EngineBase {
   virtual void check_solve() = 0;
   virtual std::vector<KernelStat> kernel_stats() = 0;
   virtual void reset_kernel_stats() = 0;
+
+ private:
+  struct Work { double* p = nullptr; size_t cap = 0; };
+  std::vector<Work> work_dev_, work_host_;
 };
 
 EngineBase* make_engine(int nlevels, int skip, int precision_bytes, int device);

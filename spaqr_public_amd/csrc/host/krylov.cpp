@@ -2,11 +2,45 @@
 
 #include <cmath>
 #include <cstdio>
+#include <string>
 #include <vector>
 
 #include "../kernels/launch.hpp"
 
 namespace spq {
+
+EngineBase::~EngineBase() {
+  for (auto& w : work_dev_) if (w.p) (void)hipFree(w.p);
+  for (auto& w : work_host_) if (w.p) (void)hipHostFree(w.p);
+}
+
+double* EngineBase::work_dev(int slot, size_t elems) {
+  if ((int)work_dev_.size() <= slot) work_dev_.resize(slot + 1);
+  Work& w = work_dev_[slot];
+  elems = std::max<size_t>(elems, 1);
+  if (w.cap < elems) {
+    if (w.p) SPQ_HIP(hipFree(w.p));
+    w.p = nullptr;
+    w.cap = 0;
+    SPQ_HIP(hipMalloc((void**)&w.p, sizeof(double) * elems));
+    w.cap = elems;
+  }
+  return w.p;
+}
+
+double* EngineBase::work_host(int slot, size_t elems) {
+  if ((int)work_host_.size() <= slot) work_host_.resize(slot + 1);
+  Work& w = work_host_[slot];
+  elems = std::max<size_t>(elems, 1);
+  if (w.cap < elems) {
+    if (w.p) SPQ_HIP(hipHostFree(w.p));
+    w.p = nullptr;
+    w.cap = 0;
+    SPQ_HIP(hipHostMalloc((void**)&w.p, sizeof(double) * elems, hipHostMallocDefault));
+    w.cap = elems;
+  }
+  return w.p;
+}
 
 DevMatrix::~DevMatrix() {
   hipFree(rp); hipFree(ci); hipFree(v);
@@ -36,8 +70,30 @@ DevMatrix* upload_matrix(const CscMat& A, bool with_transpose) {
   return M;
 }
 
+GmresWork gmres_work_tree(EngineBase& M, int n, int restart, bool refine) {
+  // slots 0-4 and host slot 0: gmres_dev; slots 5-8 and host slot 1: the
+  // refinement loop around it
+  GmresWork w{};
+  n = std::max(n, 1);
+  restart = std::max(1, std::min(restart, n));
+  w.V = M.work_dev(0, (size_t)n * (restart + 1));
+  w.w = M.work_dev(1, n);
+  w.p0 = M.work_dev(2, n);
+  w.h = M.work_dev(3, (size_t)(restart + 2) * 2);
+  w.dpart = M.work_dev(4, dots_part_elems(restart + 1));
+  w.hh = M.work_host(0, (size_t)(restart + 2) * 2);
+  if (refine) {
+    w.r = M.work_dev(5, n);
+    w.d = M.work_dev(6, n);
+    w.h_ir = M.work_dev(7, 2);
+    w.dpart_ir = M.work_dev(8, dots_part_elems(1));
+    w.hh_ir = M.work_host(1, 2);
+  }
+  return w;
+}
+
 KrylovResult gmres_dev(EngineBase& M, const DevMatrix& A, const double* b, double* x, int iters, int restart,
-                       double tol, int verbose, bool trans) {
+                       double tol, int verbose, bool trans, const GmresWork* wk) {
   KrylovResult res;
   hipStream_t s = M.stream();
   const int n = A.m;
@@ -47,15 +103,16 @@ KrylovResult gmres_dev(EngineBase& M, const DevMatrix& A, const double* b, doubl
   const double* av = trans ? A.vt : A.v;
   restart = std::max(1, std::min(restart, n));
   const double t0 = wall();
-  double *V = nullptr, *w = nullptr, *p0 = nullptr, *h = nullptr, *hh = nullptr;
-  SPQ_HIP(hipMalloc((void**)&V, sizeof(double) * (size_t)n * (restart + 1)));
-  SPQ_HIP(hipMalloc((void**)&w, sizeof(double) * n));
-  SPQ_HIP(hipMalloc((void**)&p0, sizeof(double) * n));
-  SPQ_HIP(hipMalloc((void**)&h, sizeof(double) * (restart + 2) * 2));
-  SPQ_HIP(hipHostMalloc((void**)&hh, sizeof(double) * (restart + 2) * 2, hipHostMallocDefault));
-  double* dpart = nullptr;
-  SPQ_HIP(hipMalloc((void**)&dpart, sizeof(double) * dots_part_elems(restart + 1)));
-  auto cleanup = [&]() { hipFree(V); hipFree(w); hipFree(p0); hipFree(h); hipFree(dpart); hipHostFree(hh); };
+  // work vectors: the caller's, or the tree's own (kept across calls, so
+  // repeated calls allocate nothing)
+  const GmresWork own = wk ? GmresWork{} : gmres_work_tree(M, n, restart, false);
+  if (!wk) wk = &own;
+  double* V = wk->V;
+  double* w = wk->w;
+  double* p0 = wk->p0;
+  double* h = wk->h;
+  double* dpart = wk->dpart;
+  double* hh = wk->hh;
   auto norm_dev = [&](const double* v) {
     launch_dots(s, n, 1, v, n, v, h, dpart);
     SPQ_HIP(hipMemcpyAsync(hh, h, sizeof(double), hipMemcpyDeviceToHost, s));
@@ -72,17 +129,20 @@ KrylovResult gmres_dev(EngineBase& M, const DevMatrix& A, const double* b, doubl
   if (norm_dev(b) <= std::numeric_limits<double>::min()) {
     SPQ_HIP(hipMemsetAsync(x, 0, sizeof(double) * A.n, s));
     SPQ_HIP(hipStreamSynchronize(s));
-    cleanup();
     return res;
   }
   std::vector<double> H((size_t)(restart + 1) * restart), g(restart + 1), cs(restart), sn(restart), y(restart);
   int it = 0;
+  // the stopping quantity divides by the norm of the first preconditioned
+  // residual for the whole run, restarts included (is.cpp:133, 217)
+  double r0norm = -1.0;
   for (;;) {
     // r0 = M^{-1} (b - A x)
     SPQ_HIP(hipMemcpyAsync(p0, b, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
     launch_spmv(s, n, rp, ci, av, x, p0, -1.0, 1.0);
     precond(p0, V);
     const double beta = norm_dev(V);
+    if (r0norm < 0.0) r0norm = beta;
     if (beta == 0.0) break;
     launch_axpby(s, n, 0.0, V, 1.0 / beta, V);
     std::fill(g.begin(), g.end(), 0.0);
@@ -122,7 +182,7 @@ KrylovResult gmres_dev(EngineBase& M, const DevMatrix& A, const double* b, doubl
       col[k] = 0.0;
       g[k] = -sn[k - 1] * g[k - 1];
       g[k - 1] = cs[k - 1] * g[k - 1];
-      res.est = std::fabs(g[k]) / beta;
+      res.est = std::fabs(g[k]) / r0norm;
       if (verbose > 1) printf("%d: |Ax-b|/|b| = %3.2e <? %3.2e\n", it, res.est, tol);
       stop = (k == n || res.est < tol || it >= iters || hk == 0.0);
       if (stop) break;
@@ -141,29 +201,23 @@ KrylovResult gmres_dev(EngineBase& M, const DevMatrix& A, const double* b, doubl
   SPQ_HIP(hipStreamSynchronize(s));
   res.iters = it;
   res.t_total = wall() - t0;
-  cleanup();
   return res;
 }
 
 KrylovResult gmres_ir_dev(EngineBase& M, const DevMatrix& A, const double* b, double* x, int iters, int restart,
-                          double tol, int verbose, bool trans) {
+                          double tol, int verbose, bool trans, const GmresWork* wk) {
   KrylovResult res;
   if (trans && !A.rpt) throw Error(kErrArg, "gmres: the matrix was uploaded without its transpose");
   hipStream_t s = M.stream();
   const int n = A.m;
   const double t0 = wall();
-  double *r = nullptr, *d = nullptr, *h = nullptr, *hh = nullptr;
-  SPQ_HIP(hipMalloc((void**)&r, sizeof(double) * std::max(n, 1)));
-  SPQ_HIP(hipMalloc((void**)&d, sizeof(double) * std::max(n, 1)));
-  SPQ_HIP(hipMalloc((void**)&h, sizeof(double) * 2));
-  SPQ_HIP(hipHostMalloc((void**)&hh, sizeof(double) * 2, hipHostMallocDefault));
-  double* dpart = nullptr;
-  SPQ_HIP(hipMalloc((void**)&dpart, sizeof(double) * dots_part_elems(1)));
-  struct Free {
-    std::vector<void*> dv;
-    void* hp;
-    ~Free() { for (void* q : dv) hipFree(q); hipHostFree(hp); }
-  } fr{{r, d, h, dpart}, hh};
+  const GmresWork own = wk ? GmresWork{} : gmres_work_tree(M, n, restart, true);
+  if (!wk) wk = &own;
+  double* r = wk->r;
+  double* d = wk->d;
+  double* h = wk->h_ir;
+  double* dpart = wk->dpart_ir;
+  double* hh = wk->hh_ir;
   auto norm = [&](const double* v) {
     launch_dots(s, n, 1, v, n, v, h, dpart);
     SPQ_HIP(hipMemcpyAsync(hh, h, sizeof(double), hipMemcpyDeviceToHost, s));
@@ -192,7 +246,8 @@ KrylovResult gmres_ir_dev(EngineBase& M, const DevMatrix& A, const double* b, do
     const int left = iters - res.iters;
     // no tighter than this step needs to reach tol (with a factor 2 margin)
     const double inner_tol = std::max(kIrInnerTol, 0.5 * tol / rel);
-    KrylovResult in = gmres_dev(M, A, r, d, left, std::min(restart, left), inner_tol, verbose > 1 ? verbose : 0, trans);
+    KrylovResult in = gmres_dev(M, A, r, d, left, std::min(restart, left), inner_tol, verbose > 1 ? verbose : 0, trans,
+                                wk);
     res.iters += in.iters;
     res.n_precond += in.n_precond;
     res.t_precond += in.t_precond;
@@ -216,24 +271,17 @@ KrylovResult cgls_dev(EngineBase& M, const DevMatrix& A, const double* b, double
   const int m = A.m, n = A.n;
   if (!A.rpt) throw Error(kErrArg, "cgls: the matrix was uploaded without its transpose");
   const double t0 = wall();
-  double *r = nullptr, *nr = nullptr, *tmp = nullptr, *p = nullptr, *z = nullptr, *h = nullptr, *hh = nullptr;
-  SPQ_HIP(hipMalloc((void**)&r, sizeof(double) * std::max(m, 1)));
-  SPQ_HIP(hipMalloc((void**)&tmp, sizeof(double) * std::max(m, 1)));
-  SPQ_HIP(hipMalloc((void**)&nr, sizeof(double) * std::max(n, 1)));
-  SPQ_HIP(hipMalloc((void**)&p, sizeof(double) * std::max(n, 1)));
-  SPQ_HIP(hipMalloc((void**)&z, sizeof(double) * std::max(n, 1)));
+  // slots 9-16 and host slot 2
+  double* r = M.work_dev(9, std::max(m, 1));
+  double* tmp = M.work_dev(10, std::max(m, 1));
+  double* nr = M.work_dev(11, std::max(n, 1));
+  double* p = M.work_dev(12, std::max(n, 1));
+  double* z = M.work_dev(13, std::max(n, 1));
   const bool flexible = M.precision_bytes() == 4;
-  double* zold = nullptr;
-  SPQ_HIP(hipMalloc((void**)&zold, sizeof(double) * (flexible ? std::max(n, 1) : 1)));
-  SPQ_HIP(hipMalloc((void**)&h, sizeof(double) * 4));
-  SPQ_HIP(hipHostMalloc((void**)&hh, sizeof(double) * 4, hipHostMallocDefault));
-  double* dpart = nullptr;
-  SPQ_HIP(hipMalloc((void**)&dpart, sizeof(double) * dots_part_elems(1)));
-  struct Free {
-    std::vector<void*> d;
-    void* hp;
-    ~Free() { for (void* q : d) hipFree(q); hipHostFree(hp); }
-  } fr{{r, tmp, nr, p, z, zold, h, dpart}, hh};
+  double* zold = M.work_dev(14, flexible ? std::max(n, 1) : 1);
+  double* h = M.work_dev(15, 4);
+  double* dpart = M.work_dev(16, dots_part_elems(1));
+  double* hh = M.work_host(2, 4);
   auto dot = [&](int len, const double* u, const double* v) {
     launch_dots(s, len, 1, u, len, v, h, dpart);
     SPQ_HIP(hipMemcpyAsync(hh, h, sizeof(double), hipMemcpyDeviceToHost, s));

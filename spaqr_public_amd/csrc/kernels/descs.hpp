@@ -49,6 +49,48 @@ struct GemmDesc {
   T* work = nullptr;
 };
 
+// One block step of the Householder update of a panel's targets,
+// C <- (I - V_b T_b V_b^T)^T C = C - (V_b T_b^T)(V_b^T C): the larfb of
+// util.cpp:421-443 (side 'L', trans 'T', forward, columnwise) as batched
+// GEMM descriptors.  W = V_b T_b^T (rows x w) is formed once per panel, with
+// T_b entering as a dense factor (the panel kernel stores its strict lower
+// part as zeros); then per target Y = V_b^T C (w x cols) goes to g1 and
+// C -= W Y to g3.  g1 runs before g3.  alloc(elems) returns device scratch of
+// at least elems elements of U.
+template <typename U>
+struct LarfbTarget {
+  U* C;
+  int ld, cols;
+};
+template <typename U, typename Alloc, typename Vec>
+void larfb_step_descs(const U* Vb, int ldv, const U* Tb, int ldt, int rows, int w, const LarfbTarget<U>* tg,
+                      int ntg, Alloc&& alloc, Vec& g1, Vec& g3) {
+  if (ntg <= 0) return;
+  const int ldw = rows > 1 ? rows : 1;
+  U* W = alloc((int64_t)ldw * ldt);
+  GemmDesc<U> d{};
+  d.A = Vb; d.lda = ldv; d.B = Tb; d.ldb = ldt; d.C = W; d.ldc = ldw;
+  d.m = rows; d.n = w; d.k = w;
+  d.flags = gemm_flags(false, true, kStructUnitLower, kStructNone);
+  d.alpha = U(1); d.beta = U(0);
+  g1.push_back(d);
+  for (int t = 0; t < ntg; ++t) {
+    U* Y = alloc((int64_t)ldt * tg[t].cols);
+    GemmDesc<U> y{};
+    y.A = Vb; y.lda = ldv; y.B = tg[t].C; y.ldb = tg[t].ld; y.C = Y; y.ldc = ldt;
+    y.m = w; y.n = tg[t].cols; y.k = rows;
+    y.flags = gemm_flags(true, false, kStructUnitLower);
+    y.alpha = U(1); y.beta = U(0);
+    g1.push_back(y);
+    GemmDesc<U> c{};
+    c.A = W; c.lda = ldw; c.B = Y; c.ldb = ldt; c.C = tg[t].C; c.ldc = tg[t].ld;
+    c.m = rows; c.n = tg[t].cols; c.k = w;
+    c.flags = gemm_flags(false, false);
+    c.alpha = U(-1); c.beta = U(1);
+    g3.push_back(c);
+  }
+}
+
 // Householder panel block: factor columns [jb, jb+w) of P (m x k, ld), writing
 // reflectors below the diagonal, R on/above it, tau[j] and the nb x nb block
 // T factor of this block into Tb + jb*nb (ld nb).  The register form splits

@@ -93,7 +93,6 @@ int ktest_qr(int m, int k, double* A, double* tau, double* Tb, int nc, double* C
   const int nb = panel_nb(), kk = m < k ? m : k, nblk = (k + nb - 1) / nb;
   double *dA = dup(A, (size_t)m * k), *dt = dup<double>(nullptr, k), *dT = dup<double>(nullptr, (size_t)nb * nb * nblk);
   double* dC = dup(Cm, (size_t)m * (nc > 0 ? nc : 1));
-  double *dY = dup<double>(nullptr, (size_t)nb * (k + nc + 1)), *dZ = dup<double>(nullptr, (size_t)nb * (k + nc + 1));
   int zero = 0;
   int* ds = dup(&zero, 1);
   int* err = dup<int>(nullptr, 1);
@@ -112,22 +111,33 @@ int ktest_qr(int m, int k, double* A, double* tau, double* Tb, int nc, double* C
     const int w = (nb < kk - jb) ? nb : kk - jb, rows = m - jb;
     const double* Vb = dA + jb + (size_t)jb * m;
     const double* Tbb = dT + (size_t)jb * nb;
-    struct Tg { double* C; int cols; };
-    std::vector<Tg> tg;
+    // the engine's descriptors for this step (W = V_b T_b^T, Y = V_b^T C,
+    // C -= W Y), each list run as one batched launch as the engine does
+    std::vector<LarfbTarget<double>> tg;
     const int c0 = jb + ((nb < k - jb) ? nb : k - jb);
-    if (k > c0) tg.push_back({dA + jb + (size_t)c0 * m, k - c0});
-    if (nc > 0) tg.push_back({dC + jb, nc});
-    for (auto& t : tg) {
-      GemmDesc<double> g1{Vb, t.C, dY, m, m, nb, w, t.cols, rows, gemm_flags(true, false, kStructUnitLower), 1.0, 0.0};
-      GemmDesc<double> g2{Tbb, dY, dZ, nb, nb, nb, w, t.cols, w, gemm_flags(true, false, kStructUpper), 1.0, 0.0};
-      GemmDesc<double> g3{Vb, dZ, t.C, m, nb, m, rows, t.cols, w, gemm_flags(false, false, kStructUnitLower), -1.0, 1.0};
-      for (auto* g : {&g1, &g2, &g3}) {
-        GemmDesc<double>* dg = dup(g, 1);
-        launch_gemm<double>(0, dg, ds, 1, gemm_tiles(g->m, g->n));
-        KT(hipDeviceSynchronize());
-        (void)hipFree(dg);
-      }
+    if (k > c0) tg.push_back({dA + jb + (size_t)c0 * m, m, k - c0});
+    if (nc > 0) tg.push_back({dC + jb, m, nc});
+    std::vector<void*> scratch;
+    auto alloc = [&](int64_t elems) {
+      double* p = dup<double>(nullptr, (size_t)(elems > 0 ? elems : 1));
+      scratch.push_back(p);
+      return p;
+    };
+    std::vector<GemmDesc<double>> g1, g3;
+    larfb_step_descs<double>(Vb, m, Tbb, nb, rows, w, tg.data(), (int)tg.size(), alloc, g1, g3);
+    for (auto* gl : {&g1, &g3}) {
+      if (gl->empty()) continue;
+      std::vector<int> st(gl->size());
+      int nt = 0;
+      for (size_t i = 0; i < gl->size(); ++i) { st[i] = nt; nt += gemm_tiles((*gl)[i].m, (*gl)[i].n); }
+      GemmDesc<double>* dg = dup(gl->data(), gl->size());
+      int* dst = dup(st.data(), st.size());
+      launch_gemm<double>(0, dg, dst, (int)gl->size(), nt);
+      KT(hipDeviceSynchronize());
+      (void)hipFree(dg);
+      (void)hipFree(dst);
     }
+    for (void* p : scratch) (void)hipFree(p);
   }
   KT(hipDeviceSynchronize());
   back(A, dA, (size_t)m * k);
@@ -136,7 +146,7 @@ int ktest_qr(int m, int k, double* A, double* tau, double* Tb, int nc, double* C
   if (nc > 0) back(Cm, dC, (size_t)m * nc);
   int e = 0;
   back(&e, err, 1);
-  for (void* p : {(void*)dA, (void*)dt, (void*)dT, (void*)dC, (void*)dY, (void*)dZ, (void*)ds, (void*)err}) (void)hipFree(p);
+  for (void* p : {(void*)dA, (void*)dt, (void*)dT, (void*)dC, (void*)ds, (void*)err}) (void)hipFree(p);
   if (e) { g_err = "barrier wait timed out"; return -1; }
   return 0;
 }

@@ -185,6 +185,9 @@ def test_blocked_qr(K, oracle, mk):
         w = min(NB, kk - b * NB)
         blk = slice(b * NB, b * NB + w)
         assert rel(np.triu(Tb[:w, blk]), Tref[:w, blk]) < 1e-12
+        # the trailing update takes T_b as a dense factor (W = V_b T_b^T):
+        # its strict lower part must be stored as exact zeros
+        assert not np.any(np.tril(Tb[:w, blk], -1))
     Cref = oracle.larfb(np.asfortranarray(QR_ref[:, :kk]), oracle.larft(np.asfortranarray(QR_ref[:, :kk]), tau_ref),
                         Cm, "T")
     assert rel(C2, Cref) < 1e-12
