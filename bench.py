@@ -76,8 +76,9 @@ def cpu_info():
     return model, os.cpu_count() or 1
 
 
-def setup_tree(sq, As, n, dim, tol, nlevels, precision):
+def setup_tree(sq, As, n, dim, tol, nlevels, precision, verbose=0):
     t = sq.Tree(nlevels, 0, precision)
+    t.set_verbose(verbose)
     t.set_scale(1)
     t.set_tol(tol)
     t.set_order(1.0)
@@ -249,9 +250,12 @@ def adjoint_batch(args, count=64):
     per instance the spaQR factorization, x = A^{-1} b by GMRES and the
     gradients of ||x - x_t||^2 w.r.t. b and A's values through the adjoint
     (GMRES on A^T with M^{-T}, then dA on the pattern).  One step = all
-    instances, one after the other; the value is its wall time.  Two
+    instances, run by --workers host threads at once (each instance's tree
+    has its own HIP stream, so their kernels overlap on the GPU and their
+    host planning on the CPU); the value is the step's wall time.  Two
     instances are checked against the FP64 reference gradient (sparse LU on A
     and A^T)."""
+    from concurrent.futures import ThreadPoolExecutor
     import numpy as np
     import scipy.sparse.linalg as spl
     import torch
@@ -286,7 +290,11 @@ def adjoint_batch(args, count=64):
     for step in range(args.warmup + args.steps):
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        res = [one(item) for item in inst]
+        if args.workers > 1:
+            with ThreadPoolExecutor(max_workers=args.workers) as ex:
+                res = list(ex.map(one, inst))
+        else:
+            res = [one(item) for item in inst]
         grads = [r[0] for r in res]
         its.extend(r[1] for r in res)
         torch.cuda.synchronize()
@@ -315,7 +323,8 @@ def adjoint_batch(args, count=64):
         "value": value, "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": 1000.0 * value, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64" if precision == sq.FP64 else "f32-factor/f64-krylov", "data": "synthetic",
-        "config": {"workload": f"adjoint_batch{count}", "instances": count, "total_dofs": dofs, "tol": 1e-3,
+        "config": {"workload": f"adjoint_batch{count}", "instances": count, "workers": args.workers,
+                   "total_dofs": dofs, "tol": 1e-3,
                    "grid": "U{64..256}^2", "velocity": "U(10,1000) along (1,1)/sqrt(2)", "loss": "||x - x_t||^2",
                    "x_t": "N(0,1)", "b": "N(0,1)", "seed": 2021},
         "forward_gmres_iters_mean": float(np.mean(fw)), "adjoint_gmres_iters_mean": float(np.mean(bw)),
@@ -334,6 +343,10 @@ def main():
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0, help="threads of the CPU baseline (default: all, <= 16)")
+    ap.add_argument("--workers", type=int, default=8, help="adjoint_batch: instances processed at once (host "
+                    "threads, one HIP stream per instance)")
+    ap.add_argument("--verbose", type=int, default=0, help="tree verbosity in the timed steps (2: per-level kernel "
+                    "and host breakdown on stderr)")
     # measurement bookkeeping (no benchmark run)
     ap.add_argument("--copy-calibration", action="store_true", help="run the PMC copy calibration and exit")
     ap.add_argument("--trace-summary", nargs=2, metavar=("STATS_DIR", "OUT_MD"))
@@ -384,7 +397,7 @@ def main():
     stats_acc = {}
     M = None
     for step in range(args.warmup + args.steps):
-        t = setup_tree(sq, As, n, dim, args.tol, nlevels, precision)
+        t = setup_tree(sq, As, n, dim, args.tol, nlevels, precision, args.verbose if step >= args.warmup else 0)
         if M is None:
             M = sq.Matrix(t, As)
         timed = step >= args.warmup

@@ -158,6 +158,15 @@ int spaqr_sparsified(const spaqr_tree* t, int* out5);
  * call order: ilvl, sep lvl, sep index, extra rows before, extra rows kept. */
 int spaqr_num_extra_sparsified(const spaqr_tree* t);
 int spaqr_extra_sparsified(const spaqr_tree* t, int* out5);
+/* The factorization's profile (the reference's Profile, profile.h:13-106,
+ * which it collects but never prints) as JSON text: per level the phase
+ * times and top separator, per sparsified interface its columns before and
+ * after and its RRQR columns (neighbours), per merged cluster its size and
+ * aspect ratio, the least-squares extra-row compressions and, when kernel
+ * timing ran, the kernel families.  Writes at most buflen - 1 bytes and a
+ * NUL into buf (buf may be NULL) and returns the full length, or a negative
+ * SPAQR_E_* code.                                                          */
+int spaqr_profile_json(spaqr_tree* t, char* buf, int buflen);
 /* Cluster table: per cluster of bottoms[l] (12 ints): l, self(lvl,sep),
  * left(lvl,sep), right(lvl,sep), cstart, csize, rstart, rsize, order.      */
 int spaqr_num_clusters(const spaqr_tree* t);

@@ -40,7 +40,7 @@ SYMBOLS = [
     "spaqr_set_phase_sync", "spaqr_kernel_timing", "spaqr_kernel_stat_count", "spaqr_kernel_stat",
     "spaqr_kernel_stats_reset", "spaqr_solve_nrml_device", "spaqr_cgls", "spaqr_cgls_device",
     "spaqr_solve_transpose", "spaqr_solve_transpose_device", "spaqr_gmres_transpose_device",
-    "spaqr_csc_outer_device", "spaqr_num_extra_sparsified", "spaqr_extra_sparsified",
+    "spaqr_csc_outer_device", "spaqr_num_extra_sparsified", "spaqr_extra_sparsified", "spaqr_profile_json",
 ]
 
 
@@ -92,6 +92,7 @@ def load_library(path: str = LIB_PATH):
     L.spaqr_counters.argtypes = [C.c_void_p, C.POINTER(C.c_ulonglong), _dp]
     L.spaqr_sparsified.argtypes = [C.c_void_p, _ip]
     L.spaqr_extra_sparsified.argtypes = [C.c_void_p, _ip]
+    L.spaqr_profile_json.argtypes = [C.c_void_p, C.c_char_p, C.c_int]
     L.spaqr_cluster_table.argtypes = [C.c_void_p, _ip]
     L.spaqr_set_phase_sync.argtypes = [C.c_void_p, C.c_int]
     L.spaqr_kernel_timing.argtypes = [C.c_void_p, C.c_int]
@@ -258,6 +259,18 @@ class Tree:
         out = np.zeros(5 * max(k, 1), dtype=np.int32)
         _check(L.spaqr_sparsified(self.h, _i(out)), "sparsified")
         return out[: 5 * k].reshape(k, 5)
+
+    def profile(self):
+        """The factorization's profile (profile.h:13-106) as a dict: levels,
+        interfaces (cols before / after, neighbours, wave), merged clusters,
+        least-squares extra rows and, with kernel timing, kernel families."""
+        import json
+        L = load_library()
+        n = L.spaqr_profile_json(self.h, None, 0)
+        _check(min(n, 0), "profile_json")
+        buf = C.create_string_buffer(n + 1)
+        _check(min(L.spaqr_profile_json(self.h, buf, n + 1), 0), "profile_json")
+        return json.loads(buf.value.decode())
 
     def extra_sparsified(self):
         """Least squares: (ilvl, lvl, sep, extra rows before, kept) of every

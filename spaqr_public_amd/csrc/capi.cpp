@@ -333,6 +333,19 @@ int spaqr_sparsified(const spaqr_tree* t, int* out) {
 int spaqr_num_extra_sparsified(const spaqr_tree* t) {
   return t && t->e ? (int)t->e->extra_sparsified().size() : SPAQR_E_ARG;
 }
+int spaqr_profile_json(spaqr_tree* t, char* buf, int buflen) {
+  int n = 0;
+  const int rc = guard([&] {
+    const std::string s = eng(t).profile_json();
+    n = (int)s.size();
+    if (buf && buflen > 0) {
+      const int w = std::min(n, buflen - 1);
+      std::memcpy(buf, s.data(), (size_t)w);
+      buf[w] = '\0';
+    }
+  });
+  return rc != SPAQR_OK ? rc : n;
+}
 int spaqr_extra_sparsified(const spaqr_tree* t, int* out) {
   return guard([&] {
     int i = 0;

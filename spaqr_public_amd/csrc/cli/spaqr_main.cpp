@@ -33,7 +33,11 @@ Csc read_mm_sparse(const std::string& path) {
   std::string line;
   std::getline(f, line);
   if (line.rfind("%%MatrixMarket", 0) != 0) throw std::runtime_error("not a Matrix Market file: " + path);
-  const bool symmetric = line.find("symmetric") != std::string::npos;
+  // symmetric / hermitian mirror the off-diagonal entries, skew-symmetric
+  // mirrors them negated (mmio.hpp:200-212)
+  const bool skew = line.find("skew-symmetric") != std::string::npos;
+  const bool symmetric = !skew && (line.find("symmetric") != std::string::npos ||
+                                   line.find("hermitian") != std::string::npos);
   const bool pattern = line.find("pattern") != std::string::npos;
   if (line.find("coordinate") == std::string::npos) throw std::runtime_error("expected a coordinate matrix: " + path);
   while (std::getline(f, line))
@@ -48,6 +52,7 @@ Csc read_mm_sparse(const std::string& path) {
     if (!pattern) f >> v;
     cols[c - 1].push_back({(int)(r - 1), v});
     if (symmetric && r != c) cols[r - 1].push_back({(int)(c - 1), v});
+    if (skew && r != c) cols[r - 1].push_back({(int)(c - 1), -v});
   }
   Csc A;
   A.m = (int)m;
@@ -65,18 +70,34 @@ Csc read_mm_sparse(const std::string& path) {
   return A;
 }
 
-// Dense Matrix Market array reader (mmio.hpp:235-316): column-major values.
+// Matrix Market dense array reader (mmio.hpp:235-320): a general real
+// array, entries column by column, comment and blank lines skipped; the
+// number of entries must be rows * cols.
 std::vector<double> read_mm_dense(const std::string& path, int* rows, int* cols) {
   std::ifstream f(path);
   if (!f) throw std::runtime_error("cannot open " + path);
   std::string line;
   std::getline(f, line);
+  if (line.rfind("%%MatrixMarket", 0) != 0) throw std::runtime_error("not a Matrix Market file: " + path);
+  if (line.find("array") == std::string::npos) throw std::runtime_error("expected a dense array: " + path);
+  if (line.find("general") == std::string::npos) throw std::runtime_error("expected a general array: " + path);
+  if (line.find("pattern") != std::string::npos || line.find("complex") != std::string::npos)
+    throw std::runtime_error("expected a real array: " + path);
   while (std::getline(f, line))
     if (!line.empty() && line[0] != '%') break;
   int m = 0, n = 1;
   std::istringstream(line) >> m >> n;
-  std::vector<double> v((size_t)m * n);
-  for (auto& x : v) f >> x;
+  std::vector<double> v;
+  v.reserve((size_t)m * n);
+  while (std::getline(f, line)) {
+    if (line.empty() || line[0] == '%') continue;
+    std::istringstream ls(line);
+    double x;
+    if (ls >> x) v.push_back(x);
+  }
+  if (v.size() != (size_t)m * n)
+    throw std::runtime_error(path + ": " + std::to_string(v.size()) + " entries for a " + std::to_string(m) + " x " +
+                             std::to_string(n) + " array");
   *rows = m;
   *cols = n;
   return v;
@@ -156,7 +177,7 @@ void usage() {
 
 }  // namespace
 
-int main(int argc, char** argv) {
+static int run(int argc, char** argv) {
   // option table: long name -> (short, default)
   std::map<std::string, std::string> val = {
       {"coordinates_n", "-1"}, {"coordinates_d", "-1"}, {"hsl", "0"},        {"tol", "1e-1"},
@@ -250,7 +271,10 @@ int main(int argc, char** argv) {
     int r, c;
     std::vector<double> X = read_mm_dense(val["coordinates"], &r, &c);
     printf("Coordinate file %dx%d loaded from %s\n", r, c, val["coordinates"].c_str());
-    if (c != ncols) printf("Error: coordinate file should hold a matrix of size d x N\n");
+    if (c != ncols) {
+      printf("Error: coordinate file should hold a matrix of size d x N\n");
+      return 1;
+    }
     // stored column-major d x N == dof-major coordinates
     check(spaqr_set_coordinates(t, X.data(), r, c), "coordinates");
   } else {
@@ -297,7 +321,11 @@ int main(int argc, char** argv) {
       check(spaqr_random(nrows, 2021, b.data()), "random");
     } else {
       int r, c;
-      b = read_mm_dense(val["rhs"], &r, &c);
+      b = read_mm_dense(val["rhs"], &r, &c);  // vector_mmread (mmio.hpp:277-320)
+      if (c != 1 || r != nrows) {
+        fprintf(stderr, "--rhs: expected a %d x 1 array, got %d x %d\n", nrows, r, c);
+        return 1;
+      }
     }
     spaqr_matrix* M = spaqr_matrix_create(t, A.m, A.n, A.p.data(), A.i.data(), A.x.data());
     if (!M) { fprintf(stderr, "matrix upload failed: %s\n", spaqr_last_error()); return 2; }
@@ -333,4 +361,13 @@ int main(int argc, char** argv) {
   }
   spaqr_tree_destroy(t);
   return 0;
+}
+
+int main(int argc, char** argv) {
+  try {
+    return run(argc, argv);
+  } catch (const std::exception& e) {  // unreadable input files, bad flag values
+    fprintf(stderr, "spaqr: %s\n", e.what());
+    return 1;
+  }
 }

@@ -13,6 +13,7 @@
 #include <numeric>
 
 #include "../kernels/launch.hpp"
+#include "tuner.hpp"
 
 namespace spq {
 
@@ -36,8 +37,15 @@ class Engine final : public EngineBase {
     SPQ_HIP(hipMemset(qp3_err_, 0, 256));
     if (const char* e = std::getenv("SPAQR_QP3_MAX_WG")) qp3_max_wg_ = std::max(1, std::atoi(e));
     if (const char* e = std::getenv("SPAQR_QP3_REGISTER")) qp3_reg_ = std::atoi(e) != 0;
+    if (const char* e = std::getenv("SPAQR_QP3_LOG")) qp3_log_ = std::atoi(e) != 0;
     if (const char* e = std::getenv("SPAQR_REFL_REGISTER")) reflr_ = std::atoi(e) != 0;
     SPQ_HIP(hipDeviceGetAttribute(&num_cus_, hipDeviceAttributeMultiprocessorCount, device));
+    {
+      hipDeviceProp_t prop;
+      SPQ_HIP(hipGetDeviceProperties(&prop, device));
+      std::string arch = prop.gcnArchName;
+      global_tuner().init(arch.substr(0, arch.find(':')));
+    }
     if (const char* e = std::getenv("SPAQR_SOLVE_STAGES")) stage_prof_ = std::atoi(e) != 0;
     if (const char* e = std::getenv("SPAQR_SOLVE_GRAPH")) solve_graph_ = std::atoi(e) != 0;
     if (const char* e = std::getenv("SPAQR_QP3_RC")) {
@@ -77,6 +85,8 @@ class Engine final : public EngineBase {
   const std::vector<LevelInfo>& level_info() const override { return info_; }
   const std::vector<SparsifyInfo>& sparsified() const override { return sprs_; }
   const std::vector<SparsifyInfo>& extra_sparsified() const override { return extra_; }
+  const std::vector<MergeInfo>& merged() const override { return merges_; }
+  int skip() const override { return skip_; }
   int precision_bytes() const override { return (int)sizeof(T); }
 
   void set_kernel_timing(int on) override { kprof_ = on != 0; }
@@ -533,7 +543,10 @@ class Engine final : public EngineBase {
     T* V = nullptr;
     T* tau = nullptr;
   };
-  void rrqr_batch(std::vector<RrqrJob>& jobs);
+  // rc_force > 0: rows per lane of the register form (the autotuner's trial
+  // runs); else the tuned or closed-form choice
+  void rrqr_batch(std::vector<RrqrJob>& jobs, int rc_force = 0);
+  bool trial_ = false;  // inside an autotuner trial: no ledger, no logging
   void sparsify_extra_level(const std::vector<int>& cs);
   void build_solve_program();
 
@@ -553,6 +566,7 @@ class Engine final : public EngineBase {
   std::vector<LevelInfo> info_;
   std::vector<SparsifyInfo> sprs_;
   std::vector<SparsifyInfo> extra_;  // sparsify_extra: extra rows before / after (least squares)
+  std::vector<MergeInfo> merges_;    // clusters created by merge_all, in creation order
   unsigned long long nnzA_ = 0, nnzR_ = 0, nnzH_ = 0, nnzQ_ = 0;
   double flops_ = 0, t_partition_ = 0, t_assemble_ = 0, t_factorize_ = 0, top_size_ = 0;
   double factor_bytes_ = 0, solve_bytes_ = 0;
@@ -635,6 +649,7 @@ class Engine final : public EngineBase {
   bool stage_prof_ = false;  // SPAQR_SOLVE_STAGES=1: time every solve stage (uncaptured)
   bool solve_graph_ = true;  // SPAQR_SOLVE_GRAPH=0: launch the solve program directly (no hipGraph)
   bool qp3_reg_ = true;      // SPAQR_QP3_REGISTER=0: streaming truncated QR only
+  bool qp3_log_ = false;     // SPAQR_QP3_LOG=1: per-launch RRQR lines (with kernel timing)
   bool reflr_ = true;        // SPAQR_REFL_REGISTER=0: solve reflector stages in the LDS form only
   // per-cluster scratch of the host bookkeeping (sized to cl_, reused)
   std::vector<unsigned> mark_;  // == mark_stamp_: in the set being built
@@ -841,8 +856,9 @@ void Engine<T>::eliminate_level(const std::vector<int>& cs, int fine_wave) {
   // An FP32 tree factors its top separator in FP64 (the last level's QR):
   // the gathered panel is converted, factored in double, and the op keeps
   // the double factors for the solve.
-  // (square trees; the least-squares form keeps its R-only top in FP32)
-  const bool top64 = sizeof(T) == 4 && square_ && fine_wave < 0 && ilvl_ == nlevels_ - 1;
+  // (square and least-squares trees: the R-only top of a least-squares tree
+  // keeps its FP64 R for both triangular sweeps of the normal equations)
+  const bool top64 = sizeof(T) == 4 && fine_wave < 0 && ilvl_ == nlevels_ - 1;
   std::vector<Plan> plans;
   std::vector<Block> zero;
   for (int c : cs) {
@@ -1072,21 +1088,19 @@ void Engine<T>::scale_level(const std::vector<int>& cs) {
 // within the resident capacity.  Fills rank, V (k x k, ld k) and tau; on
 // exit Ct(0:rank, j) = triu(R(0:rank,:)) P^T for every column j.
 template <typename T>
-void Engine<T>::rrqr_batch(std::vector<RrqrJob>& jobs) {
+void Engine<T>::rrqr_batch(std::vector<RrqrJob>& jobs, int rc_force) {
   if (jobs.empty()) return;
-  int kmax = 0;
-  for (auto& j : jobs) kmax = std::max(kmax, j.k);
-  const bool lds_ok = sizeof(T) * (size_t)std::max(kmax, 1) <= (size_t)48 * 1024;
-  int* ranks_dev = (int*)scratch_.alloc(sizeof(int) * jobs.size());
-  std::vector<Qp3Desc<T>> qd;
-  std::vector<int> sync_off, nts;
-  int sync_words = 0;
+  int kmax = 0, nmax = 0;
+  for (auto& j : jobs) { kmax = std::max(kmax, j.k); nmax = std::max(nmax, j.N); }
   // rows per lane of the register form, one for the whole batch (one launch
-  // per thread count): the rc whose lanes per column, over the resident
+  // per thread count): the autotuner's choice for this batch's size class
+  // when it has one, else the rc whose lanes per column, over the resident
   // 256-thread workgroups at its register budget, cover the batch with the
   // fewest rounds (ties keep the earlier candidate)
   int rc = 32;
-  if (qp3_rc_ > 0) {
+  if (rc_force > 0) {
+    rc = rc_force;
+  } else if (qp3_rc_ > 0) {
     rc = qp3_rc_;
   } else {
     double best = -1;
@@ -1100,7 +1114,50 @@ void Engine<T>::rrqr_batch(std::vector<RrqrJob>& jobs) {
       }
       if (best < 0 || cost < best) { best = cost; rc = r; }
     }
+    if (qp3_reg_ && tol_ < 1.0) {
+      char key[96];
+      snprintf(key, sizeof key, "qp3r/%s/k%d/n%d/p%d", sizeof(T) == 8 ? "f64" : "f32", log2_bucket(kmax),
+               log2_bucket(nmax), log2_bucket((long)jobs.size()));
+      Tuner& tu = global_tuner();
+      const int cached = tu.lookup(key);
+      if (cached > 0 && qp3r_cap(256, cached) > 0) {
+        rc = cached;
+      } else if (tu.tuning()) {
+        // time every form on copies of the batch's inputs (the RRQR works
+        // in place), keep the fastest
+        std::vector<std::pair<int, double>> ms;
+        double best_ms = -1;
+        for (int r : {16, 20, 24, 28, 32, 40, 48}) {
+          if (qp3r_cap(256, r) <= 0) continue;
+          std::vector<RrqrJob> trial;
+          std::vector<CopyDesc> cp;
+          for (auto& j : jobs) {
+            RrqrJob t{alloc_scr((size_t)std::max(j.N, 1) * j.k), j.k, j.N};
+            cp.push_back(CopyDesc{j.Ct, t.Ct, std::max(j.N, 1), std::max(j.N, 1), j.N, j.k, 0, 0});
+            trial.push_back(t);
+          }
+          const bool kp = kprof_;
+          kprof_ = false;
+          copy_blocks(cp);
+          sync();
+          const double t0 = wall();
+          trial_ = true;
+          rrqr_batch(trial, r);
+          trial_ = false;
+          const double t1 = 1e3 * (wall() - t0);
+          kprof_ = kp;
+          ms.push_back({r, t1});
+          if (best_ms < 0 || t1 < best_ms) { best_ms = t1; rc = r; }
+        }
+        tu.record(key, rc, ms);
+      }
+    }
   }
+  const bool lds_ok = sizeof(T) * (size_t)std::max(kmax, 1) <= (size_t)48 * 1024;
+  int* ranks_dev = (int*)scratch_.alloc(sizeof(int) * jobs.size());
+  std::vector<Qp3Desc<T>> qd;
+  std::vector<int> sync_off, nts;
+  int sync_words = 0;
   for (size_t q = 0; q < jobs.size(); ++q) {
     RrqrJob& jb = jobs[q];
     const int N = std::max(jb.N, 1);
@@ -1134,7 +1191,7 @@ void Engine<T>::rrqr_batch(std::vector<RrqrJob>& jobs) {
     d.part_col = G ? alloc_scr((size_t)2 * nwg * jb.k) : nullptr;
     qd.push_back(d);
     const int kk = std::min(jb.k, jb.N);
-    flops_ += 4.0 * jb.k * jb.N * kk - 2.0 * (jb.k + jb.N) * kk * kk + 4.0 / 3.0 * kk * kk * kk;
+    if (!trial_) flops_ += 4.0 * jb.k * jb.N * kk - 2.0 * (jb.k + jb.N) * kk * kk + 4.0 / 3.0 * kk * kk * kk;
   }
   if (tol_ >= 1.0) {
     for (auto& j : jobs) j.rank = 0;  // util.cpp:798-799
@@ -1176,6 +1233,10 @@ void Engine<T>::rrqr_batch(std::vector<RrqrJob>& jobs) {
     if (!single.d.empty()) groups.push_back(single);
     if (!cur.d.empty()) groups.push_back(cur);
   }
+  // SPAQR_QP3_LOG=1 with kernel timing: one stderr line per launch (problem
+  // shapes, ranks and device time), for tuning the launch forms
+  struct LaunchLog { const char* fam; size_t ev; std::vector<const int*> ranks; int nd, nwg, kmax, nmax, wgmax; };
+  std::vector<LaunchLog> llog;
   for (auto& gq : groups) {
     std::vector<int> cnt, st;
     double qby = 0;
@@ -1202,12 +1263,36 @@ void Engine<T>::rrqr_batch(std::vector<RrqrJob>& jobs) {
                  : launch_qp3<T>(stream_, dqd, dst, (int)gq.d.size(), nt, kmax, gq.coop);
     });
     if (rc != 0) throw Error(kErrHip, std::string("qp3 launch failed: ") + hipGetErrorString((hipError_t)rc));
+    if (qp3_log_ && kprof_ && !trial_) {
+      LaunchLog L{fam, kev_[fam].ev.size() - 1, {}, (int)gq.d.size(), nt, 0, 0, 0};
+      for (auto& d : gq.d) {
+        L.ranks.push_back(d.rank_out);
+        L.kmax = std::max(L.kmax, d.k);
+        L.nmax = std::max(L.nmax, d.N);
+        L.wgmax = std::max(L.wgmax, d.nwg);
+      }
+      llog.push_back(L);
+    }
   }
   SPQ_HIP(hipMemcpyAsync(rank_host_, ranks_dev, sizeof(int) * jobs.size(), hipMemcpyDeviceToHost, stream_));
   SPQ_HIP(hipMemcpyAsync(rank_host_ + 65535, qp3_err_, sizeof(int), hipMemcpyDeviceToHost, stream_));
   sync();
   if (rank_host_[65535] != 0) throw Error(kErrNumeric, "qp3: an inter-workgroup barrier wait timed out");
   for (size_t q = 0; q < jobs.size(); ++q) jobs[q].rank = rank_host_[q];
+  for (auto& L : llog) {
+    float ms = 0;
+    auto& e = kev_[L.fam].ev[L.ev];
+    SPQ_HIP(hipEventElapsedTime(&ms, e.first, e.second));
+    long rsum = 0;
+    int rmax = 0;
+    for (const int* r : L.ranks) {
+      const int v = rank_host_[r - ranks_dev];
+      rsum += v;
+      rmax = std::max(rmax, v);
+    }
+    fprintf(stderr, "qp3 lvl %d %-12s problems %5d wgs %5d max(k %5d N %6d nwg %4d rank %4d) sum rank %7ld ms %.3f\n",
+            ilvl_, L.fam, L.nd, L.nwg, L.kmax, L.nmax, L.wgmax, rmax, rsum, ms);
+  }
   if (kprof_) {
     // truncated RRQR: 4kNr - 2(k+N)r^2 + 4r^3/3 + 2kN (initial norms)
     for (size_t q = 0; q < jobs.size(); ++q) {
@@ -1478,6 +1563,10 @@ void Engine<T>::sparsify_level(const std::vector<int>& cs) {
       ed_[es].a.cols = rank;
       recs[slot[c]] = {ilvl_, cl_[c].id.self.lvl, cl_[c].id.self.sep, before, rank};
     }
+    for (auto& jb : jobs) {  // profile: the RRQR's columns (neighbour columns) and wave
+      recs[slot[jb.c]].nbrs = jb.N;
+      recs[slot[jb.c]].wave = w;
+    }
     copy_blocks(scatter);
     if (!lf.empty()) {
       std::vector<int> st;
@@ -1728,6 +1817,10 @@ void Engine<T>::sparsify0_level(const std::vector<int>& cs) {
       fines.push_back(cf);
       recs[slot[c]] = {ilvl_, cl_[c].id.self.lvl, cl_[c].id.self.sep, k, rank};
     }
+    for (auto& jb : jobs) {  // profile: the RRQR's columns and wave
+      recs[slot[jb.c]].nbrs = jb.N;
+      recs[slot[jb.c]].wave = w;
+    }
     copy_blocks(vcopy);
     if (!lf.empty()) {
       std::vector<int> st;
@@ -1924,6 +2017,7 @@ void Engine<T>::merge_all() {  // tree.cpp:872-881, 414-475
     for (int s : cl_[snew].children) { cl_[s].rpos = rs; cl_[s].cpos = cs; rs += cl_[s].rsize; cs += cl_[s].csize; }
     Clu& p = cl_[snew];
     p.rsize = rs; p.csize = cs; p.rsize_org = rs; p.csize_org = cs;
+    merges_.push_back({ilvl_, p.id.self.lvl, p.id.self.sep, rs, cs});
     alloc_x(snew, rs);
     Op o;
     o.kind = OpMerge; o.level = ilvl_; o.c = snew;
@@ -2102,9 +2196,12 @@ void Engine<T>::build_solve_program() {
       x64_len = std::max<int64_t>(x64_len, o.m);
     }
   if (x64_len > 0) SPQ_HIP(hipMalloc((void**)&x64_, sizeof(double) * x64_len));
-  auto qr64_stages = [&](const Op& o, int trans, bool bwd, std::vector<HStage>& out) {
+  // xb: the x buffer the op's segments live in (xbuf_, or xtbuf_ for the
+  // tops program of the normal equations)
+  auto qr64_stages = [&](const Op& o, int trans, bool bwd, std::vector<HStage>& out, T* xb) {
     HStage a = stage(6, 0), b = stage(bwd ? 5 : 4, 0), c = stage(7, 0);
     int64_t off = 0;
+    T* const xbuf_ = xb;  // the conversions address this buffer
     if (!bwd) {
       for (auto& sg : o.segs) {
         a.cvt.push_back(CopyDesc{xbuf_ + cl_[sg.first].xoff, x64_ + off, sg.second, sg.second, sg.second, 1, 0, 0});
@@ -2255,7 +2352,7 @@ void Engine<T>::build_solve_program() {
         if (wv[i] == w && !L.qr[i]->V64) { st.refl.push_back(refl_of(*L.qr[i], 1)); st.mmax = std::max(st.mmax, L.qr[i]->m); }
       push(hf, std::move(st));
       for (size_t i = 0; i < L.qr.size(); ++i)
-        if (wv[i] == w && L.qr[i]->V64) qr64_stages(*L.qr[i], 1, false, hf);
+        if (wv[i] == w && L.qr[i]->V64) qr64_stages(*L.qr[i], 1, false, hf, xbuf_);
     }
     HStage sra = stage(2, 0);
     for (auto o : L.ra) reassign(o, sra, true);
@@ -2319,7 +2416,7 @@ void Engine<T>::build_solve_program() {
     push(hb, std::move(sqrr));
     push(hb, std::move(sqr));
     for (auto o : L.qr)
-      if (o->V64) qr64_stages(*o, 0, true, hb);
+      if (o->V64) qr64_stages(*o, 0, true, hb, xbuf_);
   }
   if (!square_) {
     // ---- normal equations (tree.cpp:1471-1517): tops forward on xt, tops
@@ -2328,9 +2425,12 @@ void Engine<T>::build_solve_program() {
     for (int lvl = 0; lvl < nlevels_; ++lvl) {
       LevelOps L = level_ops(lvl);
       HStage sqr = stage(1, 1), sqrr = stage(3, 1);  // R^{-T}, then x_n -= A_n^T x_c (toperations.cpp:12-24)
-      for (auto o : L.qr) sqr.bwd.push_back(bwd_of(*o, 1, sqrr));
+      for (auto o : L.qr)
+        if (!o->V64) sqr.bwd.push_back(bwd_of(*o, 1, sqrr));
       push(htf, std::move(sqr));
       push(htf, std::move(sqrr));
+      for (auto o : L.qr)  // FP64 top of an FP32 tree: R^{-T} in double on xt
+        if (o->V64) qr64_stages(*o, 1, true, htf, xtbuf_);
       HStage ssc = stage(1, 1);
       for (auto o : L.sc) ssc.bwd.push_back(scaled_of(*o, 1));
       push(htf, std::move(ssc));

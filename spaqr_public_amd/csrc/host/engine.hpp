@@ -22,6 +22,12 @@ struct LevelInfo {
 
 struct SparsifyInfo {
   int ilvl, lvl, sep, before, after;
+  int nbrs = 0;  // columns of the interface's RRQR matrix (its neighbours' columns, tree.cpp:1052)
+  int wave = 0;  // dependency wave within the level
+};
+// A cluster created by merge_all (tree.cpp:414-434): sizes after the merge.
+struct MergeInfo {
+  int ilvl, lvl, sep, rows, cols;
 };
 
 // Per kernel family: launches, device time (HIP events on the engine stream),
@@ -70,6 +76,14 @@ class EngineBase {
   virtual const std::vector<LevelInfo>& level_info() const = 0;
   virtual const std::vector<SparsifyInfo>& sparsified() const = 0;
   virtual const std::vector<SparsifyInfo>& extra_sparsified() const = 0;
+  virtual const std::vector<MergeInfo>& merged() const = 0;
+  virtual int skip() const = 0;
+  // The factorization's profile as JSON (profile.h:13-106): per level the
+  // phase times and top separator, per sparsified interface its columns
+  // before / after, RRQR columns (neighbours) and wave, per merged cluster
+  // its size and aspect ratio, the least-squares extra-row compressions and,
+  // when kernel timing ran, the kernel families' device time.
+  std::string profile_json();
   virtual std::vector<std::array<int, 12>> cluster_table() const = 0;
   virtual void counters(unsigned long long* nnz4, double* dbl8) const = 0;
   virtual int precision_bytes() const = 0;

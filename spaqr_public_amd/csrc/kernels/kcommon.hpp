@@ -259,17 +259,29 @@ template <typename T>
 __device__ __forceinline__ T agent_load(const T* p) {
   return __hip_atomic_load(const_cast<T*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+// Ordering.  The handed-over values and the counters are all agent-scope
+// atomics, which live at the device's coherence point.  The hardware issues
+// a wave's memory operations in program order and does not speculate past
+// the spin's branch; what the compiler could reorder (relaxed atomics to
+// different addresses) is pinned by compiler-only fences: no store moves
+// below the arrival's s_waitcnt, no load above the end of the spin.
 __device__ __forceinline__ bool barrier_wait_rx(unsigned int* ctr, unsigned int target, int* err) {
   unsigned int spins = 0;
+  bool ok = true;
   while (__hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
     __builtin_amdgcn_s_sleep(2);
     ++spins;
-    if ((spins & 255u) == 0 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return false;
-    if (spins > (1u << 20)) { atomicOr(err, 1); return false; }
+    if ((spins & 255u) == 0 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+      ok = false;
+      break;
+    }
+    if (spins > (1u << 20)) { atomicOr(err, 1); ok = false; break; }
   }
-  return true;
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  return ok;
 }
 __device__ __forceinline__ void group_barrier_rx(unsigned int* ctr, unsigned int target, int* err) {
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
   __builtin_amdgcn_s_waitcnt(0);  // this wave's agent stores have completed
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -280,6 +292,7 @@ __device__ __forceinline__ void group_barrier_rx(unsigned int* ctr, unsigned int
 }
 // Two-level form (fan-in counters as in group_barrier2).
 __device__ __forceinline__ void group_barrier2_rx(unsigned int* sync, int g, int nwg, unsigned int phase, int* err) {
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
   __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
   if (threadIdx.x == 0) {

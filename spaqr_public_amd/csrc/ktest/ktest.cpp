@@ -39,19 +39,35 @@ extern "C" {
 
 const char* ktest_last_error() { return g_err.c_str(); }
 
-// C = alpha op(A) op(B) + beta C (single problem, double)
-int ktest_gemm(int m, int n, int k, const double* A, int lda, int acols, const double* B, int ldb, int bcols,
-               double* Cm, int ldc, int flags, double alpha, double beta) {
-  double *dA = dup(A, (size_t)lda * acols), *dB = dup(B, (size_t)ldb * bcols), *dC = dup(Cm, (size_t)ldc * n);
-  GemmDesc<double> g{dA, dB, dC, lda, ldb, ldc, m, n, k, flags, alpha, beta};
-  GemmDesc<double>* dg = dup(&g, 1);
+}  // extern "C"
+
+namespace {
+// C = alpha op(A) op(B) + beta C (single problem)
+template <typename X>
+int gemm_one(int m, int n, int k, const X* A, int lda, int acols, const X* B, int ldb, int bcols, X* Cm, int ldc,
+             int flags, X alpha, X beta) {
+  X *dA = dup(A, (size_t)lda * acols), *dB = dup(B, (size_t)ldb * bcols), *dC = dup(Cm, (size_t)ldc * n);
+  GemmDesc<X> g{dA, dB, dC, lda, ldb, ldc, m, n, k, flags, alpha, beta};
+  GemmDesc<X>* dg = dup(&g, 1);
   int tiles = gemm_tiles(m, n), zero = 0;
   int* ds = dup(&zero, 1);
-  launch_gemm<double>(0, dg, ds, 1, tiles);
+  launch_gemm<X>(0, dg, ds, 1, tiles);
   KT(hipDeviceSynchronize());
   back(Cm, dC, (size_t)ldc * n);
   (void)hipFree(dA); (void)hipFree(dB); (void)hipFree(dC); (void)hipFree(dg); (void)hipFree(ds);
   return 0;
+}
+}  // namespace
+
+extern "C" {
+
+int ktest_gemm(int m, int n, int k, const double* A, int lda, int acols, const double* B, int ldb, int bcols,
+               double* Cm, int ldc, int flags, double alpha, double beta) {
+  return gemm_one<double>(m, n, k, A, lda, acols, B, ldb, bcols, Cm, ldc, flags, alpha, beta);
+}
+int ktest_gemm_f32(int m, int n, int k, const float* A, int lda, int acols, const float* B, int ldb, int bcols,
+                   float* Cm, int ldc, int flags, float alpha, float beta) {
+  return gemm_one<float>(m, n, k, A, lda, acols, B, ldb, bcols, Cm, ldc, flags, alpha, beta);
 }
 
 // Same product split along K into ksplit chunks (partials + ordered reduce).

@@ -144,6 +144,17 @@ def config_matrix(name: str, **kw) -> sp.csc_matrix:
     raise KeyError(name)
 
 
+def write_mm_array(path: str, X) -> None:
+    """Matrix Market dense array writer (general, real): the header, the
+    shape, then the entries column by column, one per line (the layout
+    mmio.hpp:235-320 reads)."""
+    X = np.atleast_2d(np.asarray(X, dtype=np.float64))
+    with open(path, "w") as f:
+        f.write("%%MatrixMarket matrix array real general\n")
+        f.write(f"{X.shape[0]} {X.shape[1]}\n")
+        np.savetxt(f, X.ravel(order="F"), fmt="%.17g")
+
+
 def write_mm(path: str, A: sp.spmatrix) -> None:
     """Matrix Market coordinate writer (general, real), 1-based indices."""
     A = A.tocoo()

@@ -29,6 +29,9 @@ def K(ktest):
     ktest.ktest_gemm.argtypes = [C.c_int, C.c_int, C.c_int, dp, C.c_int, C.c_int, dp, C.c_int, C.c_int, dp, C.c_int,
                                  C.c_int, C.c_double, C.c_double]
     ktest.ktest_gemm_split.argtypes = ktest.ktest_gemm.argtypes + [C.c_int]
+    fp = C.POINTER(C.c_float)
+    ktest.ktest_gemm_f32.argtypes = [C.c_int, C.c_int, C.c_int, fp, C.c_int, C.c_int, fp, C.c_int, C.c_int, fp,
+                                     C.c_int, C.c_int, C.c_float, C.c_float]
     ktest.ktest_smin.argtypes = [C.c_int, dp, C.c_int, dp]
     ktest.ktest_qr.argtypes = [C.c_int, C.c_int, dp, dp, dp, C.c_int, dp]
     ktest.ktest_larft.argtypes = [C.c_int, C.c_int, dp, dp, dp]
@@ -101,6 +104,30 @@ def test_gemm(K, mnk, ta, tb, sa, sb):
     flags = ta | (tb << 1) | (sa << 4) | (sb << 6)
     assert K.ktest_gemm(m, n, k, d(A), lda, Ac, d(B), ldb, Bc, d(Cm), ldc, flags, alpha, beta) == 0
     assert rel(Cm[:m, :n], ref) < 1e-13
+
+
+# FP32 matrix-core path (the mixed form's factorization), every tile shape:
+# 32 x 128 (m <= 32), 128 x 32 (n <= 32) and 64 x 64, against an FP64 product
+# of the same FP32 inputs
+@pytest.mark.parametrize("ta,tb,sa,sb", [(0, 0, 0, 0), (1, 0, 1, 0), (0, 1, 0, 0), (1, 1, 3, 2)])
+@pytest.mark.parametrize("mnk", [(1, 1, 1), (32, 300, 700), (65, 17, 130), (200, 129, 33), (7, 300, 64)])
+def test_gemm_f32(K, mnk, ta, tb, sa, sb):
+    m, n, k = mnk
+    rng = np.random.default_rng(m * 5 + n)
+    Ar, Ac = (k, m) if ta else (m, k)
+    Br, Bc = (n, k) if tb else (k, n)
+    lda, ldb, ldc = Ar + 1, Br + 3, m + 2
+    A = np.asfortranarray(rng.standard_normal((lda, Ac)).astype(np.float32))
+    B = np.asfortranarray(rng.standard_normal((ldb, Bc)).astype(np.float32))
+    Cm = np.asfortranarray(rng.standard_normal((ldc, n)).astype(np.float32))
+    alpha, beta = -1.0, 1.0
+    Am = struct_mask(A[:Ar, :Ac].astype(np.float64), sa)
+    Bm = struct_mask(B[:Br, :Bc].astype(np.float64), sb)
+    ref = alpha * (Am.T if ta else Am) @ (Bm.T if tb else Bm) + beta * Cm[:m, :n].astype(np.float64)
+    flags = ta | (tb << 1) | (sa << 4) | (sb << 6)
+    f = lambda a: a.ctypes.data_as(C.POINTER(C.c_float))
+    assert K.ktest_gemm_f32(m, n, k, f(A), lda, Ac, f(B), ldb, Bc, f(Cm), ldc, flags, alpha, beta) == 0
+    assert rel(Cm[:m, :n].astype(np.float64), ref) < 2e-6 * np.sqrt(k)
 
 
 # split-K: K cut into ksplit chunks (some empty when k < 16 * ksplit), partial
