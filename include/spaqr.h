@@ -105,6 +105,17 @@ int spaqr_gmres(spaqr_tree* t, const spaqr_matrix* A, const double* b, double* x
 int spaqr_gmres_device(spaqr_tree* t, const spaqr_matrix* A, const double* b, double* x, int iters, int restart,
                        double tol, int verbose, int* iters_out, double* est_out);
 
+/* spaqr_gmres_device without allocation.  No counterpart in the reference:
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, serving loops, harnesses that forbid hipMalloc).  Same arguments,
+ * iterations and results as spaqr_gmres_device; the Krylov basis and partial
+ * sums live in a fixed 64 MiB device array that every GPU the library runs
+ * on holds until the process ends, so n * (restart + 1) plus O(n) work must
+ * fit in 2^23 doubles and restart <= 1024 (else SPAQR_E_ARG before any GPU
+ * work).  Calls that overlap on one GPU race on that array.               */
+int spaqr_gmres_device_noalloc(spaqr_tree* t, const spaqr_matrix* A, const double* b, double* x, int iters,
+                               int restart, double tol, int verbose, int* iters_out, double* est_out);
+
 /* Adjoint of x = A^{-1} b (differentiable solve).  No counterpart in the
  * reference, whose tOperations (toperations.h:14-20) transpose only the
  * R-only least-squares factors: these transpose the full square factors.

@@ -41,6 +41,7 @@ SYMBOLS = [
     "spaqr_kernel_stats_reset", "spaqr_solve_nrml_device", "spaqr_cgls", "spaqr_cgls_device",
     "spaqr_solve_transpose", "spaqr_solve_transpose_device", "spaqr_gmres_transpose_device",
     "spaqr_csc_outer_device", "spaqr_num_extra_sparsified", "spaqr_extra_sparsified", "spaqr_profile_json",
+    "spaqr_gmres_device_noalloc",
 ]
 
 
@@ -84,6 +85,7 @@ def load_library(path: str = LIB_PATH):
     L.spaqr_solve_transpose.argtypes = [C.c_void_p, _dp, _dp]
     L.spaqr_solve_transpose_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
     L.spaqr_gmres_transpose_device.argtypes = L.spaqr_gmres_device.argtypes
+    L.spaqr_gmres_device_noalloc.argtypes = L.spaqr_gmres_device.argtypes
     L.spaqr_csc_outer_device.argtypes = [C.c_void_p, C.c_double, C.c_void_p, C.c_void_p, C.c_void_p]
     L.spaqr_colscale.argtypes = [C.c_int, C.c_int, _ip, _ip, _dp, _dp]
     L.spaqr_random.argtypes = [C.c_int, C.c_int, _dp]
