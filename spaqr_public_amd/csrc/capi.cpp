@@ -113,9 +113,9 @@ int spaqr_solve(spaqr_tree* t, const double* b, double* x) {
   return guard([&] {
     EngineBase& e = eng(t);
     double *db = nullptr, *dx = nullptr;
-    SPQ_HIP(hipMalloc((void**)&db, sizeof(double) * std::max(e.rows(), 1)));
-    SPQ_HIP(hipMalloc((void**)&dx, sizeof(double) * std::max(e.cols(), 1)));
-    struct F { double* p; ~F() { hipFree(p); } } fb{db}, fx{dx};
+    SPQ_HIP(dev_malloc((void**)&db, sizeof(double) * std::max(e.rows(), 1)));
+    SPQ_HIP(dev_malloc((void**)&dx, sizeof(double) * std::max(e.cols(), 1)));
+    struct F { double* p; ~F() { dev_free(p); } } fb{db}, fx{dx};
     SPQ_HIP(hipMemcpy(db, b, sizeof(double) * e.rows(), hipMemcpyHostToDevice));
     e.solve_dev(db, dx);
     SPQ_HIP(hipStreamSynchronize(e.stream()));
@@ -127,9 +127,9 @@ int spaqr_solve_nrml(spaqr_tree* t, const double* b, double* x) {
   return guard([&] {
     EngineBase& e = eng(t);
     double *db = nullptr, *dx = nullptr;
-    SPQ_HIP(hipMalloc((void**)&db, sizeof(double) * std::max(e.cols(), 1)));
-    SPQ_HIP(hipMalloc((void**)&dx, sizeof(double) * std::max(e.cols(), 1)));
-    struct F { double* p; ~F() { hipFree(p); } } fb{db}, fx{dx};
+    SPQ_HIP(dev_malloc((void**)&db, sizeof(double) * std::max(e.cols(), 1)));
+    SPQ_HIP(dev_malloc((void**)&dx, sizeof(double) * std::max(e.cols(), 1)));
+    struct F { double* p; ~F() { dev_free(p); } } fb{db}, fx{dx};
     SPQ_HIP(hipMemcpy(db, b, sizeof(double) * e.cols(), hipMemcpyHostToDevice));
     e.solve_nrml_dev(db, dx);
     SPQ_HIP(hipStreamSynchronize(e.stream()));
@@ -202,8 +202,8 @@ int spaqr_solve_transpose(spaqr_tree* t, const double* b, double* x) {
     if (!b || !x) throw Error(kErrArg, "null vector");
     const int n = e.rows();
     double* d = nullptr;
-    SPQ_HIP(hipMalloc((void**)&d, sizeof(double) * std::max(n, 1)));
-    struct F { double* p; ~F() { hipFree(p); } } fd{d};
+    SPQ_HIP(dev_malloc((void**)&d, sizeof(double) * std::max(n, 1)));
+    struct F { double* p; ~F() { dev_free(p); } } fd{d};
     SPQ_HIP(hipMemcpy(d, b, sizeof(double) * n, hipMemcpyHostToDevice));
     e.solve_t_dev(d, d);
     SPQ_HIP(hipStreamSynchronize(e.stream()));
@@ -241,9 +241,9 @@ int spaqr_cgls(spaqr_tree* t, const spaqr_matrix* A, const double* b, double* x,
     const int n = A->d->n, m = A->d->m;
     if (m != e.rows() || n != e.cols()) throw Error(kErrArg, "cgls needs the matrix of the tree");
     double *db = nullptr, *dx = nullptr;
-    SPQ_HIP(hipMalloc((void**)&db, sizeof(double) * std::max(m, 1)));
-    SPQ_HIP(hipMalloc((void**)&dx, sizeof(double) * std::max(n, 1)));
-    struct F { double* p; ~F() { hipFree(p); } } fb{db}, fx{dx};
+    SPQ_HIP(dev_malloc((void**)&db, sizeof(double) * std::max(m, 1)));
+    SPQ_HIP(dev_malloc((void**)&dx, sizeof(double) * std::max(n, 1)));
+    struct F { double* p; ~F() { dev_free(p); } } fb{db}, fx{dx};
     SPQ_HIP(hipMemcpy(db, b, sizeof(double) * m, hipMemcpyHostToDevice));
     SPQ_HIP(hipMemcpy(dx, x, sizeof(double) * n, hipMemcpyHostToDevice));
     KrylovResult r = cgls_dev(e, *A->d, db, dx, iters, tol, verbose);
@@ -260,9 +260,9 @@ int spaqr_gmres(spaqr_tree* t, const spaqr_matrix* A, const double* b, double* x
     if (!A || !A->d) throw Error(kErrArg, "null matrix");
     const int n = A->d->n, m = A->d->m;
     double *db = nullptr, *dx = nullptr;
-    SPQ_HIP(hipMalloc((void**)&db, sizeof(double) * std::max(m, 1)));
-    SPQ_HIP(hipMalloc((void**)&dx, sizeof(double) * std::max(n, 1)));
-    struct F { double* p; ~F() { hipFree(p); } } fb{db}, fx{dx};
+    SPQ_HIP(dev_malloc((void**)&db, sizeof(double) * std::max(m, 1)));
+    SPQ_HIP(dev_malloc((void**)&dx, sizeof(double) * std::max(n, 1)));
+    struct F { double* p; ~F() { dev_free(p); } } fb{db}, fx{dx};
     SPQ_HIP(hipMemcpy(db, b, sizeof(double) * m, hipMemcpyHostToDevice));
     SPQ_HIP(hipMemcpy(dx, x, sizeof(double) * n, hipMemcpyHostToDevice));
     if (A->d->m != A->d->n || A->d->m != e.rows()) throw Error(kErrArg, "gmres needs the square matrix of the tree");

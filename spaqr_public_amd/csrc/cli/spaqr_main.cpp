@@ -172,7 +172,8 @@ void usage() {
          "      --useEigenQR arg       Eigen baseline (not available)\n"
          "      --useCholesky arg      Eigen baseline (not available)\n"
          "      --precision arg        fp64 or fp32 factor storage (default: fp64)\n"
-         "      --device arg           HIP device (default: 0)\n");
+         "      --device arg           HIP device (default: 0)\n"
+         "      --profile arg          Write the factorization profile (JSON) to this file\n");
 }
 
 }  // namespace
@@ -188,7 +189,8 @@ static int run(int argc, char** argv) {
                                                  {"d", "coordinates_d"}, {"t", "tol"}, {"i", "iterations"}};
   const std::vector<std::string> known = {"matrix", "lvl", "coordinates", "coordinates_n", "coordinates_d", "hsl",
                                           "tol", "skip", "order", "scale", "iterations", "rhs", "res",
-                                          "useEigenLSCG", "useEigenQR", "useCholesky", "precision", "device"};
+                                          "useEigenLSCG", "useEigenQR", "useCholesky", "precision", "device",
+                                          "profile"};
   std::map<std::string, int> given;
   for (int a = 1; a < argc; ++a) {
     std::string s = argv[a], key, v;
@@ -358,6 +360,17 @@ static int run(int argc, char** argv) {
       printf("<<<<CGLS=%d\n", it);
     }
     spaqr_matrix_destroy(M);
+  }
+  if (given.count("profile")) {
+    const int len = spaqr_profile_json(t, nullptr, 0);
+    if (len < 0) check(len, "profile_json");
+    std::string js((size_t)len + 1, '\0');
+    check(spaqr_profile_json(t, &js[0], len + 1) < 0 ? SPAQR_E_ARG : SPAQR_OK, "profile_json");
+    js.resize((size_t)len);
+    std::ofstream pf(val["profile"]);
+    if (!pf) throw std::runtime_error("cannot write " + val["profile"]);
+    pf << js;
+    printf("Profile written to %s\n", val["profile"].c_str());
   }
   spaqr_tree_destroy(t);
   return 0;

@@ -4,6 +4,11 @@
 
 namespace spq {
 
+std::recursive_mutex& capture_lock() {
+  static std::recursive_mutex m;
+  return m;
+}
+
 long Arena::n_chunks = 0;
 double Arena::t_chunks = 0;
 
@@ -22,7 +27,7 @@ void* Arena::alloc(size_t bytes, size_t align) {
   size_t sz = bytes > chunk_ ? (bytes + align) : chunk_;
   void* p = nullptr;
   const double t0 = wall();
-  hipError_t e = hipMalloc(&p, sz);
+  hipError_t e = dev_malloc(&p, sz);
   t_chunks += wall() - t0;
   ++n_chunks;
   if (e != hipSuccess)
@@ -43,7 +48,7 @@ void Arena::rewind() {
 }
 
 void Arena::release() {
-  for (auto& c : chunks_) hipFree(c.p);
+  for (auto& c : chunks_) dev_free(c.p);
   chunks_.clear();
   cur_ = 0;
   used_ = 0;
@@ -51,15 +56,15 @@ void Arena::release() {
 }
 
 Stager::~Stager() {
-  if (host_) hipHostFree(host_);
-  if (up_host_) hipHostFree(up_host_);
-  if (dev_) hipFree(dev_);
+  host_free(host_);
+  host_free(up_host_);
+  dev_free(dev_);
 }
 
 void Stager::upload_raw(void* dst, const void* src, size_t bytes) {
   constexpr size_t kUpCap = size_t(32) << 20;
   if (!up_host_) {
-    SPQ_HIP(hipHostMalloc((void**)&up_host_, kUpCap, hipHostMallocDefault));
+    SPQ_HIP(host_malloc((void**)&up_host_, kUpCap));
     up_cap_ = kUpCap;
     up_off_ = 0;
   }
@@ -99,8 +104,8 @@ void Stager::flush_uploads() {
 void Stager::init(hipStream_t s, size_t bytes) {
   stream_ = s;
   cap_ = bytes;
-  SPQ_HIP(hipHostMalloc((void**)&host_, bytes, hipHostMallocDefault));
-  SPQ_HIP(hipMalloc((void**)&dev_, bytes));
+  SPQ_HIP(host_malloc((void**)&host_, bytes));
+  SPQ_HIP(dev_malloc((void**)&dev_, bytes));
   off_ = 0;
 }
 

@@ -5,6 +5,7 @@
 #include <stdint.h>
 
 #include <chrono>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -24,6 +25,30 @@ enum : int { kOk = 0, kErrHip = -1, kErrArg = -2, kErrNumeric = -3, kErrOom = -4
       throw ::spq::Error(::spq::kErrHip, std::string("HIP error ") + hipGetErrorString(e_) + " at " + \
                                              __FILE__ + ":" + std::to_string(__LINE__));            \
   } while (0)
+
+// Stream capture and device-memory management are serialised process-wide:
+// a free (or pinned allocation) synchronises the device, which invalidates a
+// stream capture running in another thread (factorize captures the solve
+// graph), so trees that run in parallel threads take this lock around both.
+std::recursive_mutex& capture_lock();
+inline void dev_free(void* p) {
+  if (!p) return;
+  std::lock_guard<std::recursive_mutex> lk(capture_lock());
+  (void)hipFree(p);
+}
+inline void host_free(void* p) {
+  if (!p) return;
+  std::lock_guard<std::recursive_mutex> lk(capture_lock());
+  (void)hipHostFree(p);
+}
+inline hipError_t dev_malloc(void** p, size_t bytes) {
+  std::lock_guard<std::recursive_mutex> lk(capture_lock());
+  return hipMalloc(p, bytes);
+}
+inline hipError_t host_malloc(void** p, size_t bytes) {
+  std::lock_guard<std::recursive_mutex> lk(capture_lock());
+  return hipHostMalloc(p, bytes, hipHostMallocDefault);
+}
 
 inline double wall() {
   return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();

@@ -10,7 +10,9 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <numeric>
+#include <set>
 
 #include "../kernels/launch.hpp"
 #include "tuner.hpp"
@@ -32,7 +34,7 @@ class Engine final : public EngineBase {
     bottoms_.resize(nlevels);
     info_.resize(nlevels);
     nb_ = panel_nb();
-    SPQ_HIP(hipHostMalloc((void**)&rank_host_, sizeof(int) * 65536, hipHostMallocDefault));
+    SPQ_HIP(host_malloc((void**)&rank_host_, sizeof(int) * 65536));
     qp3_err_ = (int*)perm_.alloc(256);
     SPQ_HIP(hipMemset(qp3_err_, 0, 256));
     if (const char* e = std::getenv("SPAQR_QP3_MAX_WG")) qp3_max_wg_ = std::max(1, std::atoi(e));
@@ -50,21 +52,23 @@ class Engine final : public EngineBase {
     if (const char* e = std::getenv("SPAQR_SOLVE_GRAPH")) solve_graph_ = std::atoi(e) != 0;
     if (const char* e = std::getenv("SPAQR_QP3_RC")) {
       const int r = std::atoi(e);
-      if (r >= 16 && r <= 48 && r % 4 == 0 && r != 36 && r != 44) qp3_rc_ = r;
+      for (int c : qp3r_rcs())
+        if (r == c) qp3_rc_ = r;
     }
   }
   ~Engine() override {
     (void)hipStreamSynchronize(stream_);
     for (auto e : ev_pool_) (void)hipEventDestroy(e);
+    std::lock_guard<std::recursive_mutex> lk(capture_lock());
     if (graph_exec_fwd_) hipGraphExecDestroy(graph_exec_fwd_);
     if (graph_exec_nrml_) hipGraphExecDestroy(graph_exec_nrml_);
     if (graph_exec_t_) hipGraphExecDestroy(graph_exec_t_);
-    if (xtbuf_) hipFree(xtbuf_);
-    if (x64_) hipFree(x64_);
-    if (rank_host_) hipHostFree(rank_host_);
-    if (xbuf_) hipFree(xbuf_);
-    if (bin_) hipFree(bin_);
-    if (xout_) hipFree(xout_);
+    dev_free(xtbuf_);
+    dev_free(x64_);
+    host_free(rank_host_);
+    dev_free(xbuf_);
+    dev_free(bin_);
+    dev_free(xout_);
     hipStreamDestroy(stream_);
   }
 
@@ -660,11 +664,15 @@ class Engine final : public EngineBase {
     if (panel_cap_ < 0) panel_cap_ = panel_r_capacity<T>();
     return panel_cap_;
   }
-  int qp3r_cap_[2][9] = {{-1, -1, -1, -1, -1, -1, -1, -1, -1}, {-1, -1, -1, -1, -1, -1, -1, -1, -1}};
+  int qp3r_cap_[2][13] = {{-1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1},
+                          {-1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1}};
   int qp3_rc_ = 0;  // SPAQR_QP3_RC: force the register form's rows per lane
+  // rows per lane of the register form's kernels: 16 to 32 (256 and 1024
+  // threads), 40 and 48 (256 threads)
+  static std::vector<int> qp3r_rcs() { return {16, 20, 24, 28, 32, 40, 48}; }
   // resident workgroups of the register-form QR with nt threads per
   // workgroup (largest LDS footprint, G = 64)
-  int qp3r_cap(int nt, int rc) {  // rc in {16, 20, ..., 48} (40 and 48: 256 threads only)
+  int qp3r_cap(int nt, int rc) {
     int& c = qp3r_cap_[nt == 1024 ? 1 : 0][(rc - 16) / 4];
     if (c < 0) c = qp3r_capacity<T>(64, nt, rc);
     return c;
@@ -902,12 +910,41 @@ void Engine<T>::eliminate_level(const std::vector<int>& cs, int fine_wave) {
     plans.push_back(std::move(p));
   }
   init_blocks(zero, 0);
-  // waves: a cluster waits for every earlier cluster sharing a target block
-  // (wave = 1 + the largest wave of an earlier cluster touching one of its
-  // target blocks, tracked per block)
+  // waves: two clusters that update a common target block may not share a
+  // launch (their read-modify-writes of the block would race).  Their
+  // updates commute: no row of A couples two same-level subtrees, so their
+  // reflectors act on disjoint rows of a shared block (tree.cpp:280-316 runs
+  // them in creation order; the order changes nothing but rounding where row
+  // supports meet through fill).  So the waves are a greedy colouring in
+  // creation order: the smallest wave no earlier cluster sharing a block
+  // holds (per-block masks of used waves).  The inline elimination of
+  // --scale 0 fine clusters keeps creation order (wave = 1 + the largest
+  // wave of an earlier cluster sharing a block).
   std::vector<int> wave(plans.size(), 0);
   int nw = plans.empty() ? 0 : 1;
-  {
+  if (fine_wave < 0) {
+    constexpr int kWords = 4;  // up to 256 waves by colouring
+    std::vector<std::array<uint64_t, kWords>> used(ed_.size(), std::array<uint64_t, kWords>{});
+    std::vector<int> maxw(ed_.size(), -1);
+    for (size_t i = 0; i < plans.size(); ++i) {
+      std::array<uint64_t, kWords> m{};
+      int wl = 0;
+      for (int f : plans[i].touched) {
+        for (int q = 0; q < kWords; ++q) m[q] |= used[f][q];
+        wl = std::max(wl, maxw[f] + 1);
+      }
+      int w = -1;
+      for (int q = 0; q < kWords && w < 0; ++q)
+        if (~m[q]) w = 64 * q + __builtin_ctzll(~m[q]);
+      if (w < 0) w = wl;  // more than 256 waves: after every conflicting one
+      wave[i] = w;
+      for (int f : plans[i].touched) {
+        if (w < 64 * kWords) used[f][w / 64] |= 1ull << (w % 64);
+        maxw[f] = std::max(maxw[f], w);
+      }
+      nw = std::max(nw, w + 1);
+    }
+  } else {
     std::vector<int> maxw(ed_.size(), -1);
     for (size_t i = 0; i < plans.size(); ++i) {
       int w = 0;
@@ -1127,7 +1164,7 @@ void Engine<T>::rrqr_batch(std::vector<RrqrJob>& jobs, int rc_force) {
         // in place), keep the fastest
         std::vector<std::pair<int, double>> ms;
         double best_ms = -1;
-        for (int r : {16, 20, 24, 28, 32, 40, 48}) {
+        for (int r : qp3r_rcs()) {
           if (qp3r_cap(256, r) <= 0) continue;
           std::vector<RrqrJob> trial;
           std::vector<CopyDesc> cp;
@@ -1205,7 +1242,7 @@ void Engine<T>::rrqr_batch(std::vector<RrqrJob>& jobs, int rc_force) {
   std::vector<Group> groups;
   std::vector<std::pair<int, int>> forms = {{0, 0}};  // (threads, rows per lane); 0: streaming form
   for (int nt : {256, 1024})
-    for (int r : {16, 20, 24, 28, 32, 40, 48})
+    for (int r : qp3r_rcs())
       if (nt == 256 || r <= 32) forms.push_back({nt, r});
   for (auto form : forms) {
     const int nt = form.first, rcf = form.second;
@@ -1250,11 +1287,15 @@ void Engine<T>::rrqr_batch(std::vector<RrqrJob>& jobs, int rc_force) {
     int rc = 0;
     // one timing family per kernel: streaming, register 256 / 1024 threads;
     // flops are added once the ranks are known
-    static const char* const kFam[2][9] = {
-        {"qp3r256_16", "qp3r256_20", "qp3r256_24", "qp3r256_28", "qp3r256_32", "qp3r256_36", "qp3r256_40",
-         "qp3r256_44", "qp3r256_48"},
-        {"qp3r1024_16", "qp3r1024_20", "qp3r1024_24", "qp3r1024_28", "qp3r1024_32", "", "", "", ""}};
-    const char* fam = gq.nt == 0 ? "qp3s" : kFam[gq.nt == 1024 ? 1 : 0][(gq.rc - 16) / 4];
+    const char* fam = "qp3s";
+    if (gq.nt) {
+      static std::mutex mu;
+      static std::set<std::string> names;  // interned family names (the map below keeps pointers)
+      char buf[32];
+      snprintf(buf, sizeof buf, "qp3r%d_%d", gq.nt, gq.rc);
+      std::lock_guard<std::mutex> lk(mu);
+      fam = names.insert(buf).first->c_str();
+    }
     for (auto& d : gq.d) qp3_family_[d.rank_out] = fam;
     timed(fam, 0.0, qby, [&] {
       int nmax = 0;
@@ -2156,10 +2197,10 @@ void Engine<T>::build_solve_program() {
   std::map<const Op*, int64_t> split_tmp;
   for (auto& o : ops_)
     if (o.kind == OpSplitD && o.crows > o.ccols) { split_tmp[&o] = xsize_; xsize_ += o.crows - o.ccols; }
-  SPQ_HIP(hipMalloc((void**)&xbuf_, sizeof(T) * std::max<int64_t>(xsize_, 1)));
-  if (!square_) SPQ_HIP(hipMalloc((void**)&xtbuf_, sizeof(T) * std::max<int64_t>(xsize_, 1)));
-  SPQ_HIP(hipMalloc((void**)&bin_, sizeof(double) * std::max(nrows_, 1)));
-  SPQ_HIP(hipMalloc((void**)&xout_, sizeof(double) * std::max(ncols_, 1)));
+  SPQ_HIP(dev_malloc((void**)&xbuf_, sizeof(T) * std::max<int64_t>(xsize_, 1)));
+  if (!square_) SPQ_HIP(dev_malloc((void**)&xtbuf_, sizeof(T) * std::max<int64_t>(xsize_, 1)));
+  SPQ_HIP(dev_malloc((void**)&bin_, sizeof(double) * std::max(nrows_, 1)));
+  SPQ_HIP(dev_malloc((void**)&xout_, sizeof(double) * std::max(ncols_, 1)));
   std::vector<Seg> segs;
   std::vector<R12Desc<T>> r12;
   // buf: the x buffer a stage works on (0 = x, 1 = xt); moves copy sbuf -> buf
@@ -2195,7 +2236,7 @@ void Engine<T>::build_solve_program() {
       if (o.kind != OpQR || !o.r12.empty()) throw Error(kErrState, "FP64 top separator: unexpected op structure");
       x64_len = std::max<int64_t>(x64_len, o.m);
     }
-  if (x64_len > 0) SPQ_HIP(hipMalloc((void**)&x64_, sizeof(double) * x64_len));
+  if (x64_len > 0) SPQ_HIP(dev_malloc((void**)&x64_, sizeof(double) * x64_len));
   // xb: the x buffer the op's segments live in (xbuf_, or xtbuf_ for the
   // tops program of the normal equations)
   auto qr64_stages = [&](const Op& o, int trans, bool bwd, std::vector<HStage>& out, T* xb) {
@@ -2798,12 +2839,15 @@ void Engine<T>::run_program(const double* b, double* x, bool prof, int mode) {
 }
 
 // One solve program (mode as in run_program) captured into an executable
-// graph that reads bin_ and writes xout_.
+// graph that reads bin_ and writes xout_.  The capture only enqueues kernels
+// on this tree's stream; relaxed mode keeps other threads (other trees, the
+// caller's allocations and synchronisations) free to run meanwhile.
 template <typename T>
 hipGraphExec_t Engine<T>::capture_program(int mode) {
   hipGraph_t g = nullptr;
   hipGraphExec_t ex = nullptr;
-  SPQ_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+  std::lock_guard<std::recursive_mutex> lk(capture_lock());
+  SPQ_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeRelaxed));
   run_program(bin_, xout_, false, mode);
   SPQ_HIP(hipStreamEndCapture(stream_, &g));
   const hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
@@ -2862,6 +2906,7 @@ void Engine<T>::solve_t_dev(const double* b, double* x) {
 }
 
 EngineBase* make_engine(int nlevels, int skip, int precision_bytes, int device) {
+  host_prof_start();
   if (precision_bytes == 4) return new Engine<float>(nlevels, skip, device);
   return new Engine<double>(nlevels, skip, device);
 }

@@ -101,4 +101,8 @@ class EngineBase {
 
 EngineBase* make_engine(int nlevels, int skip, int precision_bytes, int device);
 
+// SPAQR_HOST_PROF=1: sample the calling thread's host time (diagnostics;
+// the report goes to stderr at process exit).  Called when a tree is made.
+void host_prof_start();
+
 }  // namespace spq

@@ -10,8 +10,8 @@
 namespace spq {
 
 EngineBase::~EngineBase() {
-  for (auto& w : work_dev_) if (w.p) (void)hipFree(w.p);
-  for (auto& w : work_host_) if (w.p) (void)hipHostFree(w.p);
+  for (auto& w : work_dev_) dev_free(w.p);
+  for (auto& w : work_host_) host_free(w.p);
 }
 
 double* EngineBase::work_dev(int slot, size_t elems) {
@@ -19,10 +19,10 @@ double* EngineBase::work_dev(int slot, size_t elems) {
   Work& w = work_dev_[slot];
   elems = std::max<size_t>(elems, 1);
   if (w.cap < elems) {
-    if (w.p) SPQ_HIP(hipFree(w.p));
+    dev_free(w.p);
     w.p = nullptr;
     w.cap = 0;
-    SPQ_HIP(hipMalloc((void**)&w.p, sizeof(double) * elems));
+    SPQ_HIP(dev_malloc((void**)&w.p, sizeof(double) * elems));
     w.cap = elems;
   }
   return w.p;
@@ -33,23 +33,22 @@ double* EngineBase::work_host(int slot, size_t elems) {
   Work& w = work_host_[slot];
   elems = std::max<size_t>(elems, 1);
   if (w.cap < elems) {
-    if (w.p) SPQ_HIP(hipHostFree(w.p));
+    host_free(w.p);
     w.p = nullptr;
     w.cap = 0;
-    SPQ_HIP(hipHostMalloc((void**)&w.p, sizeof(double) * elems, hipHostMallocDefault));
+    SPQ_HIP(host_malloc((void**)&w.p, sizeof(double) * elems));
     w.cap = elems;
   }
   return w.p;
 }
 
 DevMatrix::~DevMatrix() {
-  hipFree(rp); hipFree(ci); hipFree(v);
-  hipFree(rpt); hipFree(cit); hipFree(vt);
+  for (void* p : {(void*)rp, (void*)ci, (void*)v, (void*)rpt, (void*)cit, (void*)vt}) dev_free(p);
 }
 
 namespace {
 void up(void** d, const void* h, size_t bytes) {
-  SPQ_HIP(hipMalloc(d, bytes > 0 ? bytes : 8));
+  SPQ_HIP(dev_malloc(d, bytes > 0 ? bytes : 8));
   if (bytes) SPQ_HIP(hipMemcpy(*d, h, bytes, hipMemcpyHostToDevice));
 }
 }  // namespace

@@ -29,6 +29,8 @@ sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.join(ROOT, "oracle"))
 
 WORKLOADS = [("advdiff3d_64", 64, 3, 1e-2, False), ("hicontrast3d_64", 64, 3, 1e-2, True)]
+# cluster-table columns (l, self, left, right, cstart, order) that do not depend on ranks
+STRUCT_COLS = [0, 1, 2, 3, 4, 5, 6, 7, 11]
 
 
 def sha(a):
@@ -49,6 +51,10 @@ def main():
             workload=np.array(name), tol=np.array(tol), backend=np.array(backend),
             cperm_sha=np.array(sha(cperm)), rperm_sha=np.array(sha(rperm)),
             cluster_table_sha=np.array(sha(t.cluster_table())),
+            # the tree's structure: level, ids, column start and creation
+            # order of every cluster (the size columns of merged clusters
+            # follow the ranks of the levels below)
+            cluster_struct_sha=np.array(sha(t.cluster_table()[:, STRUCT_COLS])),
             sparsified=t.sparsified().astype(np.int32),
             top_size=np.array(r["top_size"]), nlevels=np.array(r["nlevels"]),
             nnz=np.array([r["nnzA"], r["nnzR"], r["nnzH"], r["nnzQ"]], dtype=np.uint64),

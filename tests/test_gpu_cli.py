@@ -1,0 +1,82 @@
+"""The spaqr command-line driver (spaQR.cpp:25-323 flags and flow) on Matrix
+Market input: a sparse coordinate matrix (-m), a dense array of coordinates
+(--coordinates, d x N, mmio.hpp:235-275) and a right-hand side vector
+(--rhs, mmio.hpp:277-320).  Its printed one-shot residual and GMRES count
+must match the library flow (core.run_flow) on the same inputs, and the
+coordinate file must give the same partition as the tensor-grid flags.
+"""
+import json
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from spaqr_public_amd import gen
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CLI = os.path.join(ROOT, "spaqr_public_amd", "_lib", "spaqr")
+
+
+def run_cli(*args):
+    out = subprocess.run([CLI, *args], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr + out.stdout
+    return out.stdout
+
+
+def parse(stdout):
+    one = float(re.search(r"<<<<\|\(Ax-b\)\|/\|b\| : ([0-9.eE+-]+)", stdout).group(1))
+    m = re.search(r"GMRES: #iterations: (\d+), residual \|Ax-b\|/\|b\|: ([0-9.eE+-]+)", stdout)
+    return one, int(m.group(1)), float(m.group(2))
+
+
+def linspace_coords(n, dim):
+    """linspace_nd (util.cpp:852-881): dof i*n + j at (i, j) (2-D)."""
+    g = np.arange(n, dtype=np.float64)
+    axes = np.meshgrid(*([g] * dim), indexing="ij")
+    return np.stack([a.ravel() for a in axes])  # dim x n^dim
+
+
+def test_cli_matrix_market_input(gpu_lib, tmp_path):
+    import spaqr_public_amd as sq
+    n, dim, tol = 32, 2, 1e-2
+    A = gen.config_matrix("advdiff2d_32")
+    b = np.random.default_rng(7).standard_normal(A.shape[0])
+    mfile, xfile, bfile = (str(tmp_path / f) for f in ("A.mm", "X.mm", "b.mm"))
+    gen.write_mm(mfile, A)
+    gen.write_mm_array(xfile, linspace_coords(n, dim))
+    gen.write_mm_array(bfile, b[:, None])
+    pfile = str(tmp_path / "profile.json")
+    out_file = run_cli("-m", mfile, "--coordinates", xfile, "--rhs", bfile, "-t", str(tol), "--profile", pfile)
+    prof = json.load(open(pfile))
+    assert len(prof["levels"]) == prof["nlevels"] and prof["cols"] == A.shape[1]
+    assert len(prof["interfaces"]) > 0
+    out_grid = run_cli("-m", mfile, "-n", str(n), "-d", str(dim), "--rhs", bfile, "-t", str(tol))
+    assert "GMRES converged!" in out_file
+    one_f, it_f, res_f = parse(out_file)
+    one_g, it_g, res_g = parse(out_grid)
+    # the coordinate file and the tensor-grid flags give the same tree
+    assert abs(one_f - one_g) <= 1e-6 * one_g and it_f == it_g
+    # the library flow on the same inputs: the one-shot uses random(n, 2021),
+    # GMRES the --rhs vector
+    ref_one = sq.run_flow(A, n, dim, tol)
+    ref_rhs = sq.run_flow(A, n, dim, tol, rhs=b)
+    print(f"CLI one-shot {one_f:.3e} (library {ref_one['oneshot_res']:.3e}), GMRES {it_f} its {res_f:.2e} "
+          f"(library {ref_rhs['iters']} its {ref_rhs['final_res']:.2e})")
+    assert abs(one_f - ref_one["oneshot_res"]) <= 1e-3 * ref_one["oneshot_res"]
+    assert abs(it_f - ref_rhs["iters"]) <= 1
+    assert res_f <= 1e-12
+
+
+def test_cli_reports_nonconvergence(gpu_lib, tmp_path):
+    """With too few iterations the driver says so instead of claiming
+    convergence."""
+    A = gen.config_matrix("advdiff2d_32")
+    mfile = str(tmp_path / "A.mm")
+    gen.write_mm(mfile, A)
+    out = run_cli("-m", mfile, "-n", "32", "-d", "2", "-t", "0.5", "-i", "2")
+    assert "GMRES converged!" not in out
+    assert "GMRES failed to converge in 2 iterations" in out
