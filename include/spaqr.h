@@ -58,6 +58,17 @@ int spaqr_set_hsl(spaqr_tree* t, int hsl); /* only 0 (no matching) is available 
 int spaqr_set_coordinates(spaqr_tree* t, const double* X, int dim, int n);
 /* linspace_nd(n, dim) coordinates (util.cpp:852-881), as spaQR.cpp:159.  */
 int spaqr_set_tensor_coordinates(spaqr_tree* t, int n, int dim);
+/* Row partition for the non-geometric path (trees without coordinates,
+ * tree.cpp:608-617): parts[i] in [0, 2^(nlevels-1)) for each of the m rows,
+ * parts 2g and 2g+1 being the halves of part g one level up -- the output of
+ * partition_patoh (partition.cpp:422-448), brought by the caller.  Without
+ * it, partition uses spaqr_row_parts.                                       */
+int spaqr_set_row_parts(spaqr_tree* t, const int* parts, int m);
+/* The library's own row partitioner in place of partition_patoh
+ * (partition.cpp:422-448, PaToH): recursive bisection of the column-net
+ * hypergraph of the m x n pattern with Fiduccia-Mattheyses refinement of
+ * the cut nets.  Host only (no device needed).  Writes m parts.            */
+int spaqr_row_parts(int m, int n, const int* colptr, const int* rowind, int nlevels, int* parts);
 /* Per-level progress lines on stdout (tree.cpp:1417-1434).               */
 int spaqr_set_verbose(spaqr_tree* t, int verbose);
 

@@ -127,6 +127,25 @@ void orc_tree_set(void* p, double tol, int scale, float order, int square, int v
 void orc_tree_set_tensor_coords(void* p, int n, int dim) {
   ((Handle*)p)->t->set_Xcoo(linspace_nd(n, dim), dim);
 }
+void orc_tree_set_row_parts(void* p, const int* parts, int m) {
+  ((Handle*)p)->t->set_row_parts(std::vector<int>(parts, parts + m));
+}
+// Column labels of the non-geometric path from a row partition: 6 ints per
+// column (self, left, right as lvl, sep).
+int orc_hypergraph_labels(int m, int n, const int* colptr, const int* rowind, int nlevels, const int* parts,
+                          int* out) {
+  try {
+    std::vector<double> ones(colptr[n], 1.0);
+    Csc A = make_csc(m, n, colptr, rowind, ones.data());
+    std::vector<ClusterID> cmap = hypergraph_partition(A, nlevels, std::vector<int>(parts, parts + m));
+    for (int i = 0; i < n; ++i) {
+      const ClusterID& c = cmap[i];
+      int* o = out + 6 * i;
+      o[0] = c.self.lvl; o[1] = c.self.sep; o[2] = c.l.lvl; o[3] = c.l.sep; o[4] = c.r.lvl; o[5] = c.r.sep;
+    }
+  } catch (std::exception& e) { g_err = e.what(); return -1; }
+  return 0;
+}
 int orc_partition(void* p, int m, int n, const int* colptr, const int* rowind, const double* val) {
   Handle* h = (Handle*)p;
   try {

@@ -310,6 +310,68 @@ std::vector<ClusterID> geometric_partition_ata(const Csc& A, int nlevels, const 
   return cmap;
 }
 
+static int part_at_lvl(int part_leaf, int lvl) {  // partition.cpp:54-61
+  for (int l = 0; l < lvl - 1; ++l) part_leaf /= 2;
+  return part_leaf;
+}
+
+// partition.cpp:582-623 (getInterfacesUsingA_HUND)
+static void get_interfaces_hund(const Csc& A, std::vector<ClusterID>& cmap, const std::vector<int>& parts) {
+  Csc At = csc_transpose(A);
+  for (int i = 0; i < A.n; ++i) {
+    SepID self = cmap[i].self;
+    if (self.lvl <= 0) continue;
+    SepID left(-1, 0), right(-1, 0);
+    for (int p = A.colptr[i]; p < A.colptr[i + 1]; ++p) {
+      int j = A.rowind[p];
+      int pj = part_at_lvl(parts[j], self.lvl);
+      for (int q = At.colptr[j]; q < At.colptr[j + 1]; ++q) {
+        SepID nbr = cmap[At.rowind[q]].self;
+        if (nbr.lvl >= self.lvl) continue;
+        if (pj % 2 == 0) left = left.lvl == -1 ? nbr : find_highest_common(nbr, left);
+        else right = right.lvl == -1 ? nbr : find_highest_common(nbr, right);
+      }
+    }
+    cmap[i].l = left;
+    cmap[i].r = right;
+  }
+  for (auto& c : cmap) {
+    if (c.l == SepID()) c.l = c.self;
+    if (c.r == SepID()) c.r = c.self;
+  }
+}
+
+// partition.cpp:450-499 on a given row partition (partition_patoh's output,
+// partition.cpp:422-448: PaToH itself is a binary-only library)
+std::vector<ClusterID> hypergraph_partition(const Csc& A, int nlevels, const std::vector<int>& parts) {
+  if ((int)parts.size() != A.m) throw std::runtime_error("oracle: row partition of the wrong length");
+  std::vector<ClusterID> cmap(A.n);
+  for (int depth = 0; depth < nlevels - 1; ++depth) {
+    int l = nlevels - depth - 1;
+    for (int i = 0; i < A.n; ++i) {
+      if (cmap[i].self.lvl != -1) continue;
+      bool sep = false;
+      int pi = -1;
+      for (int p = A.colptr[i]; p < A.colptr[i + 1]; ++p) {
+        int pj = part_at_lvl(parts[A.rowind[p]], l);
+        if (pi == -1) pi = pj;
+        else if (pi != pj) { sep = true; break; }
+      }
+      if (sep) cmap[i].self = SepID(l, pi / 2);
+    }
+  }
+  for (int i = 0; i < A.n; ++i) {
+    if (cmap[i].self.lvl != -1) continue;
+    if (A.colptr[i] == A.colptr[i + 1]) throw std::runtime_error("oracle: empty column");
+    int j = A.rowind[A.colptr[i]];
+    cmap[i].self = SepID(0, parts[j]);
+    cmap[i].l = SepID(0, parts[j]);
+    cmap[i].r = SepID(0, parts[j]);
+  }
+  get_interfaces_hund(A, cmap, parts);
+  return cmap;
+}
+
 void init_permutation(int nlevels, const std::vector<ClusterID>& cmap, std::vector<int>& cperm) {
   // tree.cpp:46-55
   std::vector<ClusterID> pm = cmap;
@@ -390,8 +452,10 @@ void Tree::partition(const Csc& A) {  // tree.cpp:595-723
   double t0 = now_s();
   int r = A.m, c = A.n;
   nrows = r; ncols = c;
-  if (!geo) throw std::runtime_error("oracle: only the geometric partition is restated");
-  std::vector<ClusterID> cmap = geometric_partition_ata(A, nlevels, Xcoo, xdim);
+  std::vector<ClusterID> cmap;
+  if (geo) cmap = geometric_partition_ata(A, nlevels, Xcoo, xdim);
+  else if (!row_parts.empty()) cmap = hypergraph_partition(A, nlevels, row_parts);
+  else throw std::runtime_error("oracle: no coordinates and no row partition");
   rperm.resize(r); cperm.resize(c);
   for (int i = 0; i < r; ++i) rperm[i] = i;
   for (int i = 0; i < c; ++i) cperm[i] = i;

@@ -58,6 +58,9 @@ struct ClusterID {  // cluster.h:43-111
   }
 };
 ClusterID merge_if(const ClusterID& c, int lvl);  // cluster.cpp:23-32
+// partition.cpp:450-499 + 582-623 from a row partition (parts[i] in
+// [0, 2^(nlevels-1))): column labels of the non-geometric path.
+std::vector<ClusterID> hypergraph_partition(const Csc& A, int nlevels, const std::vector<int>& parts);
 
 // Partition (partition.cpp:241-348, 506-578 ; tree.cpp:46-163)
 std::vector<ClusterID> geometric_partition_ata(const Csc& A, int nlevels, const std::vector<double>& X, int dim);
@@ -136,6 +139,8 @@ class Tree {
   void set_square(int s) { square = s; }
   void set_order(float s) { order = s; }
   void set_Xcoo(const std::vector<double>& X, int dim) { Xcoo = X; xdim = dim; geo = true; }
+  // row partition of the non-geometric path (tree.cpp:608-617)
+  void set_row_parts(const std::vector<int>& p) { row_parts = p; }
   void set_verbose(int v) { verbose = v; }
   int rows() const { return nrows; }
   int cols() const { return ncols; }
@@ -168,6 +173,7 @@ class Tree {
   bool geo = false;
   std::vector<double> Xcoo;
   int xdim = 0;
+  std::vector<int> row_parts;
   int uid_counter = 0;
   std::vector<std::list<Cluster*>> bottoms;
   std::vector<std::unique_ptr<Cluster>> owned;

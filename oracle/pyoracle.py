@@ -41,6 +41,8 @@ def lib():
         L.orc_tree_free.argtypes = [C.c_void_p]
         L.orc_tree_set.argtypes = [C.c_void_p, C.c_double, C.c_int, C.c_float, C.c_int, C.c_int]
         L.orc_tree_set_tensor_coords.argtypes = [C.c_void_p, C.c_int, C.c_int]
+        L.orc_tree_set_row_parts.argtypes = [C.c_void_p, _ip, C.c_int]
+        L.orc_hypergraph_labels.argtypes = [C.c_int, C.c_int, _ip, _ip, C.c_int, _ip, _ip]
         L.orc_partition.argtypes = [C.c_void_p, C.c_int, C.c_int, _ip, _ip, _dp]
         L.orc_assemble.argtypes = [C.c_void_p]
         L.orc_factorize.argtypes = [C.c_void_p]
@@ -213,6 +215,10 @@ class Tree:
     def set_tensor_coords(self, n, dim):
         lib().orc_tree_set_tensor_coords(self.h, n, dim)
 
+    def set_row_parts(self, parts):
+        self._parts = np.ascontiguousarray(parts, dtype=np.int32)
+        lib().orc_tree_set_row_parts(self.h, _i(self._parts), len(self._parts))
+
     def partition(self, A):
         A = A.tocsc()
         A.sort_indices()
@@ -303,10 +309,27 @@ def default_levels(ncols: int) -> int:
     return int(math.ceil(math.log2(ncols // 64)))
 
 
+def hypergraph_labels(A, nlevels, parts):
+    """partition.cpp:450-499 + 582-623: column labels from a row partition,
+    as an (n, 6) array of self, left, right (lvl, sep)."""
+    import scipy.sparse as sp
+    A = sp.csc_matrix(A)
+    A.sort_indices()
+    m, n = A.shape
+    cp = np.ascontiguousarray(A.indptr, dtype=np.int32)
+    ri = np.ascontiguousarray(A.indices, dtype=np.int32)
+    p = np.ascontiguousarray(parts, dtype=np.int32)
+    out = np.zeros((n, 6), dtype=np.int32)
+    if lib().orc_hypergraph_labels(m, n, _i(cp), _i(ri), nlevels, _i(p), _i(out)) != 0:
+        raise RuntimeError(lib().orc_last_error().decode())
+    return out
+
+
 def run_reference_flow(A, n, dim, tol, nlevels=None, skip=0, scale=1, order=1.0, iters=300, res=1e-12,
-                       rhs=None, verbose=0):
+                       rhs=None, verbose=0, parts=None):
     """The flow of spaQR.cpp:85-320 on a scipy matrix (column scaling, tree,
-    one-shot solve, GMRES/CGLS) ; returns a result dict."""
+    one-shot solve, GMRES/CGLS) ; returns a result dict.  n = None: no
+    coordinates, the non-geometric partition from the row `parts`."""
     import time
     import scipy.sparse as sp
     A = sp.csc_matrix(A)
@@ -320,7 +343,10 @@ def run_reference_flow(A, n, dim, tol, nlevels=None, skip=0, scale=1, order=1.0,
         skip = nlevels - 1
     t = Tree(nlevels, skip)
     t.configure(tol, scale=scale if m == ncols else 1, order=order, square=(m == ncols), verbose=verbose)
-    t.set_tensor_coords(n, dim)
+    if n is not None:
+        t.set_tensor_coords(n, dim)
+    if parts is not None:
+        t.set_row_parts(parts)
     t.partition(As)
     t.assemble()
     t.factorize()

@@ -41,7 +41,7 @@ SYMBOLS = [
     "spaqr_kernel_stats_reset", "spaqr_solve_nrml_device", "spaqr_cgls", "spaqr_cgls_device",
     "spaqr_solve_transpose", "spaqr_solve_transpose_device", "spaqr_gmres_transpose_device",
     "spaqr_csc_outer_device", "spaqr_num_extra_sparsified", "spaqr_extra_sparsified", "spaqr_profile_json",
-    "spaqr_gmres_device_noalloc",
+    "spaqr_gmres_device_noalloc", "spaqr_set_row_parts", "spaqr_row_parts",
 ]
 
 
@@ -63,6 +63,8 @@ def load_library(path: str = LIB_PATH):
         getattr(L, n).argtypes = [C.c_void_p, t]
     L.spaqr_set_coordinates.argtypes = [C.c_void_p, _dp, C.c_int, C.c_int]
     L.spaqr_set_tensor_coordinates.argtypes = [C.c_void_p, C.c_int, C.c_int]
+    L.spaqr_set_row_parts.argtypes = [C.c_void_p, _ip, C.c_int]
+    L.spaqr_row_parts.argtypes = [C.c_int, C.c_int, _ip, _ip, C.c_int, _ip]
     for n in ("spaqr_rows", "spaqr_cols", "spaqr_levels", "spaqr_num_sparsified", "spaqr_num_clusters",
               "spaqr_num_extra_sparsified"):
         getattr(L, n).argtypes = [C.c_void_p]
@@ -176,6 +178,14 @@ class Tree:
 
     def set_tensor_coordinates(self, n, dim):
         _check(load_library().spaqr_set_tensor_coordinates(self.h, n, dim), "set_tensor_coordinates")
+
+    def set_row_parts(self, parts):
+        """Row partition for a tree without coordinates: parts[i] in
+        [0, 2^(nlevels-1)), parts 2g and 2g+1 the halves of part g (the
+        output of partition_patoh, partition.cpp:422-448).  Without it the
+        library partitions the rows itself (row_parts)."""
+        p = np.ascontiguousarray(parts, dtype=np.int32)
+        _check(load_library().spaqr_set_row_parts(self.h, _i(p), len(p)), "set_row_parts")
 
     def set_coordinates(self, X):
         """X: dim x n array of coordinates (the reference's Xcoo)."""
@@ -388,10 +398,25 @@ def cgls_device(tree: Tree, M: Matrix, b_ptr, x_ptr, iters=300, tol=1e-10, verbo
     return it.value, est.value
 
 
+def row_parts(A, nlevels):
+    """The library's row partition of A into 2^(nlevels-1) parts (recursive
+    bisection of the column-net hypergraph with Fiduccia-Mattheyses
+    refinement), in place of partition.cpp:422-448 (PaToH).  Host only."""
+    A = sp.csc_matrix(A)
+    A.sort_indices()
+    m, n = A.shape
+    ip = np.ascontiguousarray(A.indptr, dtype=np.int32)
+    ix = np.ascontiguousarray(A.indices, dtype=np.int32)
+    out = np.zeros(m, dtype=np.int32)
+    _check(load_library().spaqr_row_parts(m, n, _i(ip), _i(ix), nlevels, _i(out)), "row_parts")
+    return out
+
+
 def run_flow(A, n, dim, tol, nlevels=None, skip=0, scale=1, order=1.0, iters=300, res=1e-12, precision=FP64,
-             rhs=None, verbose=0, device=0):
+             rhs=None, verbose=0, device=0, parts=None):
     """The driver flow of spaQR.cpp:85-320 on the GPU: GMRES for square
-    systems, CGLS on the normal equations for tall ones."""
+    systems, CGLS on the normal equations for tall ones.  n = None: no
+    coordinates (the non-geometric partition, from `parts` when given)."""
     import time
     A = sp.csc_matrix(A)
     if A.shape[0] < A.shape[1]:
@@ -408,7 +433,10 @@ def run_flow(A, n, dim, tol, nlevels=None, skip=0, scale=1, order=1.0, iters=300
     t.set_order(order)
     t.set_square(1 if m == ncols else 0)
     t.set_verbose(verbose)
-    t.set_tensor_coordinates(n, dim)
+    if n is not None:
+        t.set_tensor_coordinates(n, dim)
+    if parts is not None:
+        t.set_row_parts(parts)
     t.partition(As)
     t.assemble(As)
     t.factorize()

@@ -1,4 +1,5 @@
 // extern "C" boundary of libspaqr.so (declared in include/spaqr.h).
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -79,6 +80,20 @@ int spaqr_set_tensor_coordinates(spaqr_tree* t, int n, int dim) {
   return guard([&] {
     if (n < 1 || dim < 1 || dim > 3) throw Error(kErrArg, "tensor coordinates need n >= 1 and 1 <= dim <= 3");
     eng(t).set_coords(tensor_coordinates(n, dim), dim);
+  });
+}
+int spaqr_set_row_parts(spaqr_tree* t, const int* parts, int m) {
+  return guard([&] {
+    if (!parts || m < 0) throw Error(kErrArg, "invalid row partition");
+    eng(t).set_row_parts(std::vector<int>(parts, parts + m));
+  });
+}
+int spaqr_row_parts(int m, int n, const int* colptr, const int* rowind, int nlevels, int* parts) {
+  return guard([&] {
+    if (!parts || !colptr || n < 0) throw Error(kErrArg, "row_parts: null array");
+    const std::vector<double> pattern(std::max(colptr[n], 0), 1.0);
+    const std::vector<int> p = row_parts(make_csc(m, n, colptr, rowind, pattern.data()), nlevels);
+    std::copy(p.begin(), p.end(), parts);
   });
 }
 int spaqr_set_verbose(spaqr_tree* t, int v) { return guard([&] { eng(t).set_verbose(v); }); }

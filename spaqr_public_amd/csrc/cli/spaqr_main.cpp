@@ -160,6 +160,8 @@ void usage() {
          "      --coordinates arg      Coordinates MM array file (d x N)\n"
          "  -n, --coordinates_n arg    Tensor grid n^d coordinates (default: -1)\n"
          "  -d, --coordinates_d arg    Tensor grid dimension (default: -1)\n"
+         "      --parts arg            Row partition MM array file (m x 1) for a matrix without\n"
+         "                             coordinates (default: the library's own partitioner)\n"
          "      --hsl arg              Use HSL bipartite matching (default: 0; unavailable)\n"
          "  -t, --tol arg              Tolerance (default: 1e-1)\n"
          "      --skip arg             Skip sparsification (default: 0)\n"
@@ -190,7 +192,7 @@ static int run(int argc, char** argv) {
   const std::vector<std::string> known = {"matrix", "lvl", "coordinates", "coordinates_n", "coordinates_d", "hsl",
                                           "tol", "skip", "order", "scale", "iterations", "rhs", "res",
                                           "useEigenLSCG", "useEigenQR", "useCholesky", "precision", "device",
-                                          "profile"};
+                                          "profile", "parts"};
   std::map<std::string, int> given;
   for (int a = 1; a < argc; ++a) {
     std::string s = argv[a], key, v;
@@ -279,9 +281,17 @@ static int run(int argc, char** argv) {
     }
     // stored column-major d x N == dof-major coordinates
     check(spaqr_set_coordinates(t, X.data(), r, c), "coordinates");
-  } else {
-    fprintf(stderr, "Only the geometric partition is available: pass -n/-d or --coordinates.\n");
-    return 2;
+  } else if (given.count("parts")) {
+    // no coordinates: the row partition of a file (m x 1 array of parts)
+    int r, c;
+    std::vector<double> P = read_mm_dense(val["parts"], &r, &c);
+    if (r * c != A.m) {
+      printf("Error: the parts file should hold one part per row (%d)\n", A.m);
+      return 1;
+    }
+    std::vector<int> parts(P.begin(), P.end());
+    check(spaqr_set_row_parts(t, parts.data(), A.m), "row parts");
+    printf("Row partition loaded from %s\n", val["parts"].c_str());
   }
   check(spaqr_set_scale(t, scale), "set_scale");
   check(spaqr_set_tol(t, tol), "set_tol");
@@ -289,7 +299,6 @@ static int run(int argc, char** argv) {
   check(spaqr_set_hsl(t, std::stoi(val["hsl"])), "set_hsl");
   if (nrows == ncols) check(spaqr_set_square(t, 1), "set_square");
   check(spaqr_set_verbose(t, 1), "set_verbose");
-  printf("Using GeometricPartition... \n");
   check(spaqr_partition(t, A.m, A.n, A.p.data(), A.i.data(), A.x.data()), "partition");
   check(spaqr_assemble(t, A.m, A.n, A.p.data(), A.i.data(), A.x.data()), "assemble");
   check(spaqr_factorize(t), "factorize");

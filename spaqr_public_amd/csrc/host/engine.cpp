@@ -50,6 +50,7 @@ class Engine final : public EngineBase {
     }
     if (const char* e = std::getenv("SPAQR_SOLVE_STAGES")) stage_prof_ = std::atoi(e) != 0;
     if (const char* e = std::getenv("SPAQR_SOLVE_GRAPH")) solve_graph_ = std::atoi(e) != 0;
+    if (const char* e = std::getenv("SPAQR_ELIM_COLOR")) elim_color_ = std::atoi(e) != 0;
     if (const char* e = std::getenv("SPAQR_QP3_RC")) {
       const int r = std::atoi(e);
       for (int c : qp3r_rcs())
@@ -78,6 +79,7 @@ class Engine final : public EngineBase {
   void set_square(int s) override { square_ = s; }
   void set_order(float o) override { order_ = o; }
   void set_coords(const std::vector<double>& X, int dim) override { X_ = X; xdim_ = dim; geo_ = true; }
+  void set_row_parts(const std::vector<int>& parts) override { row_parts_ = parts; }
   void set_verbose(int v) override { verbose_ = v; }
   void set_timing(int t) override { timing_ = t; }
   int rows() const override { return nrows_; }
@@ -198,7 +200,7 @@ class Engine final : public EngineBase {
     std::vector<std::pair<Block, std::pair<int, int>>> r12;  // block, (cluster, s)
     // SplitD
     int crows = 0, ccols = 0, rank = 0;
-    int wave = 0;  // sparsify wave (OrthoD, SplitD, FineQR)
+    int wave = 0;  // sparsify wave (OrthoD, SplitD, FineQR); elimination wave (QR)
     bool nofwd = false;     // OrthoD without a forward step (Orthogonal, operations.h:113-129)
     const double* V64 = nullptr;   // FP64 factors of the top separator of an FP32 tree (V and R
     const double* Tb64 = nullptr;  // share the panel, as V / R do)
@@ -560,6 +562,8 @@ class Engine final : public EngineBase {
   float order_ = 1;
   bool geo_ = false;
   std::vector<double> X_;
+  std::vector<int> row_parts_;  // caller's row partition (non-geometric path)
+  bool elim_color_ = true;      // elimination waves by colouring (else creation-order longest path)
   int xdim_ = 0;
   int nrows_ = 0, ncols_ = 0, ilvl_ = -1, cur_ = 0, max_order_ = 0, nb_ = 32;
   std::vector<int> cperm_, rperm_;
@@ -686,9 +690,17 @@ void Engine<T>::partition(const CscMat& A) {  // tree.cpp:595-723
   const double t0 = wall();
   nrows_ = A.m;
   ncols_ = A.n;
-  if (!geo_) throw Error(kErrArg, "partition: coordinates are required (geometric partition)");
-  if ((int64_t)X_.size() != (int64_t)xdim_ * A.n) throw Error(kErrArg, "partition: coordinate array has wrong size");
-  std::vector<CID> lab = geometric_labels(A, nlevels_, X_, xdim_);
+  std::vector<CID> lab;
+  if (geo_) {
+    if ((int64_t)X_.size() != (int64_t)xdim_ * A.n) throw Error(kErrArg, "partition: coordinate array has wrong size");
+    if (verbose_) printf("Using GeometricPartition... \n");
+    lab = geometric_labels(A, nlevels_, X_, xdim_);
+  } else {
+    // no coordinates: a row partition (the caller's, else the library's
+    // own) and the labels it induces (tree.cpp:608-617)
+    if (verbose_) printf("Using Hypergraph Partition... \n");
+    lab = hypergraph_labels(A, nlevels_, row_parts_.empty() ? row_parts(A, nlevels_) : row_parts_);
+  }
   rperm_.resize(A.m);
   cperm_.resize(A.n);
   std::iota(rperm_.begin(), rperm_.end(), 0);
@@ -911,18 +923,20 @@ void Engine<T>::eliminate_level(const std::vector<int>& cs, int fine_wave) {
   }
   init_blocks(zero, 0);
   // waves: two clusters that update a common target block may not share a
-  // launch (their read-modify-writes of the block would race).  Their
-  // updates commute: no row of A couples two same-level subtrees, so their
-  // reflectors act on disjoint rows of a shared block (tree.cpp:280-316 runs
-  // them in creation order; the order changes nothing but rounding where row
-  // supports meet through fill).  So the waves are a greedy colouring in
-  // creation order: the smallest wave no earlier cluster sharing a block
-  // holds (per-block masks of used waves).  The inline elimination of
-  // --scale 0 fine clusters keeps creation order (wave = 1 + the largest
-  // wave of an earlier cluster sharing a block).
+  // launch (their read-modify-writes of the block would race).  At level 0
+  // their updates commute exactly: the partition separates the leaves in
+  // the graph of A^T A, so no row of A couples two leaves and their
+  // reflectors act on disjoint rows of a shared block.  There the waves are
+  // a greedy colouring in creation order: the smallest wave no earlier
+  // cluster sharing a block holds (per-block masks of used waves).  Above
+  // level 0 the rows of an interface have been mixed by its sparsification,
+  // so clusters sharing one do not commute and the order of
+  // tree.cpp:1320-1330 (creation order) is kept: wave = 1 + the largest wave
+  // of an earlier cluster sharing a block.  So is the inline elimination of
+  // --scale 0 fine clusters.
   std::vector<int> wave(plans.size(), 0);
   int nw = plans.empty() ? 0 : 1;
-  if (fine_wave < 0) {
+  if (fine_wave < 0 && ilvl_ == 0 && elim_color_) {
     constexpr int kWords = 4;  // up to 256 waves by colouring
     std::vector<std::array<uint64_t, kWords>> used(ed_.size(), std::array<uint64_t, kWords>{});
     std::vector<int> maxw(ed_.size(), -1);
@@ -1025,9 +1039,11 @@ void Engine<T>::eliminate_level(const std::vector<int>& cs, int fine_wave) {
     copy_blocks(scatter);
   }
   // ops and bookkeeping (tree.cpp:826-829)
-  for (auto& p : plans) {
+  for (size_t i = 0; i < plans.size(); ++i) {
+    Plan& p = plans[i];
     Op o;
     o.kind = fine_wave < 0 ? OpQR : OpFineQR; o.level = ilvl_; o.c = p.c;
+    o.wave = wave[i];
     if (fine_wave >= 0) { o.wave = fine_wave; o.fwd_refl = true; }
     o.V = p.P; o.ldv = p.m; o.m = p.m; o.k = p.k; o.Tb = p.Tb;
     for (int e : p.pe) o.segs.push_back({ed_[e].n2, cl_[ed_[e].n2].rsize});
@@ -1303,7 +1319,17 @@ void Engine<T>::rrqr_batch(std::vector<RrqrJob>& jobs, int rc_force) {
       rc = gq.nt ? launch_qp3r<T>(stream_, dqd, dst, (int)gq.d.size(), nt, gq.gmax, nmax, gq.nt, gq.rc, gq.coop)
                  : launch_qp3<T>(stream_, dqd, dst, (int)gq.d.size(), nt, kmax, gq.coop);
     });
-    if (rc != 0) throw Error(kErrHip, std::string("qp3 launch failed: ") + hipGetErrorString((hipError_t)rc));
+    if (rc != 0) {
+      int nmax = 0;
+      for (auto& d : gq.d) nmax = std::max(nmax, d.N);
+      char shape[200];
+      snprintf(shape, sizeof shape,
+               " (threads %d, rows per lane %d, cooperative %d, problems %zu, workgroups %d, lanes %d, N max %d, "
+               "k max %d, capacity %d)",
+               gq.nt, gq.rc, gq.coop, gq.d.size(), nt, gq.gmax, nmax, kmax,
+               gq.nt ? qp3r_cap(gq.nt, gq.rc) : qp3_capacity<T>(kmax));
+      throw Error(kErrHip, std::string("qp3 launch failed: ") + hipGetErrorString((hipError_t)rc) + shape);
+    }
     if (qp3_log_ && kprof_ && !trial_) {
       LaunchLog L{fam, kev_[fam].ev.size() - 1, {}, (int)gq.d.size(), nt, 0, 0, 0};
       for (auto& d : gq.d) {
@@ -2373,12 +2399,18 @@ void Engine<T>::build_solve_program() {
   std::vector<int> seg_wave(cl_.size(), -1);
   for (int lvl = 0; lvl < nlevels_; ++lvl) {
     LevelOps L = level_ops(lvl);
-    // QR forward waves: ops sharing an x segment keep their order (wave =
-    // 1 + the latest wave of an earlier op on one of its segments, tracked
-    // per segment)
+    // QR forward waves: the factorization's order (elimination wave, then
+    // creation order), where ops sharing an x segment keep that order (wave
+    // = 1 + the latest wave of an earlier op on one of its segments, tracked
+    // per segment).  Same-wave eliminations share no block, hence (the
+    // diagonal block of every panel row being a target) no segment, so the
+    // solve has as many QR stages as the factorization had waves.
     std::vector<int> wv(L.qr.size(), 0);
+    std::vector<int> ord(L.qr.size());
+    for (size_t i = 0; i < ord.size(); ++i) ord[i] = (int)i;
+    std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return L.qr[a]->wave < L.qr[b]->wave; });
     int nw = 0;
-    for (size_t i = 0; i < L.qr.size(); ++i) {
+    for (int i : ord) {
       int w = 0;
       for (auto& a2 : L.qr[i]->segs) w = std::max(w, seg_wave[a2.first] + 1);
       wv[i] = w;

@@ -56,6 +56,7 @@ class EngineBase {
   virtual void set_square(int s) = 0;
   virtual void set_order(float o) = 0;
   virtual void set_coords(const std::vector<double>& X, int dim) = 0;
+  virtual void set_row_parts(const std::vector<int>& parts) = 0;
   virtual void set_verbose(int v) = 0;
   virtual void set_timing(int t) = 0;
   virtual int rows() const = 0;

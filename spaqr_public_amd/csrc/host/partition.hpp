@@ -33,4 +33,13 @@ void order_rows_ls(const CscMat& Ap, const std::vector<CID>& cpermed, std::vecto
 
 std::vector<double> tensor_coordinates(int n, int dim);  // util.cpp:852-881
 
+// Non-geometric path (tree.cpp:608-617).  row_parts: the library's own
+// partition of the rows of A into 2^(nlevels-1) parts (recursive bisection
+// of the column-net hypergraph), in place of partition.cpp:422-448 (PaToH);
+// parts 2g and 2g+1 are the halves of part g one level up.
+std::vector<int> row_parts(const CscMat& A, int nlevels);
+// Column labels from a row partition: partition.cpp:450-499 (separators,
+// leaves) and 582-623 (getInterfacesUsingA_HUND).
+std::vector<CID> hypergraph_labels(const CscMat& A, int nlevels, const std::vector<int>& parts);
+
 }  // namespace spq

@@ -9,6 +9,24 @@ namespace spq {
 
 constexpr int kWave = 64;
 
+// Launch of a grid whose workgroups wait for one another (sized by the
+// caller within the device's resident capacity).  The cooperative launch
+// guarantees co-residency but rests on a device-wide resource that another
+// process using the GPU can hold, and the runtime then refuses it
+// (hipErrorIllegalState).  The same grid is then launched plainly: on a
+// device with no other work its workgroups are all resident (the solve's
+// multi-workgroup stages rely on the same), and every barrier wait in these
+// kernels is bounded, so a grid that is not co-resident sets the error flag
+// and ends instead of hanging.
+inline hipError_t launch_coresident(const void* f, dim3 grid, dim3 block, void** args, size_t shm, hipStream_t s) {
+  hipError_t e = hipLaunchCooperativeKernel(f, grid, block, args, shm, s);
+  if (e == hipErrorIllegalState) {
+    (void)hipGetLastError();  // clear the refusal
+    e = hipLaunchKernel(f, grid, block, args, shm, s);
+  }
+  return e;
+}
+
 template <typename T>
 __device__ __forceinline__ T wave_sum(T v) {
 #pragma unroll

@@ -260,7 +260,7 @@ int launch_panel_r(hipStream_t s, const PanelDesc<T>* d, const int* wstart, int 
     return (int)hipGetLastError();
   }
   void* args[] = {(void*)&d, (void*)&wstart, (void*)&nd, (void*)&jb};
-  return (int)hipLaunchCooperativeKernel((const void*)k_panel_r<T>, dim3(nwg_total), dim3(kThreads), args, 0, s);
+  return (int)launch_coresident((const void*)k_panel_r<T>, dim3(nwg_total), dim3(kThreads), args, 0, s);
 }
 template int panel_r_capacity<double>();
 template int panel_r_capacity<float>();

@@ -543,7 +543,7 @@ int launch_qp3r(hipStream_t s, const Qp3Desc<T>* d, const int* wstart, int nd, i
   const size_t shm = qp3r_lds<T>(gmax, rc, std::min(nmax, kPosLds));
   void* args[] = {(void*)&d, (void*)&wstart, (void*)&nd};
   if (!coop) return (int)hipLaunchKernel(f, dim3(nwg_total), dim3(nt), args, shm, s);
-  return (int)hipLaunchCooperativeKernel(f, dim3(nwg_total), dim3(nt), args, shm, s);
+  return (int)launch_coresident(f, dim3(nwg_total), dim3(nt), args, shm, s);
 }
 template int qp3r_capacity<double>(int, int, int);
 template int qp3r_capacity<float>(int, int, int);
@@ -577,7 +577,7 @@ int launch_qp3(hipStream_t s, const Qp3Desc<T>* d, const int* wstart, int nd, in
     return (int)hipGetLastError();
   }
   void* args[] = {(void*)&d, (void*)&wstart, (void*)&nd, (void*)&use_lds};
-  return (int)hipLaunchCooperativeKernel((const void*)k_qp3<T>, dim3(nwg_total), dim3(kQThreads), args, shm, s);
+  return (int)launch_coresident((const void*)k_qp3<T>, dim3(nwg_total), dim3(kQThreads), args, shm, s);
 }
 template int qp3_capacity<double>(int);
 template int qp3_capacity<float>(int);

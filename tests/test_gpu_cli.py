@@ -80,3 +80,28 @@ def test_cli_reports_nonconvergence(gpu_lib, tmp_path):
     out = run_cli("-m", mfile, "-n", "32", "-d", "2", "-t", "0.5", "-i", "2")
     assert "GMRES converged!" not in out
     assert "GMRES failed to converge in 2 iterations" in out
+
+
+def test_cli_without_coordinates(gpu_lib, tmp_path):
+    """No coordinates: the non-geometric partition (tree.cpp:608-617), from
+    the library's own row partitioner or from a --parts file; both runs agree
+    with the library flow on the same partition."""
+    import spaqr_public_amd as sq
+    A = gen.config_matrix("advdiff2d_32")
+    mfile, pfile = str(tmp_path / "A.mm"), str(tmp_path / "parts.mm")
+    gen.write_mm(mfile, A)
+    out = run_cli("-m", mfile, "-t", "1e-2")
+    assert "Using Hypergraph Partition" in out and "GMRES converged!" in out
+    one, it, res = parse(out)
+    ref = sq.run_flow(A, None, None, 1e-2)
+    assert abs(one - ref["oneshot_res"]) <= 1e-3 * ref["oneshot_res"] and abs(it - ref["iters"]) <= 1
+    assert res <= 1e-12
+    parts = (np.arange(A.shape[0]) * 8 // A.shape[0]).astype(float)  # 4 levels: 8 parts, by row blocks
+    gen.write_mm_array(pfile, parts[:, None])
+    out_p = run_cli("-m", mfile, "-t", "1e-2", "--parts", pfile)
+    assert "Row partition loaded" in out_p and "GMRES converged!" in out_p
+    ref_p = sq.run_flow(A, None, None, 1e-2, parts=parts.astype(np.int32))
+    one_p, it_p, _ = parse(out_p)
+    print(f"CLI without coordinates: one-shot {one:.3e} / {one_p:.3e} (library {ref['oneshot_res']:.3e} / "
+          f"{ref_p['oneshot_res']:.3e}), GMRES {it} / {it_p} its")
+    assert abs(one_p - ref_p["oneshot_res"]) <= 1e-3 * ref_p["oneshot_res"] and abs(it_p - ref_p["iters"]) <= 1

@@ -11,7 +11,9 @@ residual within 2x of the oracle's; GMRES iterations within +-1; final
 relative residual <= 1e-12 in FP64 (or within 2x of the oracle's true
 residual at the same stopping rule, which is 1.45e-12 on the high-contrast
 matrix).  The mixed form (FP32 factors, FP64
-refinement) reports its one-shot error against the FP64 oracle solution.
+refinement): the distance of its one-shot solution from the FP64 oracle's is
+under a tenth of the FP64 one-shot's own error (measured against the refined
+solution).
 """
 import hashlib
 import os
@@ -92,12 +94,16 @@ def test_fullsize_mixed_hicontrast(gpu_lib):
     check_sparsified(r["tree"], ref, "hicontrast3d_64 FP32")
     x32 = r["tree"].solve(r["b"])
     x64 = ref["x_oneshot_f32"].astype(np.float64)
+    xs = r["x"]  # refined to a 1e-12 residual: the solution to plotting accuracy
     err = np.linalg.norm(x32 - x64) / np.linalg.norm(x64)
+    err64 = np.linalg.norm(x64 - xs) / np.linalg.norm(xs)
     print(f"hicontrast3d_64 mixed: one-shot {r['oneshot_res']:.3e} (FP64 oracle {float(ref['oneshot_res']):.3e}), "
-          f"|x32 - x64|/|x64| = {err:.2e}, refinement {r['iters']} inner its (oracle GMRES {int(ref['iters'])}), "
-          f"final {r['final_res']:.2e}")
+          f"|x32 - x64|/|x64| = {err:.2e} (FP64 one-shot error {err64:.2e}), refinement {r['iters']} inner its "
+          f"(oracle GMRES {int(ref['iters'])}), final {r['final_res']:.2e}")
     assert r["oneshot_res"] <= 2.0 * float(ref["oneshot_res"])
-    assert err < 1e-3
+    # FP32 factors move the one-shot solution by well under the FP64
+    # one-shot's own error (the truncation at t = 1e-2 dominates rounding)
+    assert err < 0.1 * err64
     assert r["final_res"] <= 1e-12
     assert r["iters"] <= 3 * int(ref["iters"]) + 6
     r["M"].close()

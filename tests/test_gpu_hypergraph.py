@@ -1,0 +1,73 @@
+"""Non-geometric path on the GPU (no coordinates, tree.cpp:608-617): the
+library partitions the rows itself (spaqr_row_parts), labels the columns
+(HypergraphPartition + getInterfacesUsingA_HUND, partition.cpp:450-499,
+582-623) and factors; the oracle gets the same row partition.  Bar: the same
+permutations and cluster tree, the same sparsified interfaces with ranks
+agreeing on at least 95 %, one-shot residual within 2x of the oracle's,
+Krylov iterations within +-1 and the final residual at 1e-12 (1e-10 for
+CGLS on the least-squares matrix)."""
+import numpy as np
+import pytest
+
+from spaqr_public_amd import gen
+
+pytestmark = pytest.mark.gpu
+
+STRUCT_COLS = [0, 1, 2, 3, 4, 5, 6, 7, 11]  # ids, column start, creation order (sizes follow ranks)
+
+
+@pytest.mark.parametrize("name,tol", [("advdiff2d_64", 1e-3), ("advdiff3d_16", 1e-2), ("hicontrast3d_16", 1e-2),
+                                      ("ls_lapgrad_32", 1e-2)])
+def test_hypergraph_flow(oracle, gpu_lib, name, tol):
+    import spaqr_public_amd as sq
+    A = gen.config_matrix(name)
+    At = A.T.tocsc() if A.shape[0] < A.shape[1] else A
+    nlevels = sq.default_levels(At.shape[1])
+    parts = sq.row_parts(At, nlevels)  # column scaling keeps the pattern
+    res = 1e-12 if At.shape[0] == At.shape[1] else 1e-10
+    oracle.set_backend("own")
+    ref = oracle.run_reference_flow(A, None, None, tol, nlevels=nlevels, parts=parts, res=res)
+    print(f"{name}: row parts from the library, {nlevels} levels", flush=True)
+    gpu = sq.run_flow(A, None, None, tol, nlevels=nlevels, res=res)  # the library partitions by itself
+    cg, rg = gpu["tree"].perms()
+    co, ro = ref["tree"].perms()
+    assert np.array_equal(cg, co) and np.array_equal(rg, ro)
+    assert np.array_equal(gpu["tree"].cluster_table()[:, STRUCT_COLS], ref["tree"].cluster_table()[:, STRUCT_COLS])
+    sg, so = gpu["tree"].sparsified(), ref["tree"].sparsified()
+    assert sg.shape == so.shape and np.array_equal(sg[:, :4], so[:, :4])
+    agree = np.mean(sg[:, 4] == so[:, 4]) if len(so) else 1.0
+    print(f"{name}: top {gpu['top_size']} (oracle {ref['tree'].counters()['top_size']}), {len(so)} sparsified, ranks agree "
+          f"{100 * agree:.1f} %, one-shot {gpu['oneshot_res']:.3e} (oracle {ref['oneshot_res']:.3e}), "
+          f"{gpu['iters']} its (oracle {ref['iters']}), final {gpu['final_res']:.2e}")
+    assert agree >= 0.95
+    assert gpu["oneshot_res"] <= 2.0 * ref["oneshot_res"] + 1e-14
+    assert abs(gpu["iters"] - ref["iters"]) <= 1
+    assert gpu["final_res"] <= res
+    gpu["M"].close()
+    gpu["tree"].close()
+
+
+def test_given_row_parts(oracle, gpu_lib):
+    """A caller's row partition (a PaToH run elsewhere, say) replaces the
+    library's: here the geometric halves of a 32 x 32 grid by row index."""
+    import spaqr_public_amd as sq
+    A = gen.config_matrix("advdiff2d_32")
+    nlevels = 3
+    parts = (np.arange(A.shape[0]) * 4 // A.shape[0]).astype(np.int32)
+    ref = oracle.run_reference_flow(A, None, None, 1e-3, nlevels=nlevels, parts=parts)
+    gpu = sq.run_flow(A, None, None, 1e-3, nlevels=nlevels, parts=parts)
+    assert np.array_equal(gpu["tree"].perms()[0], ref["tree"].perms()[0])
+    assert np.array_equal(gpu["tree"].cluster_table()[:, STRUCT_COLS], ref["tree"].cluster_table()[:, STRUCT_COLS])
+    assert gpu["oneshot_res"] <= 2.0 * ref["oneshot_res"] + 1e-14
+    assert gpu["final_res"] <= 1e-12
+    gpu["M"].close()
+    gpu["tree"].close()
+
+
+def test_bad_row_parts_rejected(gpu_lib):
+    import spaqr_public_amd as sq
+    A = gen.config_matrix("advdiff2d_32")
+    with pytest.raises(sq.SpaqrError, match="outside"):
+        sq.run_flow(A, None, None, 1e-3, nlevels=3, parts=np.full(A.shape[0], 4, dtype=np.int32))
+    with pytest.raises(sq.SpaqrError, match="length"):
+        sq.run_flow(A, None, None, 1e-3, nlevels=3, parts=np.zeros(10, dtype=np.int32))
