@@ -566,6 +566,18 @@ Edge* Tree::add_edge(Cluster* c1, Cluster* c2) {  // tree.cpp:251-261
   return e;
 }
 
+// |A^T B|_F for blocks of one row space (A: r x p, B: r x q)
+static double norm_atb(const Mat& A, const Mat& B) {
+  double s = 0;
+  for (int i = 0; i < A.cols(); ++i)
+    for (int j = 0; j < B.cols(); ++j) {
+      double d = 0;
+      for (int r = 0; r < A.rows(); ++r) d += A(r, i) * B(r, j);
+      s += d * d;
+    }
+  return std::sqrt(s);
+}
+
 void Tree::get_sparsity(Cluster* c) {  // tree.cpp:834-870
   CSet rs;
   std::set<SepID> cs;
@@ -576,7 +588,13 @@ void Tree::get_sparsity(Cluster* c) {  // tree.cpp:834-870
     rs.insert(n);
     for (auto ein : n->edgesIn) {
       if (ein->A21 && ein->n1 != c && !ein->n1->eliminated) {
-        if (check_if_valid(c->id, ein->n1->id, cs)) rs.insert(ein->n1);
+        Cluster* nin = ein->n1;
+        bool valid = check_if_valid(c->id, nin->id, cs);
+        // no coordinates: a cluster that does not name c as an interface
+        // must couple to it numerically through n (tree.cpp:855-858)
+        if (!geo && valid && !(nin->id.l == c->id.self || nin->id.r == c->id.self))
+          valid = norm_atb(*e->A21, *ein->A21) > 1e-14;
+        if (valid) rs.insert(nin);
       }
     }
   }

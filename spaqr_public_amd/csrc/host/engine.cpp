@@ -34,6 +34,8 @@ class Engine final : public EngineBase {
     bottoms_.resize(nlevels);
     info_.resize(nlevels);
     nb_ = panel_nb();
+    for (auto& row : qp3r_cap_)
+      for (int& c : row) c = -1;
     SPQ_HIP(host_malloc((void**)&rank_host_, sizeof(int) * 65536));
     qp3_err_ = (int*)perm_.alloc(256);
     SPQ_HIP(hipMemset(qp3_err_, 0, 256));
@@ -534,6 +536,10 @@ class Engine final : public EngineBase {
 
   // ------------------------------------------------------------ phases
   void get_sparsity(int c);
+  // numeric row-sparsity tests of the non-geometric path (tree.cpp:855-858)
+  struct SparsityTest { int c, nin, e, ein; };
+  std::vector<SparsityTest> sparsity_tests_;
+  void finish_sparsity();
   // fine_wave >= 0: the inline elimination of --scale 0 fine clusters of
   // that sparsify wave (ops recorded as FineQR with reflectors)
   void eliminate_level(const std::vector<int>& cs, int fine_wave = -1);
@@ -668,16 +674,17 @@ class Engine final : public EngineBase {
     if (panel_cap_ < 0) panel_cap_ = panel_r_capacity<T>();
     return panel_cap_;
   }
-  int qp3r_cap_[2][13] = {{-1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1},
-                          {-1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1}};
+  static constexpr int kRcSlots = (100 - 16) / 4 + 1;
+  int qp3r_cap_[2][kRcSlots];
   int qp3_rc_ = 0;  // SPAQR_QP3_RC: force the register form's rows per lane
   // rows per lane of the register form's kernels: 16 to 32 (256 and 1024
-  // threads), 40 and 48 (256 threads)
-  static std::vector<int> qp3r_rcs() { return {16, 20, 24, 28, 32, 40, 48}; }
+  // threads), 40 to 100 (256 threads; from 64 on one lane holds a column of
+  // up to rc rows, so a problem needs 4x fewer workgroups than at 4 lanes)
+  static std::vector<int> qp3r_rcs() { return {16, 20, 24, 28, 32, 40, 48, 64, 80, 100}; }
   // resident workgroups of the register-form QR with nt threads per
   // workgroup (largest LDS footprint, G = 64)
   int qp3r_cap(int nt, int rc) {
-    int& c = qp3r_cap_[nt == 1024 ? 1 : 0][(rc - 16) / 4];
+    int& c = qp3r_cap_[nt == 1024 ? 1 : 0][std::min((rc - 16) / 4, kRcSlots - 1)];
     if (c < 0) c = qp3r_capacity<T>(64, nt, rc);
     return c;
   }
@@ -816,7 +823,7 @@ void Engine<T>::assemble(const CscMat& A) {  // tree.cpp:725-778
 }
 
 template <typename T>
-void Engine<T>::get_sparsity(int c) {  // tree.cpp:834-870 (+ check_if_valid 168-224, geometric)
+void Engine<T>::get_sparsity(int c) {  // tree.cpp:834-870 (+ check_if_valid 168-224)
   std::vector<SepID> colsp;
   for (int e : cl_[c].out) colsp.push_back(cl_[ed_[e].n2].id.self);
   std::sort(colsp.begin(), colsp.end());
@@ -834,8 +841,10 @@ void Engine<T>::get_sparsity(int c) {  // tree.cpp:834-870 (+ check_if_valid 168
       return s.sep / (1 << -dd) == cs.sep;
     };
     if (!side_ok(nid.l) && !side_ok(nid.r)) return false;
-    if (!has(ns)) return false;
-    if (!has(nid.l) && !has(nid.r)) return false;
+    if (geo_) {  // a separator may only connect through the column sparsity
+      if (!has(ns)) return false;
+      if (!has(nid.l) && !has(nid.r)) return false;
+    }
     return true;
   };
   // the distinct clusters found, marked as they are first accepted (a
@@ -853,11 +862,63 @@ void Engine<T>::get_sparsity(int c) {  // tree.cpp:834-870 (+ check_if_valid 168
     for (int ein : cl_[n].in) {
       const int nin = ed_[ein].n1;
       if (mark_[nin] == stamp) continue;
-      if (ed_[ein].alive && nin != c && !cl_[nin].elim && valid(cl_[nin].id)) take(nin);
+      if (!ed_[ein].alive || nin == c || cl_[nin].elim || !valid(cl_[nin].id)) continue;
+      // without coordinates, a cluster that does not name c as an interface
+      // must also couple to c numerically through n: |A(n,c)^T A(n,nin)| >
+      // 1e-14 (tree.cpp:855-858); decided for the whole level at once
+      if (!geo_ && !(cl_[nin].id.l == cs || cl_[nin].id.r == cs)) {
+        sparsity_tests_.push_back(SparsityTest{c, nin, e, ein});
+        continue;
+      }
+      take(nin);
     }
   }
   std::sort(rs.begin(), rs.end());
   cl_[c].rsp = rs;
+}
+
+// The numeric tests get_sparsity left for this level: P = A(n,c)^T A(n,nin)
+// by batched GEMMs, |P|_F per product, one read-back; a passing cluster joins
+// c's row sparsity (kept sorted).
+template <typename T>
+void Engine<T>::finish_sparsity() {
+  if (sparsity_tests_.empty()) return;
+  scratch_.rewind();
+  std::vector<GemmDesc<T>> gds;
+  std::vector<RowNormDesc<T>> nds;
+  std::vector<int> ncnt;
+  double* norms = (double*)scratch_.alloc(sizeof(double) * sparsity_tests_.size());
+  std::vector<char> empty(sparsity_tests_.size(), 0);
+  for (size_t q = 0; q < sparsity_tests_.size(); ++q) {
+    const SparsityTest& t = sparsity_tests_[q];
+    const Block& a = ed_[t.e].a;    // rows of n x cols of c
+    const Block& b = ed_[t.ein].a;  // rows of n x cols of nin
+    if (a.rows == 0 || a.cols == 0 || b.cols == 0) { empty[q] = 1; continue; }  // norm 0: no coupling
+    T* P = alloc_scr((size_t)std::max(a.cols * b.cols, 1));
+    gds.push_back(gd<T>(a.p, a.ld, b.p, b.ld, P, std::max(a.cols, 1), a.cols, b.cols, a.rows,
+                        gemm_flags(true, false), T(1), T(0)));
+    // |P|_F as the norm of one row of a.cols * b.cols entries
+    nds.push_back(RowNormDesc<T>{P, norms + q, 1, 0, 1, a.cols * b.cols});
+    ncnt.push_back(1);
+  }
+  if (gds.empty()) { sparsity_tests_.clear(); return; }
+  gemm_batch<T>(gds, "sparsity");
+  std::vector<int> nst;
+  const int nt = prefix_tiles(ncnt, nst);
+  const RowNormDesc<T>* dn = stager_.push(nds);
+  const int* dst = stager_.push(nst);
+  stager_.flush();
+  launch_rownorm<T>(stream_, dn, dst, (int)nds.size(), nt);
+  std::vector<double> h(sparsity_tests_.size());
+  SPQ_HIP(hipMemcpyAsync(h.data(), norms, sizeof(double) * h.size(), hipMemcpyDeviceToHost, stream_));
+  sync();
+  for (size_t q = 0; q < sparsity_tests_.size(); ++q) {
+    const SparsityTest& t = sparsity_tests_[q];
+    if (empty[q] || !(h[q] > 1e-14)) continue;
+    auto& rs = cl_[t.c].rsp;
+    if (!std::binary_search(rs.begin(), rs.end(), t.nin)) rs.insert(std::lower_bound(rs.begin(), rs.end(), t.nin), t.nin);
+  }
+  sparsity_tests_.clear();
 }
 
 // tree.cpp:780-832 (eliminate_cluster, householder, update_cluster) for every
@@ -2134,6 +2195,7 @@ int Engine<T>::factorize() {  // tree.cpp:1309-1437
     for (int s : bottoms_[cur_])
       if (cl_[s].level() == ilvl_) lev.push_back(s);
     for (int s : lev) get_sparsity(s);
+    finish_sparsity();
     const double t1 = wall();
     eliminate_level(lev);
     if (timing_) sync();

@@ -206,8 +206,8 @@ __global__ __launch_bounds__(kQThreads) void k_qp3(const Qp3Desc<T>* __restrict_
 // ---------------------------------------------------------------------------
 // Register form.  Column j of C lives in the registers of an aligned group of
 // G lanes (G a power of two): lane l of the group holds the RC contiguous
-// rows [l*RC, (l+1)*RC) (rows >= k zero; RC in {16, 20, 24, 28, 32} chosen
-// per problem to pad k the least), so C is read once and written
+// rows [l*RC, (l+1)*RC) (rows >= k zero; RC from 16 to 100, chosen per
+// batch), so C is read once and written
 // once and a pivot step moves only O(k) values per workgroup.  Per step every
 // workgroup publishes its local-best column next to its (norm, position,
 // column) partial; after one inter-workgroup barrier all workgroups pick the
@@ -244,8 +244,9 @@ template <int RC>
 __device__ __forceinline__ int vslot(int r) { return r + r / RC; }
 
 // resident workgroups per CU the register budget is sized for: 4 waves per
-// SIMD up to 32 rows per lane, 3 beyond (256-thread workgroups only)
-constexpr int qp3r_min_blocks(int nt, int rc) { return rc <= 32 ? 1024 / nt : 768 / nt; }
+// SIMD up to 32 rows per lane, 3 up to 48, 2 beyond (256-thread workgroups
+// only)
+constexpr int qp3r_min_blocks(int nt, int rc) { return rc <= 32 ? 1024 / nt : (rc <= 48 ? 768 / nt : 512 / nt); }
 template <typename T, int NT, int RC>
 __global__ __launch_bounds__(NT, qp3r_min_blocks(NT, RC)) void k_qp3r(const Qp3Desc<T>* __restrict__ descs,
                                                      const int* __restrict__ wstart, int nd) {
@@ -425,22 +426,46 @@ __global__ __launch_bounds__(NT, qp3r_min_blocks(NT, RC)) void k_qp3r(const Qp3D
     bc = -1;
     if (has && pos > i) {  // uniform over the group
       const int li = i - rbase;  // row i relative to this lane's first row
-      // dot product in chunks of 4 LDS reads (bounds the live registers)
+      // The reflector is read from LDS in chunks of CH rows, the next chunk's
+      // reads issued before the current chunk is used (two chunks in flight
+      // bound the live registers; the compiler fence keeps later chunks from
+      // being hoisted).  Rows above i hold zeros in the reflector, so the dot
+      // product and the update run over every row unmasked: those rows add
+      // nothing to w and are left unchanged (c - w * 0 = c).
+      constexpr int CH = sizeof(T) == 4 && RC % 8 == 0 ? 8 : 4;  // 16 VGPRs of buffers
+      T vb[2][CH];
+      auto fetch = [&](int q0, int buf) {
+#pragma unroll
+        for (int q = 0; q < CH; ++q) vb[buf][q] = vmy[q0 + q];
+      };
       double w = 0;
+      fetch(0, 0);
 #pragma unroll
-      for (int q0 = 0; q0 < RC; q0 += 4) {
-#pragma unroll
-        for (int q = q0; q < q0 + 4; ++q) w += (double)vmy[q] * (double)c[q];
+      for (int q0 = 0; q0 < RC; q0 += CH) {
+        const int cur = (q0 / CH) & 1;
+        if (q0 + CH < RC) fetch(q0 + CH, cur ^ 1);
         asm volatile("" ::: "memory");
+#pragma unroll
+        for (int q = 0; q < CH; ++q) w += (double)vb[cur][q] * (double)c[q0 + q];
       }
       w = group_sum(w, G) * tau;
       if (w != 0.0) {
+        // FP64 storage: the update in FP64.  FP32 storage: one FP32 FMA with
+        // w rounded once (the stored result is FP32 either way); forming it
+        // in FP64 would keep an FP64 copy of every value of the column live
+        // from the dot product above, doubling the register footprint.
+        const T wt = (T)w;
+        fetch(0, 0);
 #pragma unroll
-        for (int q0 = 0; q0 < RC; q0 += 4) {
-#pragma unroll
-          for (int q = q0; q < q0 + 4; ++q)
-            if (q >= li) c[q] = (T)((double)c[q] - w * (double)vmy[q]);
+        for (int q0 = 0; q0 < RC; q0 += CH) {
+          const int cur = (q0 / CH) & 1;
+          if (q0 + CH < RC) fetch(q0 + CH, cur ^ 1);
           asm volatile("" ::: "memory");
+#pragma unroll
+          for (int q = 0; q < CH; ++q) {
+            if constexpr (sizeof(T) == 8) c[q0 + q] = (T)((double)c[q0 + q] - w * (double)vb[cur][q]);
+            else c[q0 + q] = fmaf(-wt, vb[cur][q], c[q0 + q]);
+          }
         }
       }
       T mine = T(0);
@@ -518,6 +543,9 @@ template <typename T>
 const void* qp3r_fn(int nt, int rc) {
   if (nt == 256 && rc == 40) return (const void*)k_qp3r<T, 256, 40>;
   if (nt == 256 && rc == 48) return (const void*)k_qp3r<T, 256, 48>;
+  if (nt == 256 && rc == 64) return (const void*)k_qp3r<T, 256, 64>;
+  if (nt == 256 && rc == 80) return (const void*)k_qp3r<T, 256, 80>;
+  if (nt == 256 && rc == 100) return (const void*)k_qp3r<T, 256, 100>;
   return nt == 1024 ? qp3r_fn<T, 1024>(rc) : (nt == 256 ? qp3r_fn<T, 256>(rc) : nullptr);
 }
 }  // namespace

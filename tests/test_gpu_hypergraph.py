@@ -40,6 +40,10 @@ def test_hypergraph_flow(oracle, gpu_lib, name, tol):
           f"{100 * agree:.1f} %, one-shot {gpu['oneshot_res']:.3e} (oracle {ref['oneshot_res']:.3e}), "
           f"{gpu['iters']} its (oracle {ref['iters']}), final {gpu['final_res']:.2e}")
     assert agree >= 0.95
+    # the row sparsity (with its numeric test, tree.cpp:855-858) decides the
+    # fill: R and H sizes agree with the oracle's
+    co = ref["tree"].counters()
+    assert (gpu["nnzR"], gpu["nnzH"]) == (co["nnzR"], co["nnzH"])
     assert gpu["oneshot_res"] <= 2.0 * ref["oneshot_res"] + 1e-14
     assert abs(gpu["iters"] - ref["iters"]) <= 1
     assert gpu["final_res"] <= res

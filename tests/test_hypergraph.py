@@ -156,3 +156,31 @@ def test_row_parts_rejects_bad_levels(gpu_lib):
     import spaqr_public_amd as sq
     with pytest.raises(sq.SpaqrError):
         sq.row_parts(chain(8), 0)
+
+
+@pytest.mark.parametrize("name", ["advdiff2d_32", "advdiff3d_16", "ls_lapgrad_16"])
+def test_nongeometric_exact_at_tol0(gpu_lib, oracle, name):
+    """At tol 0 (no sparsification) the non-geometric tree is an exact sparse
+    QR: the one-shot solve equals spsolve / lstsq.  A row sparsity that drops
+    a cluster coupling numerically to c through a neighbour (the
+    column-sparsity rule of the geometric path, or no numeric test at all,
+    tree.cpp:846-866) leaves fill out and fails this; the numeric test
+    |A(n,c)^T A(n,nin)| > 1e-14 only drops products that are exactly zero."""
+    import scipy.sparse.linalg as spl
+    import spaqr_public_amd as sq
+    A = gen.config_matrix(name)
+    At = A.T.tocsc() if A.shape[0] < A.shape[1] else A
+    nl = sq.default_levels(At.shape[1])
+    parts = sq.row_parts(At, nl)
+    oracle.set_backend("own")
+    r = oracle.run_reference_flow(A, None, None, 0.0, nlevels=nl, parts=parts, iters=1)
+    As, b = r["A"], r["b"]
+    if As.shape[0] == As.shape[1]:
+        x = r["tree"].solve(b)
+        xe = spl.spsolve(As.tocsc(), b)
+    else:
+        x = r["tree"].solve_nrml(As.T @ b)
+        xe = np.linalg.lstsq(As.toarray(), b, rcond=None)[0]
+    err = np.linalg.norm(x - xe) / np.linalg.norm(xe)
+    print(f"{name}: non-geometric tree at tol 0, |x - x_exact| / |x_exact| = {err:.2e}")
+    assert err < 1e-10
