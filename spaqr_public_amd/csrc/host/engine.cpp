@@ -1293,7 +1293,7 @@ void Engine<T>::rrqr_batch(std::vector<RrqrJob>& jobs, int rc_force) {
     d.Ct = jb.Ct; d.V = jb.V; d.tau = jb.tau;
     d.vn = G ? nullptr : (double*)scratch_.alloc(sizeof(double) * 2 * N);
     d.pos = (int*)scratch_.alloc(sizeof(int) * (G ? (size_t)N : 2 * (size_t)N * nwg));
-    d.part_v = (double*)scratch_.alloc(sizeof(double) * 2 * nwg);
+    d.part_v = (double*)scratch_.alloc(sizeof(double) * 4 * nwg);  // (norm, squared norm below i) x 2 buffers
     d.part_p = (int*)scratch_.alloc(sizeof(int) * 4 * nwg);
     d.beta = (double*)scratch_.alloc(sizeof(double) * std::max(jb.k, 1));
     sync_off.push_back(sync_words);
@@ -1302,7 +1302,7 @@ void Engine<T>::rrqr_batch(std::vector<RrqrJob>& jobs, int rc_force) {
     d.err = qp3_err_;
     d.rank_out = ranks_dev + q;
     d.ldc = N; d.k = jb.k; d.N = jb.N; d.ldv = jb.k; d.tol = tol_; d.nwg = nwg; d.G = G; d.rc = G ? rc : 0;
-    d.part_col = G ? alloc_scr((size_t)2 * nwg * jb.k) : nullptr;
+    d.part_col = G ? alloc_scr((size_t)2 * nwg * G * rc) : nullptr;  // a column slot of G * rc rows per workgroup
     qd.push_back(d);
     const int kk = std::min(jb.k, jb.N);
     if (!trial_) flops_ += 4.0 * jb.k * jb.N * kk - 2.0 * (jb.k + jb.N) * kk * kk + 4.0 / 3.0 * kk * kk * kk;

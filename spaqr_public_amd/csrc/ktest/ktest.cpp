@@ -4,6 +4,7 @@
 // the oracle's LAPACK restatement.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -205,13 +206,13 @@ int ktest_qp3(int k, int N, double* Ct, double* V, double* tau, double tol, int 
   double *dC = dup(Ct, (size_t)N * k), *dV = dup<double>(nullptr, (size_t)k * k), *dt = dup<double>(nullptr, k);
   double* vn = dup<double>(nullptr, 2 * (size_t)N);
   int* pos = dup<int>(nullptr, 2 * (size_t)N * nwg);
-  double* pv = dup<double>(nullptr, 2 * (size_t)nwg);
+  double* pv = dup<double>(nullptr, 4 * (size_t)nwg);
   int* pp = dup<int>(nullptr, 4 * (size_t)nwg);
   double* beta = dup<double>(nullptr, k);
   unsigned int* sync = dup<unsigned int>(nullptr, barrier_words(nwg));
   int* err = dup<int>(nullptr, 1);
   int* rk = dup<int>(nullptr, 1);
-  double* pcol = dup<double>(nullptr, 2 * (size_t)nwg * k);
+  double* pcol = dup<double>(nullptr, 2 * (size_t)nwg * std::max(k, G * rcv));
   Qp3Desc<double> d{dC, dV, dt, vn, pos, pv, pp, beta, sync, err, rk, pcol, N, k, N, k, tol, nwg, G, rcv};
   Qp3Desc<double>* dd = dup(&d, 1);
   int zero = 0;
@@ -252,13 +253,13 @@ int ktest_qp3_bench(int k, int N, const double* Ct, double tol, int form, int nw
   double *dV = dup<double>(nullptr, (size_t)k * k), *dt = dup<double>(nullptr, k);
   double* vn = dup<double>(nullptr, 2 * (size_t)N);
   int* pos = dup<int>(nullptr, 2 * (size_t)N * nwg);
-  double* pv = dup<double>(nullptr, 2 * (size_t)nwg);
+  double* pv = dup<double>(nullptr, 4 * (size_t)nwg);
   int* pp = dup<int>(nullptr, 4 * (size_t)nwg);
   double* beta = dup<double>(nullptr, k);
   unsigned int* sync = dup<unsigned int>(nullptr, barrier_words(nwg));
   int* err = dup<int>(nullptr, 1);
   int* rk = dup<int>(nullptr, 1);
-  double* pcol = dup<double>(nullptr, 2 * (size_t)nwg * k);
+  double* pcol = dup<double>(nullptr, 2 * (size_t)nwg * std::max(k, G * rcv));
   Qp3Desc<double> d{dC, dV, dt, vn, pos, pv, pp, beta, sync, err, rk, pcol, N, k, N, k, tol, nwg, G, rcv};
   Qp3Desc<double>* dd = dup(&d, 1);
   int zero = 0;
