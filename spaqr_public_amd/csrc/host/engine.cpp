@@ -13,6 +13,7 @@
 #include <mutex>
 #include <numeric>
 #include <set>
+#include <type_traits>
 
 #include "../kernels/launch.hpp"
 #include "tuner.hpp"
@@ -368,9 +369,10 @@ class Engine final : public EngineBase {
   template <typename U>
   void gemm_batch(std::vector<GemmDesc<U>>& ds, const char* fam = "gemm") {  // ds may get split-K fields
     if (ds.empty()) return;
-    auto tiles_of = [&](const GemmDesc<U>& d) { return gemm_tiles(d.m, d.n); };
+    std::vector<int>& cnt = tile_cnt_;
+    cnt.resize(ds.size());
     int tiles = 0;
-    for (auto& d : ds) tiles += tiles_of(d);
+    for (size_t i = 0; i < ds.size(); ++i) tiles += cnt[i] = gemm_tiles(ds[i].m, ds[i].n);
     const int want = 4 * num_cus_;
     bool split = false;
     if (tiles < want) {
@@ -378,7 +380,8 @@ class Engine final : public EngineBase {
       // batch is ordered after it on the stream, so the arena is reused.
       gws_.rewind();
       size_t room = kSplitWsBytes;  // bytes, allocations rounded to 256
-      for (auto& d : ds) {
+      for (size_t i = 0; i < ds.size(); ++i) {
+        auto& d = ds[i];
         if (d.m <= 0 || d.n <= 0) continue;
         const size_t mn = (size_t)d.m * d.n;
         int ks = std::min(std::max(1, want / std::max(tiles, 1)), std::max(1, d.k / 256));
@@ -392,8 +395,9 @@ class Engine final : public EngineBase {
         }
       }
     }
-    std::vector<int> cnt(ds.size()), st;
-    for (size_t i = 0; i < ds.size(); ++i) cnt[i] = tiles_of(ds[i]) * ds[i].ksplit;
+    for (size_t i = 0; i < ds.size(); ++i)
+      if (ds[i].ksplit > 1) cnt[i] *= ds[i].ksplit;
+    std::vector<int>& st = tile_st_;
     const int nt = prefix_tiles(cnt, st);
     double fl = 0, by = 0;
     for (auto& d : ds) {
@@ -510,11 +514,10 @@ class Engine final : public EngineBase {
       // once per problem in the same launch as the Y = V_b^T C of its
       // targets (W and Y live in ws_, rewound per step); T_b's strict lower
       // part is stored as zeros, so it enters as a dense factor
-      std::vector<GemmDesc<U>> g1, g3;
-      size_t ntg = 0;
-      for (int q : live) ntg += 1 + jobs[q].extra.size();
-      g1.reserve(ntg + live.size()); g3.reserve(ntg);
-      std::vector<LarfbTarget<U>> tg;
+      auto& bufs = larfb_bufs<U>();
+      auto &g1 = bufs.g1, &g3 = bufs.g3;
+      auto& tg = bufs.tg;
+      g1.clear(); g3.clear();
       auto ws_alloc = [this](int64_t elems) { return (U*)ws_.alloc(sizeof(U) * (size_t)std::max<int64_t>(elems, 1)); };
       for (int q : live) {
         auto& j = jobs[q];
@@ -533,6 +536,23 @@ class Engine final : public EngineBase {
       gemm_batch(g3, "gemm_c");
     }
   }
+
+  // tables of the larfb block steps and gemm_batch's tile counts, kept for
+  // the whole factorization: a level-0 step plans ~300k descriptors, and
+  // fresh vectors per step cost more host time than filling them
+  template <typename U>
+  struct LarfbBufs {
+    std::vector<GemmDesc<U>> g1, g3;
+    std::vector<LarfbTarget<U>> tg;
+  };
+  LarfbBufs<double> larfb_d_;
+  LarfbBufs<float> larfb_f_;
+  template <typename U>
+  LarfbBufs<U>& larfb_bufs() {
+    if constexpr (std::is_same<U, double>::value) return larfb_d_;
+    else return larfb_f_;
+  }
+  std::vector<int> tile_cnt_, tile_st_;
 
   // ------------------------------------------------------------ phases
   void get_sparsity(int c);
@@ -2249,6 +2269,10 @@ int Engine<T>::factorize() {  // tree.cpp:1309-1437
              ilvl_, li.t_fill, li.t_elmn, li.t_shift, li.t_scale, li.t_sprsfy, li.t_merge, ts.first, ts.second,
              (double)ts.first / (double)ts.second);
   }
+  larfb_d_ = {};
+  larfb_f_ = {};
+  tile_cnt_ = {};
+  tile_st_ = {};
   const double tbp = wall();
   build_solve_program();
   // the apply's graph is instantiated here, so solves (and the Krylov drivers

@@ -33,6 +33,16 @@ enum : int { kStructNone = 0, kStructUnitLower = 1, kStructUpper = 2, kStructLow
 constexpr int gemm_flags(bool ta, bool tb, int sa = 0, int sb = 0) {
   return (ta ? kTransA : 0) | (tb ? kTransB : 0) | (sa << 4) | (sb << 6);
 }
+// Tile shape of a descriptor (k_gemm): 64 x 64, 32 x 128 (m <= 32) or
+// 128 x 32 (n <= 32); gemm_tiles = workgroups of one K chunk.  Inline: the
+// planner counts tiles for every descriptor of every launch.
+constexpr int gemm_shape(int m, int n) { return m <= 32 ? 1 : (n <= 32 ? 2 : 0); }
+constexpr int gemm_shape_m(int shape) { return shape == 1 ? 32 : (shape == 2 ? 128 : 64); }
+constexpr int gemm_shape_n(int shape) { return shape == 1 ? 128 : (shape == 2 ? 32 : 64); }
+constexpr int gemm_tiles(int m, int n) {
+  const int sh = gemm_shape(m, n), tm = gemm_shape_m(sh), tn = gemm_shape_n(sh);
+  return ((m + tm - 1) / tm) * ((n + tn - 1) / tn);
+}
 // Split-K (ksplit > 1): the K range is cut into ksplit chunks whose partial
 // products alpha * op(A) op(B) go to work (ksplit x m x n, ld m); a reduce
 // pass sums them in chunk order and applies beta (deterministic).

@@ -37,9 +37,6 @@ __device__ __forceinline__ T structured(const T* __restrict__ M, int ld, int r, 
 constexpr int TK = 16;
 constexpr int kTileMaxM = 128, kTileMaxN = 128;
 
-__host__ __device__ inline int gemm_shape(int m, int n) { return m <= 32 ? 1 : (n <= 32 ? 2 : 0); }
-__host__ __device__ inline int gemm_shape_m(int shape) { return shape == 1 ? 32 : (shape == 2 ? 128 : 64); }
-__host__ __device__ inline int gemm_shape_n(int shape) { return shape == 1 ? 128 : (shape == 2 ? 32 : 64); }
 
 // Operands: lane l holds A(l % 16, l / 16) of the 16 x 4 A block and
 // B(l / 16, l % 16) of the 4 x 16 B block.  Accumulator c of lane l holds
@@ -203,10 +200,6 @@ __global__ __launch_bounds__(256) void k_gemm_reduce(const GemmDesc<T>* __restri
   *c = (D.beta == T(0)) ? s : s + D.beta * (*c);
 }
 
-int gemm_tiles(int m, int n) {
-  const int sh = gemm_shape(m, n), tm = gemm_shape_m(sh), tn = gemm_shape_n(sh);
-  return ((m + tm - 1) / tm) * ((n + tn - 1) / tn);
-}
 int gemm_tile_k() { return TK; }
 
 template <typename T>

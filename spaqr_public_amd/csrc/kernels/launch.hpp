@@ -18,8 +18,7 @@ void launch_rownorm(hipStream_t s, const RowNormDesc<T>* d, const int* tstart, i
 template <typename T>
 void launch_rowgather(hipStream_t s, const RowGatherDesc<T>* d, const int* tstart, int nd, int ntiles);
 
-// gemm.hip (tiles per descriptor: gemm_tiles(m, n) * ksplit)
-int gemm_tiles(int m, int n);
+// gemm.hip (tiles per descriptor: gemm_tiles(m, n) * ksplit, descs.hpp)
 int gemm_tile_k();
 template <typename T>
 void launch_gemm_reduce(hipStream_t s, const GemmDesc<T>* d, const int* tstart, int nd, int ntiles);
