@@ -116,15 +116,29 @@ class Arena {
   size_t cur_ = 0, used_ = 0, reserved_ = 0;
 };
 
+// Process-wide pool of staging buffers (pinned host or device memory of the
+// current device), so trees created one after another reuse them instead of
+// allocating and pinning again.  pool_take returns a buffer of at least
+// `bytes` (the smallest pooled one that fits, or the largest with `largest`,
+// else a new one) and sets `bytes` to its size; pool_give returns it to the pool of device `dev` (the device
+// current when it was taken; the caller's stream work on it must be
+// complete).  At most 2 GiB per kind and device stay pooled.
+char* pool_take(bool host, size_t& bytes, bool largest = false);
+void pool_give(bool host, char* p, size_t bytes, int dev);
+
 // Pinned host staging + device ring for descriptor tables.  push() copies a
 // table into the pinned ring, enqueues the H2D copy on the stream and returns
-// the device address.  The ring is reset at host synchronisation points;
-// overflow forces a synchronisation.
+// the device address.  The ring is reset at host synchronisation points.
+// A table that does not fit behind the last one moves to a ring of twice the
+// size (the old ring is kept until the next synchronisation, then pooled),
+// up to kMaxRing; past that the ring wraps, which waits for the stream.
 class Stager {
  public:
+  static constexpr size_t kMaxRing = size_t(1) << 30;
   Stager() {}
   ~Stager();
   void init(hipStream_t s, size_t bytes);
+  long n_grow = 0, n_wrap = 0;  // diagnostics
   template <typename X>
   const X* push(const std::vector<X>& v) {
     return (const X*)push_raw(v.data(), sizeof(X) * v.size());
@@ -141,6 +155,7 @@ class Stager {
       SPQ_HIP(hipStreamSynchronize(stream_));
     }
     off_ = 0; pend_ = 0; up_off_ = 0;
+    release_retired();
   }
   // Persistent copy (for solve programs): allocated from `perm` and kept.
   // The table goes through a pinned buffer with an asynchronous copy; the
@@ -160,9 +175,14 @@ class Stager {
   void flush_uploads();
 
  private:
+  struct Ring { char* host; char* dev; size_t host_bytes, dev_bytes; };
+  void release_retired();
   hipStream_t stream_ = nullptr;
+  int device_ = 0;
   char* host_ = nullptr;
   char* dev_ = nullptr;
+  size_t host_bytes_ = 0, dev_bytes_ = 0;
+  std::vector<Ring> retired_;              // outgrown rings still read by queued work
   size_t cap_ = 0, off_ = 0, pend_ = 0;  // [pend_, off_) not yet copied
   char* up_host_ = nullptr;               // pinned source of persistent uploads
   size_t up_cap_ = 0, up_off_ = 0;

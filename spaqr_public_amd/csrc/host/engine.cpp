@@ -2210,7 +2210,7 @@ int Engine<T>::factorize() {  // tree.cpp:1309-1437
   for (ilvl_ = 0; ilvl_ < nlevels_; ++ilvl_) {
     LevelInfo& li = info_[ilvl_];
     const double t0 = wall(), w0 = t_wait_, f0 = t_flush_, l0 = t_launch_, a0 = Arena::t_chunks;
-    const long n0 = n_launch_, c0 = Arena::n_chunks;
+    const long n0 = n_launch_, c0 = Arena::n_chunks, rg0 = stager_.n_grow, rw0 = stager_.n_wrap;
     std::vector<int> lev;
     for (int s : bottoms_[cur_])
       if (cl_[s].level() == ilvl_) lev.push_back(s);
@@ -2256,10 +2256,10 @@ int Engine<T>::factorize() {  // tree.cpp:1309-1437
       }
       fprintf(stderr,
               "lvl %d kernel ms:%s | wall %.1f (host busy %.1f: uploads %.1f, launches %.1f, %ld launches, %ld chunk "
-              "allocations %.1f) | elim %d in %d waves, sprs %d in %d waves\n",
+              "allocations %.1f, staging ring %ld grown %ld wrapped) | elim %d in %d waves, sprs %d in %d waves\n",
               ilvl_, line.c_str(), 1e3 * (t6 - t0), 1e3 * ((t6 - t0) - (t_wait_ - w0)), 1e3 * (t_flush_ - f0),
-              1e3 * (t_launch_ - l0), n_launch_ - n0, Arena::n_chunks - c0, 1e3 * (Arena::t_chunks - a0), li.n_elim,
-              li.elim_waves, li.n_sprs, li.sprs_waves);
+              1e3 * (t_launch_ - l0), n_launch_ - n0, Arena::n_chunks - c0, 1e3 * (Arena::t_chunks - a0),
+              stager_.n_grow - rg0, stager_.n_wrap - rw0, li.n_elim, li.elim_waves, li.n_sprs, li.sprs_waves);
     }
     auto ts = topsize();
     li.top_rows = ts.first; li.top_cols = ts.second;
