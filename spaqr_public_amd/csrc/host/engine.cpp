@@ -3,6 +3,7 @@
 // std::set<Cluster*> are kept sorted by creation sequence, which is the address
 // order of the reference's consecutive `new Cluster` allocations.
 #include "engine.hpp"
+#include "parallel.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -332,15 +333,20 @@ class Engine final : public EngineBase {
   // esrc / edst: element bytes of source and destination (a conversion when
   // they differ)
   void copy_blocks(const std::vector<CopyDesc>& ds, int esrc = (int)sizeof(T), int edst = (int)sizeof(T)) {
-    if (ds.empty()) return;
-    std::vector<int> cnt(ds.size()), st;
-    for (size_t i = 0; i < ds.size(); ++i) cnt[i] = iceil(std::max(ds[i].rows, 1), 32) * iceil(std::max(ds[i].cols, 1), 32);
-    const int nt = prefix_tiles(cnt, st);
+    copy_blocks(ds.data(), ds.size(), esrc, edst);
+  }
+  void copy_blocks(const CopyDesc* ds, size_t n, int esrc = (int)sizeof(T), int edst = (int)sizeof(T)) {
+    if (n == 0) return;
+    std::vector<int> cnt(n), st;
     double bytes = 0;
-    for (auto& d : ds) bytes += (double)(esrc + edst) * d.rows * d.cols;
-    const CopyDesc* dd = stager_.push(ds);
+    for (size_t i = 0; i < n; ++i) {
+      cnt[i] = iceil(std::max(ds[i].rows, 1), 32) * iceil(std::max(ds[i].cols, 1), 32);
+      bytes += (double)(esrc + edst) * ds[i].rows * ds[i].cols;
+    }
+    const int nt = prefix_tiles(cnt, st);
+    const CopyDesc* dd = (const CopyDesc*)stager_.push_raw(ds, sizeof(CopyDesc) * n);
     const int* dst = stager_.push(st);
-    timed("copy", 0.0, bytes, [&] { launch_copy(stream_, esrc, edst, dd, dst, (int)ds.size(), nt); });
+    timed("copy", 0.0, bytes, [&] { launch_copy(stream_, esrc, edst, dd, dst, (int)n, nt); });
   }
   void init_blocks(const std::vector<Block>& bs, int identity) {
     if (bs.empty()) return;
@@ -368,11 +374,31 @@ class Engine final : public EngineBase {
   // fam: timing family (every family runs k_gemm, plus k_gemm_reduce)
   template <typename U>
   void gemm_batch(std::vector<GemmDesc<U>>& ds, const char* fam = "gemm") {  // ds may get split-K fields
-    if (ds.empty()) return;
-    std::vector<int>& cnt = tile_cnt_;
-    cnt.resize(ds.size());
+    gemm_batch(ds.data(), ds.size(), fam);
+  }
+  template <typename U>
+  void gemm_batch(GemmDesc<U>* ds, size_t n, const char* fam = "gemm") {
+    if (n == 0) return;
+    std::vector<int>& cnt = tile_bufs().cnt;
+    cnt.resize(n);
+    // tile counts and the ledger's flops / bytes, in chunks joined in order
+    const int grain = 4096, nch = std::max(plan_chunks((int)n, grain), 1);
+    std::vector<int> ctiles(nch, 0);
+    std::vector<double> cfl(nch, 0.0), cby(nch, 0.0);
+    parallel_for((int)n, grain, [&](int ch, int b, int e) {
+      int t = 0;
+      double f = 0, y = 0;
+      for (int i = b; i < e; ++i) {
+        const GemmDesc<U>& d = ds[i];
+        t += cnt[i] = gemm_tiles(d.m, d.n);
+        f += 2.0 * d.m * d.n * d.k;
+        y += sizeof(U) * ((double)d.m * d.k + (double)d.k * d.n + (d.beta == U(0) ? 1.0 : 2.0) * d.m * d.n);
+      }
+      ctiles[ch] = t; cfl[ch] = f; cby[ch] = y;
+    });
     int tiles = 0;
-    for (size_t i = 0; i < ds.size(); ++i) tiles += cnt[i] = gemm_tiles(ds[i].m, ds[i].n);
+    double fl = 0, by = 0;
+    for (int ch = 0; ch < nch; ++ch) { tiles += ctiles[ch]; fl += cfl[ch]; by += cby[ch]; }
     const int want = 4 * num_cus_;
     bool split = false;
     if (tiles < want) {
@@ -380,7 +406,7 @@ class Engine final : public EngineBase {
       // batch is ordered after it on the stream, so the arena is reused.
       gws_.rewind();
       size_t room = kSplitWsBytes;  // bytes, allocations rounded to 256
-      for (size_t i = 0; i < ds.size(); ++i) {
+      for (size_t i = 0; i < n; ++i) {
         auto& d = ds[i];
         if (d.m <= 0 || d.n <= 0) continue;
         const size_t mn = (size_t)d.m * d.n;
@@ -395,27 +421,22 @@ class Engine final : public EngineBase {
         }
       }
     }
-    for (size_t i = 0; i < ds.size(); ++i)
+    for (size_t i = 0; i < n; ++i)
       if (ds[i].ksplit > 1) cnt[i] *= ds[i].ksplit;
-    std::vector<int>& st = tile_st_;
+    std::vector<int>& st = tile_bufs().st;
     const int nt = prefix_tiles(cnt, st);
-    double fl = 0, by = 0;
-    for (auto& d : ds) {
-      fl += 2.0 * d.m * d.n * d.k;
-      by += sizeof(U) * ((double)d.m * d.k + (double)d.k * d.n + (d.beta == U(0) ? 1.0 : 2.0) * d.m * d.n);
-    }
-    const GemmDesc<U>* dd = stager_.push(ds);
+    const GemmDesc<U>* dd = (const GemmDesc<U>*)stager_.push_raw(ds, sizeof(GemmDesc<U>) * n);
     const int* dst = stager_.push(st);
     std::vector<GemmDesc<U>> rd;
     std::vector<int> rcnt, rst;
     if (split)
-      for (auto& d : ds)
-        if (d.ksplit > 1) { rd.push_back(d); rcnt.push_back(iceil(d.m * d.n, 256)); }
+      for (size_t i = 0; i < n; ++i)
+        if (ds[i].ksplit > 1) { rd.push_back(ds[i]); rcnt.push_back(iceil(ds[i].m * ds[i].n, 256)); }
     const int rnt = prefix_tiles(rcnt, rst);
     const GemmDesc<U>* drd = rd.empty() ? nullptr : stager_.push(rd);
     const int* drst = rd.empty() ? nullptr : stager_.push(rst);
     timed(fam, fl, by, [&] {
-      launch_gemm<U>(stream_, dd, dst, (int)ds.size(), nt);
+      launch_gemm<U>(stream_, dd, dst, (int)n, nt);
       if (!rd.empty()) launch_gemm_reduce<U>(stream_, drd, drst, (int)rd.size(), rnt);
     });
   }
@@ -516,49 +537,115 @@ class Engine final : public EngineBase {
       // part is stored as zeros, so it enters as a dense factor
       auto& bufs = larfb_bufs<U>();
       auto &g1 = bufs.g1, &g3 = bufs.g3;
-      auto& tg = bufs.tg;
-      g1.clear(); g3.clear();
-      auto ws_alloc = [this](int64_t elems) { return (U*)ws_.alloc(sizeof(U) * (size_t)std::max<int64_t>(elems, 1)); };
-      for (int q : live) {
+      // the step's targets of job q: its trailing columns, then its extra
+      // matrices with columns, in that order (larfb_step_descs' order)
+      auto targets = [&](int q, std::vector<LarfbTarget<U>>& tg) {
         auto& j = jobs[q];
-        const int w = std::min(nb_, std::min(j.m, j.k) - jb);
-        const int rows = j.m - jb;
-        const U* Vb = j.P + jb + (int64_t)jb * j.ld;
-        const U* Tb = j.Tb + (int64_t)jb * nb_;
         tg.clear();
         const int c0 = jb + std::min(nb_, j.k - jb);
         if (j.k > c0) tg.push_back({j.P + jb + (int64_t)c0 * j.ld, j.ld, j.k - c0});
         for (auto& b : j.extra)
           if (b.cols > 0) tg.push_back({b.p + jb, b.ld, b.cols});
-        larfb_step_descs<U>(Vb, j.ld, Tb, nb_, rows, w, tg.data(), (int)tg.size(), ws_alloc, g1, g3);
-      }
-      gemm_batch(g1, "gemm_y");
-      gemm_batch(g3, "gemm_c");
+      };
+      // pass 1: per live job its descriptor counts and the bytes of its W
+      // and Y products (each allocation 256-aligned, as the arena places
+      // them), so every job's tables and scratch have fixed offsets
+      const int nl = (int)live.size();
+      std::vector<size_t> n1(nl + 1, 0), n3(nl + 1, 0), nby(nl + 1, 0);
+      auto al = [](size_t b) { return (b + 255) / 256 * 256; };
+      parallel_for(nl, 256, [&](int, int b, int e) {
+        std::vector<LarfbTarget<U>> tg;
+        for (int i = b; i < e; ++i) {
+          targets(live[i], tg);
+          if (tg.empty()) continue;
+          const int rows = jobs[live[i]].m - jb;
+          size_t by = al(sizeof(U) * (size_t)std::max<int64_t>((int64_t)std::max(rows, 1) * nb_, 1));
+          for (auto& t : tg) by += al(sizeof(U) * (size_t)std::max<int64_t>((int64_t)nb_ * t.cols, 1));
+          n1[i + 1] = 1 + tg.size();
+          n3[i + 1] = tg.size();
+          nby[i + 1] = by;
+        }
+      });
+      for (int i = 0; i < nl; ++i) { n1[i + 1] += n1[i]; n3[i + 1] += n3[i]; nby[i + 1] += nby[i]; }
+      if (g1.size() < n1[nl]) g1.resize(n1[nl]);
+      if (g3.size() < n3[nl]) g3.resize(n3[nl]);
+      char* wbase = nby[nl] ? (char*)ws_.alloc(nby[nl]) : nullptr;
+      // pass 2: the descriptors, written at those offsets
+      parallel_for(nl, 64, [&](int, int b, int e) {
+        std::vector<LarfbTarget<U>> tg;
+        struct Out {
+          GemmDesc<U>* p;
+          void push_back(const GemmDesc<U>& d) { *p++ = d; }
+        };
+        for (int i = b; i < e; ++i) {
+          const int q = live[i];
+          auto& j = jobs[q];
+          targets(q, tg);
+          if (tg.empty()) continue;
+          const int w = std::min(nb_, std::min(j.m, j.k) - jb);
+          const int rows = j.m - jb;
+          const U* Vb = j.P + jb + (int64_t)jb * j.ld;
+          const U* Tb = j.Tb + (int64_t)jb * nb_;
+          char* cur = wbase + nby[i];
+          auto alloc = [&](int64_t elems) {
+            U* p = (U*)cur;
+            cur += al(sizeof(U) * (size_t)std::max<int64_t>(elems, 1));
+            return p;
+          };
+          Out o1{g1.data() + n1[i]}, o3{g3.data() + n3[i]};
+          larfb_step_descs<U>(Vb, j.ld, Tb, nb_, rows, w, tg.data(), (int)tg.size(), alloc, o1, o3);
+        }
+      });
+      gemm_batch(g1.data(), n1[nl], "gemm_y");
+      gemm_batch(g3.data(), n3[nl], "gemm_c");
     }
   }
 
-  // tables of the larfb block steps and gemm_batch's tile counts, kept for
-  // the whole factorization: a level-0 step plans ~300k descriptors, and
-  // fresh vectors per step cost more host time than filling them
+  // tables of the larfb block steps and gemm_batch's tile counts, kept per
+  // host thread across block steps and trees: a level-0 step plans ~300k
+  // descriptors, and fresh memory costs more host time (page faults) than
+  // filling it.  The memory is held until the thread ends.
   template <typename U>
   struct LarfbBufs {
     std::vector<GemmDesc<U>> g1, g3;
     std::vector<LarfbTarget<U>> tg;
   };
-  LarfbBufs<double> larfb_d_;
-  LarfbBufs<float> larfb_f_;
   template <typename U>
-  LarfbBufs<U>& larfb_bufs() {
-    if constexpr (std::is_same<U, double>::value) return larfb_d_;
-    else return larfb_f_;
+  static LarfbBufs<U>& larfb_bufs() {
+    static thread_local LarfbBufs<U> b;
+    return b;
   }
-  std::vector<int> tile_cnt_, tile_st_;
+  struct TileBufs {
+    std::vector<int> cnt, st;
+  };
+  // copy tables of the elimination's gather and scatter (reused; only the
+  // leading entries a call fills are valid)
+  struct CopyBufs {
+    std::vector<CopyDesc> a, b;
+  };
+  static CopyBufs& copy_bufs() {
+    static thread_local CopyBufs b;
+    return b;
+  }
+  static TileBufs& tile_bufs() {
+    static thread_local TileBufs b;
+    return b;
+  }
 
   // ------------------------------------------------------------ phases
-  void get_sparsity(int c);
   // numeric row-sparsity tests of the non-geometric path (tree.cpp:855-858)
   struct SparsityTest { int c, nin, e, ein; };
   std::vector<SparsityTest> sparsity_tests_;
+  // per-thread state of get_sparsity: cluster marks (== stamp: in the set
+  // being built) and the numeric tests found, in cluster order
+  struct SparsityScratch {
+    std::vector<unsigned> mark;
+    unsigned stamp = 0;
+    std::vector<SparsityTest> tests;
+  };
+  void get_sparsity(int c, SparsityScratch& sc);
+  // the row sparsity of every cluster of the level, on the host pool
+  void fill_sparsity(const std::vector<int>& lev);
   void finish_sparsity();
   // fine_wave >= 0: the inline elimination of --scale 0 fine clusters of
   // that sparsify wave (ops recorded as FineQR with reflectors)
@@ -686,8 +773,6 @@ class Engine final : public EngineBase {
   bool qp3_log_ = false;     // SPAQR_QP3_LOG=1: per-launch RRQR lines (with kernel timing)
   bool reflr_ = true;        // SPAQR_REFL_REGISTER=0: solve reflector stages in the LDS form only
   // per-cluster scratch of the host bookkeeping (sized to cl_, reused)
-  std::vector<unsigned> mark_;  // == mark_stamp_: in the set being built
-  unsigned mark_stamp_ = 0;
   std::vector<int> pos_;        // panel position of a cluster, -1 when none
   int panel_cap_ = -1;
   int panel_r_cap() {
@@ -843,7 +928,7 @@ void Engine<T>::assemble(const CscMat& A) {  // tree.cpp:725-778
 }
 
 template <typename T>
-void Engine<T>::get_sparsity(int c) {  // tree.cpp:834-870 (+ check_if_valid 168-224)
+void Engine<T>::get_sparsity(int c, SparsityScratch& sc) {  // tree.cpp:834-870 (+ check_if_valid 168-224)
   std::vector<SepID> colsp;
   for (int e : cl_[c].out) colsp.push_back(cl_[ed_[e].n2].id.self);
   std::sort(colsp.begin(), colsp.end());
@@ -870,10 +955,11 @@ void Engine<T>::get_sparsity(int c) {  // tree.cpp:834-870 (+ check_if_valid 168
   // the distinct clusters found, marked as they are first accepted (a
   // cluster already in the set needs no further test), then sorted
   std::vector<int> rs;
-  if (mark_.size() < cl_.size()) mark_.resize(cl_.size(), 0);
-  const unsigned stamp = ++mark_stamp_;
+  std::vector<unsigned>& mark = sc.mark;
+  if (mark.size() < cl_.size()) mark.resize(cl_.size(), 0);
+  const unsigned stamp = ++sc.stamp;
   auto take = [&](int n) {
-    if (mark_[n] != stamp) { mark_[n] = stamp; rs.push_back(n); }
+    if (mark[n] != stamp) { mark[n] = stamp; rs.push_back(n); }
   };
   for (int e : cl_[c].out) {
     if (!ed_[e].alive) continue;
@@ -881,20 +967,32 @@ void Engine<T>::get_sparsity(int c) {  // tree.cpp:834-870 (+ check_if_valid 168
     take(n);
     for (int ein : cl_[n].in) {
       const int nin = ed_[ein].n1;
-      if (mark_[nin] == stamp) continue;
+      if (mark[nin] == stamp) continue;
       if (!ed_[ein].alive || nin == c || cl_[nin].elim || !valid(cl_[nin].id)) continue;
       // without coordinates, a cluster that does not name c as an interface
       // must also couple to c numerically through n: |A(n,c)^T A(n,nin)| >
       // 1e-14 (tree.cpp:855-858); decided for the whole level at once
       if (!geo_ && !(cl_[nin].id.l == cs || cl_[nin].id.r == cs)) {
-        sparsity_tests_.push_back(SparsityTest{c, nin, e, ein});
+        sc.tests.push_back(SparsityTest{c, nin, e, ein});
         continue;
       }
       take(nin);
     }
   }
   std::sort(rs.begin(), rs.end());
-  cl_[c].rsp = rs;
+  cl_[c].rsp = std::move(rs);
+}
+
+// Clusters of one level only read the tree and write their own row
+// sparsity; the numeric tests are joined in cluster order.
+template <typename T>
+void Engine<T>::fill_sparsity(const std::vector<int>& lev) {
+  const int n = (int)lev.size(), grain = 64;
+  std::vector<SparsityScratch> sc(std::max(plan_chunks(n, grain), 1));
+  parallel_for(n, grain, [&](int ch, int b, int e) {
+    for (int i = b; i < e; ++i) get_sparsity(lev[i], sc[ch]);
+  });
+  for (auto& x : sc) sparsity_tests_.insert(sparsity_tests_.end(), x.tests.begin(), x.tests.end());
 }
 
 // The numeric tests get_sparsity left for this level: P = A(n,c)^T A(n,nin)
@@ -949,7 +1047,7 @@ void Engine<T>::eliminate_level(const std::vector<int>& cs, int fine_wave) {
     int c, m, k, Nw;
     std::vector<int> pe;                  // panel edges
     std::vector<int> tg;                  // target clusters
-    std::vector<std::vector<int>> te;     // target edges aligned with pe
+    std::vector<int> te;                  // target edges: te[t * |pe| + s] for target t, panel block s
     std::vector<int> touched;             // sorted target edge ids
     T *P, *W, *tau, *Tb;
     double *P64 = nullptr, *W64 = nullptr, *tau64 = nullptr, *Tb64 = nullptr;  // FP64 top separator
@@ -974,9 +1072,14 @@ void Engine<T>::eliminate_level(const std::vector<int>& cs, int fine_wave) {
     // panel position), the missing ones are added in panel order
     if (pos_.size() < cl_.size()) pos_.resize(cl_.size(), -1);
     for (size_t q = 0; q < p.pe.size(); ++q) pos_[ed_[p.pe[q]].n2] = (int)q;
+    const size_t npe = p.pe.size();
+    p.te.reserve(cl_[c].rsp.size() * npe);
+    p.touched.reserve(cl_[c].rsp.size() * npe);
     for (int n : cl_[c].rsp) {
       if (cl_[n].elim || n == c) continue;
-      std::vector<int> row(p.pe.size(), -1);
+      const size_t base = p.te.size();
+      p.te.resize(base + npe, -1);
+      int* row = p.te.data() + base;  // p.te is not resized until this row is done
       {
         const Clu& cn = cl_[n];
         for (size_t q = 0; q < cn.outn.size(); ++q) {
@@ -984,12 +1087,11 @@ void Engine<T>::eliminate_level(const std::vector<int>& cs, int fine_wave) {
           if (kc < (int)pos_.size() && pos_[kc] >= 0 && row[pos_[kc]] < 0) row[pos_[kc]] = cn.out[q];
         }
       }
-      for (size_t q = 0; q < p.pe.size(); ++q) {
+      for (size_t q = 0; q < npe; ++q) {
         if (row[q] < 0) row[q] = add_edge(n, ed_[p.pe[q]].n2, &zero);
         p.touched.push_back(row[q]);
       }
       p.tg.push_back(n);
-      p.te.push_back(row);
       p.Nw += cl_[n].csize;
       nnzR_ += (unsigned long long)(cl_[c].csize * cl_[n].csize);
     }
@@ -1066,34 +1168,53 @@ void Engine<T>::eliminate_level(const std::vector<int>& cs, int fine_wave) {
       factor_bytes_ += sizeof(T) * ((double)p.m * p.k + (double)nb_ * iceil(p.k, nb_) * nb_);
     }
   }
+  std::vector<CopyDesc>& gather = copy_bufs().a;
+  std::vector<CopyDesc>& scatter = copy_bufs().b;
   for (int w = 0; w < nw; ++w) {
     scratch_.rewind();
-    std::vector<CopyDesc> gather, scatter;
     std::vector<QrJob> jobs;
+    // the wave's plans in creation order; each one's panel blocks and the
+    // target blocks of every panel row go to the gather table (panel block s,
+    // then the targets of row s), the target blocks alone to the scatter
+    // table, at offsets fixed here so the tables are written in parallel
+    std::vector<int> idx;
+    std::vector<size_t> goff(1, 0), soff(1, 0);
     for (size_t i = 0; i < plans.size(); ++i) {
       if (wave[i] != w) continue;
       Plan& p = plans[i];
       p.W = alloc_scr((size_t)p.m * std::max(p.Nw, 1));
-      int r = 0;
-      for (size_t s = 0; s < p.pe.size(); ++s) {
-        const Block& b = ed_[p.pe[s]].a;
-        gather.push_back(CopyDesc{b.p, p.P + r, b.ld, p.m, b.rows, b.cols, 0, 0});
-        int col = 0;
-        for (size_t t = 0; t < p.tg.size(); ++t) {
-          const Block& tb = ed_[p.te[t][s]].a;
-          CopyDesc d{tb.p, p.W + r + (int64_t)col * p.m, tb.ld, p.m, tb.rows, tb.cols, 0, 0};
-          gather.push_back(d);
-          scatter.push_back(CopyDesc{d.dst, (void*)tb.p, p.m, tb.ld, tb.rows, tb.cols, 0, 0});
-          col += cl_[p.tg[t]].csize;
-        }
-        r += b.rows;
-      }
+      idx.push_back((int)i);
+      goff.push_back(goff.back() + p.pe.size() * (1 + p.tg.size()));
+      soff.push_back(soff.back() + p.pe.size() * p.tg.size());
       QrJob j;
       j.P = p.P; j.ld = p.m; j.m = p.m; j.k = p.k; j.tau = p.tau; j.Tb = p.Tb;
       if (p.Nw > 0) j.extra.push_back(Block{p.W, p.m, p.Nw, p.m});
       jobs.push_back(std::move(j));
     }
-    copy_blocks(gather);
+    if (gather.size() < goff.back()) gather.resize(goff.back());
+    if (scatter.size() < soff.back()) scatter.resize(soff.back());
+    parallel_for((int)idx.size(), 8, [&](int, int qb, int qe) {
+      for (int q = qb; q < qe; ++q) {
+        const Plan& p = plans[idx[q]];
+        CopyDesc* g = gather.data() + goff[q];
+        CopyDesc* sc = scatter.data() + soff[q];
+        int r = 0;
+        for (size_t s = 0; s < p.pe.size(); ++s) {
+          const Block& b = ed_[p.pe[s]].a;
+          *g++ = CopyDesc{b.p, p.P + r, b.ld, p.m, b.rows, b.cols, 0, 0};
+          int col = 0;
+          for (size_t t = 0; t < p.tg.size(); ++t) {
+            const Block& tb = ed_[p.te[t * p.pe.size() + s]].a;
+            const CopyDesc d{tb.p, p.W + r + (int64_t)col * p.m, tb.ld, p.m, tb.rows, tb.cols, 0, 0};
+            *g++ = d;
+            *sc++ = CopyDesc{d.dst, (void*)tb.p, p.m, tb.ld, tb.rows, tb.cols, 0, 0};
+            col += cl_[p.tg[t]].csize;
+          }
+          r += b.rows;
+        }
+      }
+    });
+    copy_blocks(gather.data(), goff.back());
     if (top64) {
       std::vector<QrJobT<double>> j64;
       std::vector<CopyDesc> up, down;
@@ -1117,7 +1238,7 @@ void Engine<T>::eliminate_level(const std::vector<int>& cs, int fine_wave) {
     } else {
       blocked_qr(jobs);
     }
-    copy_blocks(scatter);
+    copy_blocks(scatter.data(), soff.back());
   }
   // ops and bookkeeping (tree.cpp:826-829)
   for (size_t i = 0; i < plans.size(); ++i) {
@@ -1144,25 +1265,38 @@ void Engine<T>::scale_level(const std::vector<int>& cs) {
   std::vector<QrJob> jobs;
   std::vector<Block> ident;
   struct TrsmJob { Block R; std::vector<Block> B; };
-  std::vector<TrsmJob> tj;
-  for (int c : cs) {
-    const int es = find_out(c, c);
+  // the blocks each interface's QR updates (in-edges) and its TRSM scales
+  // (out-edges), found on the host pool: no interface's lists include
+  // another interface's diagonal block, which is the only block replaced below
+  const int nc = (int)cs.size();
+  std::vector<int> eself(nc);
+  jobs.resize(nc);
+  std::vector<TrsmJob> tj(nc);
+  parallel_for(nc, 64, [&](int, int b, int e) {
+    for (int i = b; i < e; ++i) {
+      const int c = cs[i];
+      eself[i] = find_out(c, c);
+      for (int e2 : cl_[c].in)
+        if (!cl_[ed_[e2].n1].elim) jobs[i].extra.push_back(ed_[e2].a);
+      for (int e2 : cl_[c].out)
+        if (!cl_[ed_[e2].n2].elim && ed_[e2].n2 != c) tj[i].B.push_back(ed_[e2].a);
+    }
+  });
+  for (int i = 0; i < nc; ++i) {
+    const int c = cs[i];
+    const int es = eself[i];
     const Block q = ed_[es].a;  // becomes the ScaleD factor storage
     const int rr = q.rows, cc = cl_[c].csize, kk = std::min(rr, cc);
     Block nb = new_block(rr, cc);
     ident.push_back(nb);
     ed_[es].a = nb;
-    QrJob j;
+    QrJob& j = jobs[i];
     j.P = q.p; j.ld = q.ld; j.m = rr; j.k = cc;
     j.tau = alloc_perm(cc);
     j.Tb = alloc_perm((size_t)nb_ * iceil(cc, nb_) * nb_);
     factor_bytes_ += sizeof(T) * ((double)rr * cc + (double)nb_ * iceil(cc, nb_) * nb_);
-    for (int e : cl_[c].in)
-      if (!cl_[ed_[e].n1].elim) j.extra.push_back(ed_[e].a);
-    TrsmJob t;
+    TrsmJob& t = tj[i];
     t.R = q;
-    for (int e : cl_[c].out)
-      if (!cl_[ed_[e].n2].elim && ed_[e].n2 != c) t.B.push_back(ed_[e].a);
     nnzR_ = (unsigned long long)((double)nnzR_ + 0.5 * cc * cc);
     nnzH_ += (unsigned long long)(cl_[c].rsize * cl_[c].csize);
     flops_ += 4.0 * rr * cc * kk - 2.0 * (rr + cc) * kk * kk + 4.0 / 3.0 * kk * kk * kk + (double)rr * kk * kk -
@@ -1175,8 +1309,6 @@ void Engine<T>::scale_level(const std::vector<int>& cs) {
     o.segs.push_back({c, cl_[c].rsize});
     o.R = q.p; o.ldr = q.ld; o.kr = cc;
     ops_.push_back(std::move(o));
-    jobs.push_back(std::move(j));
-    tj.push_back(std::move(t));
   }
   info_[ilvl_].n_scale = (int)cs.size();
   init_blocks(ident, 1);
@@ -1539,64 +1671,83 @@ void Engine<T>::sparsify_level(const std::vector<int>& cs) {
       int rest = 0;
       T *Ct, *V, *tau;
     };
-    std::vector<Job> jobs;
     std::vector<CopyDesc> gather;
     std::vector<RrqrJob> rj;
     int kmax = 0;
     std::vector<int> wave_idx;
     for (size_t i = 0; i < S.size(); ++i)
       if (wave[i] == w) { wave_idx.push_back((int)i); kmax = std::max(kmax, cl_[S[i]].csize); }
-    for (int i : wave_idx) {
-      const int c = S[i];
-      Job jb;
-      jb.c = c; jb.k = cl_[c].csize; jb.smp = 0;
-      int spm = 0;
-      {
+    // per interface of the wave: its edge lists and sizes (read-only scans,
+    // on the host pool), then the scratch in creation order, then the
+    // gather tables at fixed offsets (on the pool)
+    const int nj = (int)wave_idx.size();
+    std::vector<Job> jobs(nj);
+    std::vector<size_t> goff(nj + 1, 0);
+    parallel_for(nj, 64, [&](int, int b, int e) {
+      for (int q = b; q < e; ++q) {
+        const int c = S[wave_idx[q]];
+        Job& jb = jobs[q];
+        jb.c = c; jb.k = cl_[c].csize; jb.smp = 0;
+        int spm = 0;
         const Clu& k = cl_[c];
-        for (size_t q = 0; q < k.in.size(); ++q)
-          if (!cl_[k.inn[q]].elim) { jb.apm.push_back(k.in[q]); spm += ed_[k.in[q]].a.cols; }
-        for (size_t q = 0; q < k.out.size(); ++q)
-          if (!cl_[k.outn[q]].elim && k.outn[q] != c) { jb.amp.push_back(k.out[q]); jb.smp += ed_[k.out[q]].a.rows; }
+        for (size_t t = 0; t < k.in.size(); ++t)
+          if (!cl_[k.inn[t]].elim) { jb.apm.push_back(k.in[t]); spm += ed_[k.in[t]].a.cols; }
+        for (size_t t = 0; t < k.out.size(); ++t)
+          if (!cl_[k.outn[t]].elim && k.outn[t] != c) { jb.amp.push_back(k.out[t]); jb.smp += ed_[k.out[t]].a.rows; }
+        jb.N = jb.smp + spm;
+        jb.spm = spm;
+        jb.rest = cl_[c].rsize - jb.k;
+        size_t ng = jb.amp.size() + jb.apm.size();
+        if (jb.rest > 0 && spm > 0) ng += jb.apm.size();
+        if (imp && spm > 0) ng += jb.apm.size();
+        goff[q + 1] = ng;
       }
-      jb.N = jb.smp + spm;
-      const int N = std::max(jb.N, 1);
-      jb.Ct = alloc_scr((size_t)N * jb.k);
-      int off = 0;
-      for (int e : jb.amp) {
-        const Block& b = ed_[e].a;
-        gather.push_back(CopyDesc{b.p, jb.Ct + off, b.ld, N, b.rows, jb.k, 0, 0});
-        off += b.rows;
-      }
-      for (int e : jb.apm) {
-        const Block& b = ed_[e].a;
-        gather.push_back(CopyDesc{b.p, jb.Ct + off, b.ld, N, jb.k, b.cols, 1, 0});
-        off += b.cols;
-      }
-      jb.spm = spm;
-      jb.rest = cl_[c].rsize - jb.k;
-      if (jb.rest > 0 && spm > 0) {
-        // Apmc_rest (tree.cpp:1016), moved up under the rank rows afterwards
-        jb.Rest = alloc_scr((size_t)jb.rest * spm);
-        int q = 0;
-        for (int e : jb.apm) {
-          const Block& b = ed_[e].a;
-          gather.push_back(CopyDesc{b.p + jb.k, jb.Rest + (int64_t)q * jb.rest, b.ld, jb.rest, jb.rest, b.cols, 0, 0});
-          q += b.cols;
-        }
-      }
-      if (imp && spm > 0) {
-        // Apmc_top (tree.cpp:1130-1140), kept before the blocks are rewritten
-        jb.W = alloc_scr((size_t)jb.k * spm);
-        int q = 0;
-        for (int e : jb.apm) {
-          const Block& b = ed_[e].a;
-          gather.push_back(CopyDesc{b.p, jb.W + (int64_t)q * jb.k, b.ld, jb.k, jb.k, b.cols, 0, 0});
-          q += b.cols;
-        }
-      }
+    });
+    for (int q = 0; q < nj; ++q) {
+      Job& jb = jobs[q];
+      goff[q + 1] += goff[q];
+      jb.Ct = alloc_scr((size_t)std::max(jb.N, 1) * jb.k);
+      // Apmc_rest (tree.cpp:1016), moved up under the rank rows afterwards
+      if (jb.rest > 0 && jb.spm > 0) jb.Rest = alloc_scr((size_t)jb.rest * jb.spm);
+      // Apmc_top (tree.cpp:1130-1140), kept before the blocks are rewritten
+      if (imp && jb.spm > 0) jb.W = alloc_scr((size_t)jb.k * jb.spm);
       rj.push_back(RrqrJob{jb.Ct, jb.k, jb.N});
-      jobs.push_back(std::move(jb));
     }
+    gather.resize(goff[nj]);
+    parallel_for(nj, 64, [&](int, int b, int e) {
+      for (int q = b; q < e; ++q) {
+        const Job& jb = jobs[q];
+        const int N = std::max(jb.N, 1);
+        CopyDesc* g = gather.data() + goff[q];
+        int off = 0;
+        for (int e2 : jb.amp) {
+          const Block& bl = ed_[e2].a;
+          *g++ = CopyDesc{bl.p, jb.Ct + off, bl.ld, N, bl.rows, jb.k, 0, 0};
+          off += bl.rows;
+        }
+        for (int e2 : jb.apm) {
+          const Block& bl = ed_[e2].a;
+          *g++ = CopyDesc{bl.p, jb.Ct + off, bl.ld, N, jb.k, bl.cols, 1, 0};
+          off += bl.cols;
+        }
+        if (jb.Rest) {
+          int c2 = 0;
+          for (int e2 : jb.apm) {
+            const Block& bl = ed_[e2].a;
+            *g++ = CopyDesc{bl.p + jb.k, jb.Rest + (int64_t)c2 * jb.rest, bl.ld, jb.rest, jb.rest, bl.cols, 0, 0};
+            c2 += bl.cols;
+          }
+        }
+        if (jb.W) {
+          int c2 = 0;
+          for (int e2 : jb.apm) {
+            const Block& bl = ed_[e2].a;
+            *g++ = CopyDesc{bl.p, jb.W + (int64_t)c2 * jb.k, bl.ld, jb.k, jb.k, bl.cols, 0, 0};
+            c2 += bl.cols;
+          }
+        }
+      }
+    });
     copy_blocks(gather);
     rrqr_batch(rj);
     std::vector<int> ranks(jobs.size(), 0);
@@ -2173,27 +2324,44 @@ void Engine<T>::merge_all() {  // tree.cpp:872-881, 414-475
   }
   std::vector<Block> zero;
   std::vector<CopyDesc> copies;
-  for (int snew : bottoms_[cur_]) {
-    std::vector<int> nb;
-    for (int s : cl_[snew].children)
-      for (int e : cl_[s].out)
-        if (!cl_[ed_[e].n2].elim) nb.push_back(cl_[ed_[e].n2].parent);
-    std::sort(nb.begin(), nb.end());
-    nb.erase(std::unique(nb.begin(), nb.end()), nb.end());
+  if (pos_.size() < cl_.size()) pos_.resize(cl_.size(), -1);
+  // every new cluster's neighbours (the parents of its children's live
+  // column neighbours): the children's edge lists do not change below, so
+  // these are found on the host pool first
+  const std::vector<int>& snews = bottoms_[cur_];
+  std::vector<std::vector<int>> nbs(snews.size());
+  parallel_for((int)snews.size(), 64, [&](int, int b, int e) {
+    for (int i = b; i < e; ++i) {
+      std::vector<int>& nb = nbs[i];
+      for (int s : cl_[snews[i]].children)
+        for (int e2 : cl_[s].out)
+          if (!cl_[ed_[e2].n2].elim) nb.push_back(cl_[ed_[e2].n2].parent);
+      std::sort(nb.begin(), nb.end());
+      nb.erase(std::unique(nb.begin(), nb.end()), nb.end());
+    }
+  });
+  for (size_t i = 0; i < snews.size(); ++i) {
+    const int snew = snews[i];
+    const std::vector<int>& nb = nbs[i];
     cl_[snew].cnbrs = nb;
     for (int n : nb) add_edge(snew, n, &zero);
     sort_out(snew);
+    // snew's out edge towards each cluster (find_out's first match), marked
+    // in pos_ for this cluster only
+    const Clu& P = cl_[snew];
+    for (size_t q = P.outn.size(); q-- > 0;) pos_[P.outn[q]] = P.out[q];
     for (int s : cl_[snew].children)
       for (int e : cl_[s].out) {
         const int nold = ed_[e].n2;
         if (cl_[nold].elim) continue;
-        const int f = find_out(snew, cl_[nold].parent);
+        const int f = pos_[cl_[nold].parent];
         Block& dst = ed_[f].a;
         const Block& src = ed_[e].a;
         copies.push_back(CopyDesc{src.p, dst.p + cl_[nold].rpos + (int64_t)cl_[s].cpos * dst.ld, src.ld, dst.ld,
                                   src.rows, src.cols, 0, 0});
         ed_[e].alive = false;
       }
+    for (int n2 : P.outn) pos_[n2] = -1;
   }
   init_blocks(zero, 0);
   copy_blocks(copies);
@@ -2214,7 +2382,7 @@ int Engine<T>::factorize() {  // tree.cpp:1309-1437
     std::vector<int> lev;
     for (int s : bottoms_[cur_])
       if (cl_[s].level() == ilvl_) lev.push_back(s);
-    for (int s : lev) get_sparsity(s);
+    fill_sparsity(lev);
     finish_sparsity();
     const double t1 = wall();
     eliminate_level(lev);
@@ -2269,10 +2437,6 @@ int Engine<T>::factorize() {  // tree.cpp:1309-1437
              ilvl_, li.t_fill, li.t_elmn, li.t_shift, li.t_scale, li.t_sprsfy, li.t_merge, ts.first, ts.second,
              (double)ts.first / (double)ts.second);
   }
-  larfb_d_ = {};
-  larfb_f_ = {};
-  tile_cnt_ = {};
-  tile_st_ = {};
   const double tbp = wall();
   build_solve_program();
   // the apply's graph is instantiated here, so solves (and the Krylov drivers
