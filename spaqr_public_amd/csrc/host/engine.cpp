@@ -272,11 +272,16 @@ class Engine final : public EngineBase {
     return -1;
   }
   void sort_out(int c) {
+    // stable sort by the column cluster's order: (order, position) keys,
+    // sorted without allocating for the short lists of the merge
     auto& o = cl_[c].out;
-    std::stable_sort(o.begin(), o.end(), [this](int a, int b) { return cl_[ed_[a].n2].order < cl_[ed_[b].n2].order; });
+    static thread_local std::vector<std::pair<std::pair<int, int>, int>> key;
+    key.resize(o.size());
+    for (size_t q = 0; q < o.size(); ++q) key[q] = {{cl_[ed_[o[q]].n2].order, (int)q}, o[q]};
+    std::sort(key.begin(), key.end(), [](const auto& a, const auto& b) { return a.first < b.first; });
     auto& on = cl_[c].outn;
     on.resize(o.size());
-    for (size_t q = 0; q < o.size(); ++q) on[q] = ed_[o[q]].n2;
+    for (size_t q = 0; q < o.size(); ++q) { o[q] = key[q].second; on[q] = ed_[o[q]].n2; }
   }
   std::pair<int, int> topsize() const {
     int r = 0, c = 0;
@@ -332,17 +337,35 @@ class Engine final : public EngineBase {
   // --------------------------------------------------- batched launch helpers
   // esrc / edst: element bytes of source and destination (a conversion when
   // they differ)
+  // cnt[i] = f(i, bytes) for i < n (cnt = tile_bufs().cnt, sized by the
+  // caller) on the host pool; returns the bytes f adds up, summed per chunk
+  // and then in chunk order
+  template <typename F>
+  double count_tiles(size_t n, F&& f) {
+    std::vector<int>& cnt = tile_bufs().cnt;
+    const int grain = 8192, nch = std::max(plan_chunks((int)n, grain), 1);
+    std::vector<double> part(nch, 0.0);
+    parallel_for((int)n, grain, [&](int ch, int b, int e) {
+      double by = 0;
+      for (int i = b; i < e; ++i) cnt[i] = f((size_t)i, by);
+      part[ch] = by;
+    });
+    double bytes = 0;
+    for (double v : part) bytes += v;
+    return bytes;
+  }
   void copy_blocks(const std::vector<CopyDesc>& ds, int esrc = (int)sizeof(T), int edst = (int)sizeof(T)) {
     copy_blocks(ds.data(), ds.size(), esrc, edst);
   }
   void copy_blocks(const CopyDesc* ds, size_t n, int esrc = (int)sizeof(T), int edst = (int)sizeof(T)) {
     if (n == 0) return;
-    std::vector<int> cnt(n), st;
-    double bytes = 0;
-    for (size_t i = 0; i < n; ++i) {
-      cnt[i] = iceil(std::max(ds[i].rows, 1), 32) * iceil(std::max(ds[i].cols, 1), 32);
-      bytes += (double)(esrc + edst) * ds[i].rows * ds[i].cols;
-    }
+    std::vector<int>& cnt = tile_bufs().cnt;
+    std::vector<int>& st = tile_bufs().st;
+    cnt.resize(n);
+    const double bytes = count_tiles(n, [&](size_t i, double& by) {
+      by += (double)(esrc + edst) * ds[i].rows * ds[i].cols;
+      return iceil(std::max(ds[i].rows, 1), 32) * iceil(std::max(ds[i].cols, 1), 32);
+    });
     const int nt = prefix_tiles(cnt, st);
     const CopyDesc* dd = (const CopyDesc*)stager_.push_raw(ds, sizeof(CopyDesc) * n);
     const int* dst = stager_.push(st);
@@ -351,19 +374,22 @@ class Engine final : public EngineBase {
   void init_blocks(const std::vector<Block>& bs, int identity) {
     if (bs.empty()) return;
     std::vector<CopyDesc> ds;
+    ds.reserve(bs.size());
     for (auto& b : bs) {
       if (b.rows == 0 || b.cols == 0) continue;
       CopyDesc d{};
-      d.dst = b.p; d.ldd = b.ld; d.rows = b.rows; d.cols = b.cols;
+      d.dst = b.p; d.ldd = b.ld; d.rows = b.rows; d.cols = b.cols; d.trans = identity;
       ds.push_back(d);
     }
     if (ds.empty()) return;
-    for (auto& d : ds) d.trans = identity;
-    std::vector<int> cnt(ds.size()), st;
-    for (size_t i = 0; i < ds.size(); ++i) cnt[i] = iceil(ds[i].rows, 32) * iceil(ds[i].cols, 32);
+    std::vector<int>& cnt = tile_bufs().cnt;
+    std::vector<int>& st = tile_bufs().st;
+    cnt.resize(ds.size());
+    const double bytes = count_tiles(ds.size(), [&](size_t i, double& by) {
+      by += (double)sizeof(T) * ds[i].rows * ds[i].cols;
+      return iceil(ds[i].rows, 32) * iceil(ds[i].cols, 32);
+    });
     const int nt = prefix_tiles(cnt, st);
-    double bytes = 0;
-    for (auto& d : ds) bytes += (double)sizeof(T) * d.rows * d.cols;
     const CopyDesc* dd = stager_.push(ds);
     const int* dst = stager_.push(st);
     timed("init", 0.0, bytes, [&] { launch_init_blocks(stream_, sizeof(T), dd, dst, (int)ds.size(), nt); });
@@ -1240,21 +1266,26 @@ void Engine<T>::eliminate_level(const std::vector<int>& cs, int fine_wave) {
     }
     copy_blocks(scatter.data(), soff.back());
   }
-  // ops and bookkeeping (tree.cpp:826-829)
-  for (size_t i = 0; i < plans.size(); ++i) {
-    Plan& p = plans[i];
-    Op o;
-    o.kind = fine_wave < 0 ? OpQR : OpFineQR; o.level = ilvl_; o.c = p.c;
-    o.wave = wave[i];
-    if (fine_wave >= 0) { o.wave = fine_wave; o.fwd_refl = true; }
-    o.V = p.P; o.ldv = p.m; o.m = p.m; o.k = p.k; o.Tb = p.Tb;
-    for (int e : p.pe) o.segs.push_back({ed_[e].n2, cl_[ed_[e].n2].rsize});
-    o.R = p.P; o.ldr = p.m; o.kr = p.k;
-    if (top64) { o.V64 = p.P64; o.Tb64 = p.Tb64; }
-    for (int e : cl_[p.c].in) o.r12.push_back({ed_[e].a, {ed_[e].n1, ed_[e].a.cols}});
-    ops_.push_back(std::move(o));
-    cl_[p.c].elim = true;
-  }
+  // ops and bookkeeping (tree.cpp:826-829): the op records are built on the
+  // host pool (each reads only its own cluster's lists), then appended in
+  // creation order
+  const size_t o0 = ops_.size();
+  ops_.resize(o0 + plans.size());
+  parallel_for((int)plans.size(), 64, [&](int, int b, int e) {
+    for (int i = b; i < e; ++i) {
+      const Plan& p = plans[i];
+      Op& o = ops_[o0 + i];
+      o.kind = fine_wave < 0 ? OpQR : OpFineQR; o.level = ilvl_; o.c = p.c;
+      o.wave = wave[i];
+      if (fine_wave >= 0) { o.wave = fine_wave; o.fwd_refl = true; }
+      o.V = p.P; o.ldv = p.m; o.m = p.m; o.k = p.k; o.Tb = p.Tb;
+      for (int e2 : p.pe) o.segs.push_back({ed_[e2].n2, cl_[ed_[e2].n2].rsize});
+      o.R = p.P; o.ldr = p.m; o.kr = p.k;
+      if (top64) { o.V64 = p.P64; o.Tb64 = p.Tb64; }
+      for (int e2 : cl_[p.c].in) o.r12.push_back({ed_[e2].a, {ed_[e2].n1, ed_[e2].a.cols}});
+    }
+  });
+  for (auto& p : plans) cl_[p.c].elim = true;
 }
 
 // tree.cpp:533-580 (diagScale) for every interface of the level: (i) QR of
@@ -2324,7 +2355,6 @@ void Engine<T>::merge_all() {  // tree.cpp:872-881, 414-475
   }
   std::vector<Block> zero;
   std::vector<CopyDesc> copies;
-  if (pos_.size() < cl_.size()) pos_.resize(cl_.size(), -1);
   // every new cluster's neighbours (the parents of its children's live
   // column neighbours): the children's edge lists do not change below, so
   // these are found on the host pool first
@@ -2340,29 +2370,69 @@ void Engine<T>::merge_all() {  // tree.cpp:872-881, 414-475
       nb.erase(std::unique(nb.begin(), nb.end()), nb.end());
     }
   });
+  // the new clusters' edge lists get their final lengths up front (out:
+  // one edge per neighbour, in: one per cluster naming it), so add_edge
+  // appends without reallocating
+  {
+    size_t tot = 0;
+    std::vector<int> indeg(cl_.size(), 0);
+    for (size_t i = 0; i < snews.size(); ++i) {
+      tot += nbs[i].size();
+      for (int n : nbs[i])
+        if (n != snews[i]) ++indeg[n];
+    }
+    ed_.reserve(ed_.size() + tot);
+    zero.reserve(tot);
+    for (size_t i = 0; i < snews.size(); ++i) {
+      Clu& k = cl_[snews[i]];
+      k.out.reserve(k.out.size() + nbs[i].size());
+      k.outn.reserve(k.outn.size() + nbs[i].size());
+      k.in.reserve(k.in.size() + indeg[snews[i]]);
+      k.inn.reserve(k.inn.size() + indeg[snews[i]]);
+    }
+  }
   for (size_t i = 0; i < snews.size(); ++i) {
     const int snew = snews[i];
     const std::vector<int>& nb = nbs[i];
     cl_[snew].cnbrs = nb;
     for (int n : nb) add_edge(snew, n, &zero);
     sort_out(snew);
-    // snew's out edge towards each cluster (find_out's first match), marked
-    // in pos_ for this cluster only
-    const Clu& P = cl_[snew];
-    for (size_t q = P.outn.size(); q-- > 0;) pos_[P.outn[q]] = P.out[q];
-    for (int s : cl_[snew].children)
-      for (int e : cl_[s].out) {
-        const int nold = ed_[e].n2;
-        if (cl_[nold].elim) continue;
-        const int f = pos_[cl_[nold].parent];
-        Block& dst = ed_[f].a;
-        const Block& src = ed_[e].a;
-        copies.push_back(CopyDesc{src.p, dst.p + cl_[nold].rpos + (int64_t)cl_[s].cpos * dst.ld, src.ld, dst.ld,
-                                  src.rows, src.cols, 0, 0});
-        ed_[e].alive = false;
-      }
-    for (int n2 : P.outn) pos_[n2] = -1;
   }
+  // every child block moves into its parent's block (rows at the child row
+  // cluster's offset, columns at the child's); per new cluster in order, on
+  // the host pool: the blocks of different new clusters are disjoint
+  const int ns = (int)snews.size();
+  std::vector<size_t> coff(ns + 1, 0);
+  parallel_for(ns, 64, [&](int, int b, int e) {
+    for (int i = b; i < e; ++i) {
+      size_t n = 0;
+      for (int s : cl_[snews[i]].children)
+        for (int e2 : cl_[s].out) n += cl_[ed_[e2].n2].elim ? 0 : 1;
+      coff[i + 1] = n;
+    }
+  });
+  for (int i = 0; i < ns; ++i) coff[i + 1] += coff[i];
+  copies.resize(coff[ns]);
+  parallel_for(ns, 64, [&](int, int b, int e) {
+    // snew's out edge towards each cluster (find_out's first match)
+    std::vector<int> pos(cl_.size(), -1);
+    for (int i = b; i < e; ++i) {
+      const Clu& P = cl_[snews[i]];
+      for (size_t q = P.outn.size(); q-- > 0;) pos[P.outn[q]] = P.out[q];
+      CopyDesc* d = copies.data() + coff[i];
+      for (int s : P.children)
+        for (int e2 : cl_[s].out) {
+          const int nold = ed_[e2].n2;
+          if (cl_[nold].elim) continue;
+          const Block& dst = ed_[pos[cl_[nold].parent]].a;
+          const Block& src = ed_[e2].a;
+          *d++ = CopyDesc{src.p, dst.p + cl_[nold].rpos + (int64_t)cl_[s].cpos * dst.ld, src.ld, dst.ld, src.rows,
+                          src.cols, 0, 0};
+          ed_[e2].alive = false;
+        }
+      for (int n2 : P.outn) pos[n2] = -1;
+    }
+  });
   init_blocks(zero, 0);
   copy_blocks(copies);
 }
