@@ -10,6 +10,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <mutex>
 #include <numeric>
@@ -337,6 +338,10 @@ class Engine final : public EngineBase {
   // --------------------------------------------------- batched launch helpers
   // esrc / edst: element bytes of source and destination (a conversion when
   // they differ)
+  // Launches planned ahead: tables are pushed to the staging ring when a
+  // launch is planned, and the launches of a phase are issued together
+  // afterwards, so their tables reach the device in one copy.
+  using Run = std::vector<std::function<void()>>;
   // cnt[i] = f(i, bytes) for i < n (cnt = tile_bufs().cnt, sized by the
   // caller) on the host pool; returns the bytes f adds up, summed per chunk
   // and then in chunk order
@@ -358,7 +363,11 @@ class Engine final : public EngineBase {
     copy_blocks(ds.data(), ds.size(), esrc, edst);
   }
   void copy_blocks(const CopyDesc* ds, size_t n, int esrc = (int)sizeof(T), int edst = (int)sizeof(T)) {
-    if (n == 0) return;
+    if (auto f = copy_plan(ds, n, esrc, edst)) f();
+  }
+  std::function<void()> copy_plan(const CopyDesc* ds, size_t n, int esrc = (int)sizeof(T),
+                                  int edst = (int)sizeof(T)) {
+    if (n == 0) return {};
     std::vector<int>& cnt = tile_bufs().cnt;
     std::vector<int>& st = tile_bufs().st;
     cnt.resize(n);
@@ -369,7 +378,10 @@ class Engine final : public EngineBase {
     const int nt = prefix_tiles(cnt, st);
     const CopyDesc* dd = (const CopyDesc*)stager_.push_raw(ds, sizeof(CopyDesc) * n);
     const int* dst = stager_.push(st);
-    timed("copy", 0.0, bytes, [&] { launch_copy(stream_, esrc, edst, dd, dst, (int)n, nt); });
+    const int nd = (int)n;
+    return [this, bytes, esrc, edst, dd, dst, nd, nt] {
+      timed("copy", 0.0, bytes, [&] { launch_copy(stream_, esrc, edst, dd, dst, nd, nt); });
+    };
   }
   void init_blocks(const std::vector<Block>& bs, int identity) {
     if (bs.empty()) return;
@@ -404,7 +416,35 @@ class Engine final : public EngineBase {
   }
   template <typename U>
   void gemm_batch(GemmDesc<U>* ds, size_t n, const char* fam = "gemm") {
-    if (n == 0) return;
+    gemm_launch(gemm_plan(ds, n, fam));
+  }
+  // A batch's tables pushed to the staging ring, ready to launch (plan and
+  // launch apart let a caller push the tables of several launches before the
+  // first of them, so they reach the device in one copy)
+  template <typename U>
+  struct GemmLaunch {
+    const char* fam = nullptr;
+    double fl = 0, by = 0;
+    const GemmDesc<U>* dd = nullptr;
+    const int* dst = nullptr;
+    int n = 0, nt = 0;
+    const GemmDesc<U>* drd = nullptr;
+    const int* drst = nullptr;
+    int rn = 0, rnt = 0;
+  };
+  template <typename U>
+  void gemm_launch(const GemmLaunch<U>& L) {
+    if (L.n == 0) return;
+    timed(L.fam, L.fl, L.by, [&] {
+      launch_gemm<U>(stream_, L.dd, L.dst, L.n, L.nt);
+      if (L.rn) launch_gemm_reduce<U>(stream_, L.drd, L.drst, L.rn, L.rnt);
+    });
+  }
+  template <typename U>
+  GemmLaunch<U> gemm_plan(GemmDesc<U>* ds, size_t n, const char* fam) {
+    GemmLaunch<U> L;
+    L.fam = fam;
+    if (n == 0) return L;
     std::vector<int>& cnt = tile_bufs().cnt;
     cnt.resize(n);
     // tile counts and the ledger's flops / bytes, in chunks joined in order
@@ -459,12 +499,15 @@ class Engine final : public EngineBase {
       for (size_t i = 0; i < n; ++i)
         if (ds[i].ksplit > 1) { rd.push_back(ds[i]); rcnt.push_back(iceil(ds[i].m * ds[i].n, 256)); }
     const int rnt = prefix_tiles(rcnt, rst);
-    const GemmDesc<U>* drd = rd.empty() ? nullptr : stager_.push(rd);
-    const int* drst = rd.empty() ? nullptr : stager_.push(rst);
-    timed(fam, fl, by, [&] {
-      launch_gemm<U>(stream_, dd, dst, (int)n, nt);
-      if (!rd.empty()) launch_gemm_reduce<U>(stream_, drd, drst, (int)rd.size(), rnt);
-    });
+    L.fl = fl; L.by = by;
+    L.dd = dd; L.dst = dst; L.n = (int)n; L.nt = nt;
+    if (!rd.empty()) {
+      L.drd = stager_.push(rd);
+      L.drst = stager_.push(rst);
+      L.rn = (int)rd.size();
+      L.rnt = rnt;
+    }
+    return L;
   }
 
   template <typename U>
@@ -492,6 +535,16 @@ class Engine final : public EngineBase {
   // separator of an FP32 tree)
   template <typename U>
   void blocked_qr(std::vector<QrJobT<U>>& jobs) {
+    Run run;
+    blocked_qr_plan(jobs, run);
+    for (auto& f : run) f();
+  }
+  // Pushes every block step's tables and appends the steps' launches to
+  // `run` (to be issued in order, after any launches already there).  Step
+  // jb + 1 reuses step jb's scratch in ws_ (the stream orders it after step
+  // jb's kernels).
+  template <typename U>
+  void blocked_qr_plan(std::vector<QrJobT<U>>& jobs, Run& run) {
     int kmax = 0;
     for (auto& j : jobs) kmax = std::max(kmax, std::min(j.m, j.k));
     // panel steps run in the register form: rows [jb, m) over nwg
@@ -503,7 +556,7 @@ class Engine final : public EngineBase {
       for (auto& j : jobs)
         if (jb < std::min(j.m, j.k) && iceil(j.m - jb, rpw) > 1) words += barrier_words(iceil(j.m - jb, rpw));
     unsigned int* pool = words ? (unsigned int*)scratch_.alloc(sizeof(unsigned int) * words) : nullptr;
-    if (words) launch_zero(stream_, 4, pool, words);
+    if (words) run.push_back([this, pool, words] { launch_zero(stream_, 4, pool, words); });
     size_t pool_used = 0;
     const int cap = std::is_same<U, T>::value ? panel_r_cap() : panel_r_capacity<U>();
     for (int jb = 0; jb < kmax; jb += nb_) {
@@ -552,10 +605,13 @@ class Engine final : public EngineBase {
         const int nt = prefix_tiles(cnt, st);
         const PanelDesc<U>* dpd = stager_.push(L.first);
         const int* dst = stager_.push(st);
-        int rc = 0;
-        timed("panel", li == 0 ? pfl : 0.0, li == 0 ? pby : 0.0,
-              [&] { rc = launch_panel_r<U>(stream_, dpd, dst, (int)L.first.size(), nt, jb, L.second); });
-        if (rc != 0) throw Error(kErrHip, std::string("panel launch failed: ") + hipGetErrorString((hipError_t)rc));
+        const double f = li == 0 ? pfl : 0.0, by = li == 0 ? pby : 0.0;
+        const int nd = (int)L.first.size(), coop = L.second;
+        run.push_back([this, f, by, dpd, dst, nd, nt, jb, coop] {
+          int rc = 0;
+          timed("panel", f, by, [&] { rc = launch_panel_r<U>(stream_, dpd, dst, nd, nt, jb, coop); });
+          if (rc != 0) throw Error(kErrHip, std::string("panel launch failed: ") + hipGetErrorString((hipError_t)rc));
+        });
       }
       // trailing matrices: C -= (V_b T_b^T)(V_b^T C).  W = V_b T_b^T is formed
       // once per problem in the same launch as the Y = V_b^T C of its
@@ -622,8 +678,12 @@ class Engine final : public EngineBase {
           larfb_step_descs<U>(Vb, j.ld, Tb, nb_, rows, w, tg.data(), (int)tg.size(), alloc, o1, o3);
         }
       });
-      gemm_batch(g1.data(), n1[nl], "gemm_y");
-      gemm_batch(g3.data(), n3[nl], "gemm_c");
+      const GemmLaunch<U> l1 = gemm_plan(g1.data(), n1[nl], "gemm_y");
+      const GemmLaunch<U> l3 = gemm_plan(g3.data(), n3[nl], "gemm_c");
+      run.push_back([this, l1, l3] {
+        gemm_launch(l1);
+        gemm_launch(l3);
+      });
     }
   }
 
@@ -1240,8 +1300,11 @@ void Engine<T>::eliminate_level(const std::vector<int>& cs, int fine_wave) {
         }
       }
     });
-    copy_blocks(gather.data(), goff.back());
+    Run run;
+    if (auto f = copy_plan(gather.data(), goff.back())) run.push_back(std::move(f));
     if (top64) {
+      for (auto& f : run) f();
+      run.clear();
       std::vector<QrJobT<double>> j64;
       std::vector<CopyDesc> up, down;
       for (size_t i = 0; i < plans.size(); ++i) {
@@ -1262,9 +1325,10 @@ void Engine<T>::eliminate_level(const std::vector<int>& cs, int fine_wave) {
       blocked_qr(j64);
       copy_blocks(down, 8, 4);
     } else {
-      blocked_qr(jobs);
+      blocked_qr_plan(jobs, run);
     }
-    copy_blocks(scatter.data(), soff.back());
+    if (auto f = copy_plan(scatter.data(), soff.back())) run.push_back(std::move(f));
+    for (auto& f : run) f();
   }
   // ops and bookkeeping (tree.cpp:826-829): the op records are built on the
   // host pool (each reads only its own cluster's lists), then appended in
@@ -1344,7 +1408,8 @@ void Engine<T>::scale_level(const std::vector<int>& cs) {
   info_[ilvl_].n_scale = (int)cs.size();
   init_blocks(ident, 1);
   scratch_.rewind();
-  blocked_qr(jobs);
+  Run run;  // the QR's steps, then the TRSM's column blocks
+  blocked_qr_plan(jobs, run);
   // (ii) B <- B R^{-1}, column block by column block
   int kmax = 0;
   for (auto& t : tj) kmax = std::max(kmax, t.R.cols);
@@ -1364,7 +1429,8 @@ void Engine<T>::scale_level(const std::vector<int>& cs) {
         cnt.push_back(iceil(b.rows, trsm_rows_per_tile()));
       }
     }
-    gemm_batch(g, "gemm_trsm");
+    const GemmLaunch<T> lg = gemm_plan(g.data(), g.size(), "gemm_trsm");
+    run.push_back([this, lg] { gemm_launch(lg); });
     if (!td.empty()) {
       std::vector<int> st;
       const int nt = prefix_tiles(cnt, st);
@@ -1372,9 +1438,13 @@ void Engine<T>::scale_level(const std::vector<int>& cs) {
       for (auto& d : td) tfl += (double)d.rows * std::min(nb_, d.k - jb) * std::min(nb_, d.k - jb);
       const TrsmDesc<T>* dtd = stager_.push(td);
       const int* dst = stager_.push(st);
-      timed("trsm", tfl, 0.0, [&] { launch_trsm_block<T>(stream_, dtd, dst, (int)td.size(), nt, jb); });
+      const int nd = (int)td.size();
+      run.push_back([this, tfl, dtd, dst, nd, nt, jb] {
+        timed("trsm", tfl, 0.0, [&] { launch_trsm_block<T>(stream_, dtd, dst, nd, nt, jb); });
+      });
     }
   }
+  for (auto& f : run) f();
 }
 
 // Truncated column-pivoted QR (the geqp3 + choose_rank of util.cpp:466-477,
