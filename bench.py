@@ -159,8 +159,45 @@ def trace_summary(d, out, cmd, top=14):
         name = r["Name"].split("(")[0].replace("void ", "")
         lines.append(f"| `{name}` | {r['Calls']} | {float(r['TotalDurationNs']) / 1e6:.1f} | "
                      f"{float(r['AverageNs']) / 1e3:.1f} | {float(r['Percentage']):.1f} |")
+    lines += timeline_summary(d)
     with open(out, "w") as f:
         f.write("\n".join(lines) + "\n")
+
+
+def timeline_summary(d, split_ns=30e6):
+    """GPU timeline of the last step in a kernel trace: the dispatches are cut
+    into runs at gaps longer than split_ns (the host work between steps), and
+    the last run (the timed factorization and its GMRES) is reported as span,
+    busy time and idle time, with the idle split by gap length."""
+    try:
+        rows = _csv_rows(d, "kernel_trace.csv")
+    except SystemExit:
+        return []
+    ev = sorted((int(r["Start_Timestamp"]), int(r["End_Timestamp"])) for r in rows)
+    runs, cur, end = [], [], None
+    for a, b in ev:
+        if cur and a - end > split_ns:
+            runs.append(cur)
+            cur = []
+        cur.append((a, b))
+        end = b if end is None or not cur[:-1] else max(end, b)
+    runs.append(cur)
+    last = runs[-1]
+    t0, t1 = last[0][0], max(b for _, b in last)
+    busy, end, gaps = 0, t0, []
+    for a, b in last:
+        if a > end:
+            gaps.append(a - end)
+        busy += max(0, b - max(a, end))
+        end = max(end, b)
+    big = [g for g in gaps if g >= 1e6]
+    small = [g for g in gaps if g < 20e3]
+    mid = [g for g in gaps if 20e3 <= g < 1e6]
+    return ["", f"Timeline of the timed step (factorization + GMRES, {len(last)} dispatches): span "
+            f"{(t1 - t0) / 1e6:.1f} ms, GPU busy {busy / 1e6:.1f} ms, idle {(t1 - t0 - busy) / 1e6:.1f} ms: "
+            f"{len(big)} gaps of 1 ms or more ({sum(big) / 1e6:.1f} ms, the host planning a phase), "
+            f"{len(mid)} of 20 us to 1 ms ({sum(mid) / 1e6:.1f} ms), {len(small)} under 20 us "
+            f"({sum(small) / 1e6:.1f} ms, dispatch to dispatch)."]
 
 
 def stage_summary(err_path, out, cmd):

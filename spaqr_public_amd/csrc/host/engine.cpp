@@ -540,9 +540,9 @@ class Engine final : public EngineBase {
     for (auto& f : run) f();
   }
   // Pushes every block step's tables and appends the steps' launches to
-  // `run` (to be issued in order, after any launches already there).  Step
-  // jb + 1 reuses step jb's scratch in ws_ (the stream orders it after step
-  // jb's kernels).
+  // `run` (to be issued in order, after any launches already there; a large
+  // step issues `run` itself).  Step jb + 1 reuses step jb's scratch in ws_
+  // (the stream orders it after step jb's kernels).
   template <typename U>
   void blocked_qr_plan(std::vector<QrJobT<U>>& jobs, Run& run) {
     int kmax = 0;
@@ -684,6 +684,13 @@ class Engine final : public EngineBase {
         gemm_launch(l1);
         gemm_launch(l3);
       });
+      // a step whose planning costs more host time than an upload (the
+      // batches of many small problems) is launched at once, so the next
+      // step is planned while it runs
+      if (n1[nl] + n3[nl] >= 4096) {
+        for (auto& f : run) f();
+        run.clear();
+      }
     }
   }
 
