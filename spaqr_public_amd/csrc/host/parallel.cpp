@@ -25,7 +25,7 @@ int host_threads() {
 int plan_chunks(int n, int grain) {
   if (n <= 0) return 0;
   const int by_grain = std::max(1, n / std::max(grain, 1));
-  return std::min(by_grain, 4 * host_threads());
+  return std::min(by_grain, kMaxChunks);
 }
 
 namespace {

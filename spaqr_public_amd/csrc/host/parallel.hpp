@@ -13,7 +13,9 @@ namespace spq {
 int host_threads();
 
 // Number of chunks parallel_for splits [0, n) into with at least `grain`
-// items per chunk (0 when n <= 0).  Chunk c covers [c*n/k, (c+1)*n/k).
+// items per chunk, at most kMaxChunks (0 when n <= 0); it does not depend on
+// the thread count.  Chunk c covers [c*n/k, (c+1)*n/k).
+constexpr int kMaxChunks = 32;
 int plan_chunks(int n, int grain);
 
 // Calls f(chunk, begin, end) for every chunk of [0, n), on the pool's
