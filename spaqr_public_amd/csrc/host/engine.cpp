@@ -2555,7 +2555,10 @@ int Engine<T>::factorize() {  // tree.cpp:1309-1437
     if (timing_) sync();
     const double t5 = wall();
     if (ilvl_ < nlevels_ - 1) merge_all();
-    sync();
+    // without phase synchronisation the next level's host planning (and,
+    // after the last level, the solve program's) overlaps this level's last
+    // kernels: the stream keeps the device work in order
+    if (timing_) sync();
     const double t6 = wall();
     li.t_fill = t1 - t0; li.t_elmn = t2 - t1; li.t_shift = t3 - t2; li.t_scale = t4 - t3;
     li.t_sprsfy = t5 - t4; li.t_merge = t6 - t5;
