@@ -133,7 +133,11 @@ __global__ __launch_bounds__(kThreads) void k_panel_r(const PanelDesc<T>* __rest
       tcol_from_dots(Ts, jj, tau, gsh);
       if (tid == 0) d.tau[j] = tau;
     }
-    // apply H_j to the later columns of this row; column j becomes v_j / beta
+    // apply H_j to the later columns of this row; column j becomes v_j / beta.
+    // No barrier closes the step: the next step rewrites red after this
+    // step's readers passed the barrier above, and rewrites rowsh, Dsh, wsh
+    // and gsh only after its own first barrier, which every thread reaches
+    // after its reads here.
     if (has && r >= j) {
       const T v = (r == j) ? T(1) : ((ssq > T(0)) ? x * scal : x);
 #pragma unroll
@@ -142,8 +146,8 @@ __global__ __launch_bounds__(kThreads) void k_panel_r(const PanelDesc<T>* __rest
         else if (c == jj) a[c] = (r == j) ? beta : v;
       }
     }
-    __syncthreads();
   }
+  __syncthreads();  // the last step's T column is in Ts
   if (has) {
 #pragma unroll
     for (int c = 0; c < kNB; ++c)
